@@ -1,0 +1,192 @@
+/*
+ * svo.h -- C ABI of the MI355X (gfx950) SVO raycast + image-warp frame
+ * pipeline (libsvo.so).
+ *
+ * The reference drives its per-frame kernels through an OpenCL host shim
+ * (src/ocl.h) from src/raycast.h.  This header is the drop-in replacement for
+ * that boundary: plain C, plain pointers and sizes, one context per GPU,
+ * status codes instead of error_stop()/exit().
+ *
+ * Conventions
+ *   - every call returns SVO_OK (0) or a negative SVO_ERR_*; the message is
+ *     in svo_last_error() (per thread).
+ *   - the library owns device memory; the caller owns host memory.
+ *   - one context = one device + one HIP stream; calls on a context are not
+ *     reentrant; launches are asynchronous unless the entry point says it
+ *     blocks (svo_read, svo_sync, svo_hole_count).
+ *   - buffer layouts are the reference's (src/raycast.h:79-85,:169-173,
+ *     :267-270): screen = 4 planes x N uint32 words, coords = 4 planes x N
+ *     float4, ids = [S counts | S offsets | H packed x|(y<<16)], with
+ *     N = res_x*res_y, S = (res_x/16)*(res_y/16), plane stride N.
+ */
+#ifndef SVO_H
+#define SVO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVO_OK 0
+#define SVO_ERR_ARG (-1)    /* bad argument / size out of range        */
+#define SVO_ERR_HIP (-2)    /* HIP runtime error                       */
+#define SVO_ERR_STATE (-3)  /* call order (no octree / no resize)       */
+#define SVO_ERR_NOMEM (-4)  /* allocation failed                       */
+#define SVO_ERR_IO (-5)     /* file I/O                                */
+
+#define SVO_HOLE 0xffffff00u /* empty screen word (kernel.cl:267)       */
+
+typedef struct svo_ctx svo_ctx;
+
+/* Per-frame arguments of the reference kernels (kernel.cl:479-482,
+ * :609-614, :858-863; host side raycast.h:159-162, :322-326).  row* are the
+ * vectors raycast_proj receives (rows of the camera matrix), col* the ones the
+ * ray kernels receive (after m.transpose()).  Only .xyz are used. */
+typedef struct svo_frame_params {
+    float m0[4];
+    float row0[4], row1[4], row2[4];
+    float col0[4], col1[4], col2[4];
+    float fovx, fovy;
+    int32_t frame;
+    int32_t reserved;
+} svo_frame_params;
+
+enum svo_buffer {
+    SVO_BUF_SCREEN = 0, /* 4 planes x N uint32                     */
+    SVO_BUF_COORDS = 1, /* 4 planes x N float4                     */
+    SVO_BUF_IDS = 2,    /* 2S + N uint32                           */
+    SVO_BUF_IMAGE = 3,  /* N uint32, b + g<<8 + r<<16 (PBO format) */
+    SVO_BUF_OCTREE = 4  /* n_words uint32                          */
+};
+
+/* ---- errors ---------------------------------------------------------- */
+/* CL_CHECK / error_stop (ocl.h:29-41, error.h:2-11): message of the last
+ * failing call on this thread. */
+const char* svo_last_error(void);
+
+/* ---- context ---------------------------------------------------------- */
+/* ocl_init (ocl.h:57-146): pick the device, create the queue. */
+int svo_create(svo_ctx** out, int device);
+/* ocl_exit (ocl.h:168-174) + raycast_exit (raycast.h:511-516). */
+int svo_destroy(svo_ctx* ctx);
+
+/* ocl_malloc(octree, COPY_HOST_PTR) (raycast.h:68, ocl.h:200-213): copy the
+ * compact word array (octree.h:232-293 layout) and its root ref
+ * (octree_root_normal) to the device.  depth = OCTREE_DEPTH (8..13). */
+int svo_upload_octree(svo_ctx* ctx, const uint32_t* words, size_t n_words,
+                      uint32_t root_ref, int depth);
+
+/* raycast_init buffer allocation (raycast.h:77-85, :168-172, :267-270) for
+ * one resolution; all planes start zeroed. */
+int svo_resize(svo_ctx* ctx, int res_x, int res_y);
+
+/* ---- whole frames ------------------------------------------------------ */
+/* raycast_draw (raycast.h:147-438): memset, proj planes 1+2, hole compaction,
+ * hole rays, 8x4 tile refresh (schedule from frame), copy plane 0 -> 2,
+ * fillhole2, colorize.  No device->host readback inside the frame. */
+int svo_frame_warp(svo_ctx* ctx, const svo_frame_params* p);
+/* raycast_fine_2 over the whole screen with add = 0, then colorize. */
+int svo_frame_full(svo_ctx* ctx, const svo_frame_params* p);
+
+/* ---- per-stage entry points (reference kernel names) ------------------- */
+/* ocl_memset(mem_screenbuffer, byte_ofs, val, bytes) (ocl.h:299-309) */
+int svo_memset(svo_ctx* ctx, size_t byte_ofs, uint32_t val, size_t bytes);
+/* raycast_proj with ofs_add = plane*N (kernel.cl:472-592, raycast.h:176-198);
+ * plane in 1..3.  Deterministic: the first source (row-major) with the
+ * smallest depth key wins a target pixel. */
+int svo_proj(svo_ctx* ctx, int plane, const svo_frame_params* p);
+/* raycast_counthole (kernel.cl:234-274) */
+int svo_counthole(svo_ctx* ctx);
+/* raycast_sumids (kernel.cl:276-296); the total stays on the device in
+ * ids[0] -- read it with svo_hole_count() if the host needs it. */
+int svo_sumids(svo_ctx* ctx);
+/* raycast_writeids (kernel.cl:298-340) */
+int svo_writeids(svo_ctx* ctx);
+/* raycast_holes (kernel.cl:594-694): one ray per id; the count is read on
+ * the device from ids[0] (no idbuf_size argument needed). */
+int svo_raycast_holes(svo_ctx* ctx, const svo_frame_params* p);
+/* raycast_fine_2 (kernel.cl:846-942): rays for pixels (add_x+i, add_y+j),
+ * i < ext_x, j < ext_y, clipped to the screen. */
+int svo_raycast_rect(svo_ctx* ctx, const svo_frame_params* p, int add_x,
+                     int add_y, int ext_x, int ext_y);
+/* ocl_memcpy plane 0 -> dst_plane of screen and coords (raycast.h:394-405) */
+int svo_copy_cache(svo_ctx* ctx, int dst_plane);
+/* raycast_fillhole2 (kernel.cl:404-470), defined on a pre-filter snapshot of
+ * plane 0 (deterministic; the reference filters in place). */
+int svo_fillhole(svo_ctx* ctx);
+/* raycast_colorize (kernel.cl:944-974) into SVO_BUF_IMAGE */
+int svo_colorize(svo_ctx* ctx);
+
+/* ---- transfers ------------------------------------------------------- */
+/* ocl_copy_to_host (ocl.h:215-221): blocking read of `bytes` at byte_ofs. */
+int svo_read(svo_ctx* ctx, int buf, size_t byte_ofs, size_t bytes, void* dst);
+/* blocking write (host -> device) into a frame buffer */
+int svo_write(svo_ctx* ctx, int buf, size_t byte_ofs, size_t bytes, const void* src);
+/* ocl_end_all_kernels (ocl.h:253-265): wait for all queued work. */
+int svo_sync(svo_ctx* ctx);
+/* blocking read of ids[0] (raycast.h:298) */
+int svo_hole_count(svo_ctx* ctx, uint32_t* holes);
+
+/* ---- host-side helpers (no device work) ------------------------------ */
+/* Camera vectors of raycast.h:113-162/:322-326 for yaw/pitch/roll (radians)
+ * and a world position; pos is wrapped like raycast.h:136-145.
+ * pos_wrapped may be NULL. */
+int svo_camera(float yaw, float pitch, float roll, const float pos_world[3],
+               int depth, int frame, svo_frame_params* out, float pos_wrapped[3]);
+/* 8x4 tile schedule (raycast.h:363-366) and the rounded-up extents of the
+ * raycast_fine_2 launch (ocl.h:188-198,:234-235). */
+int svo_tile_schedule(int res_x, int res_y, int frame, int* add_x, int* add_y,
+                      int* ext_x, int* ext_y);
+
+/* set_voxel x n then convert_tree_blocks (octree.h:32-293, raycast.h:13-39):
+ * voxels in stream order (xyz = 3 x uint16 each, rgba = uchar4 packed x in the
+ * low byte); emits the reference word array.  *words is malloc'ed; free it
+ * with svo_free. */
+int svo_build_octree(const uint16_t* xyz, const uint32_t* rgba, size_t n,
+                     int depth, uint32_t** words, size_t* n_words,
+                     uint32_t* root_ref);
+void svo_free(void* p);
+
+/* Save / load a built octree (word array + root ref + depth) as a small
+ * binary file, so large scenes are built once. */
+int svo_octree_save(const char* path, const uint32_t* words, size_t n_words,
+                    uint32_t root_ref, int depth);
+int svo_octree_load(const char* path, uint32_t** words, size_t* n_words,
+                    uint32_t* root_ref, int* depth);
+
+/* Seeded synthetic scenes (voxel streams for svo_build_octree).
+ *   kind 1: value-noise heightfield shell (2 octaves)
+ *   kind 2: 4-octave fractal terrain shell + n_spheres sphere shells
+ *   kind 3: 3-D fractal-noise cave, boundary voxels only
+ * *xyz / *rgba are malloc'ed (svo_free). */
+int svo_scene_generate(int kind, uint64_t seed, int depth, int n_spheres,
+                       uint16_t** xyz, uint32_t** rgba, size_t* n);
+
+/* Seeded Catmull-Rom flythrough: 6 floats per frame
+ * (yaw, pitch, roll, pos_x, pos_y, pos_z), world units (1 voxel = 1/8).
+ * max_voxels_per_frame / max_yaw_deg_per_frame bound the motion. */
+int svo_camera_path(uint64_t seed, int depth, int n_frames,
+                    float max_voxels_per_frame, float max_yaw_deg_per_frame,
+                    float height_voxels, float pitch, float* out);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Per-stage GPU timing with HIP events (off by default).  After a frame,
+ * svo_stage_times fills up to n floats (ms) in stage order and returns the
+ * number of stages written; names via svo_stage_name. */
+int svo_enable_timing(svo_ctx* ctx, int enable);
+/* Rays traced by the ray kernels since the last reset (blocking). */
+int svo_ray_count(svo_ctx* ctx, uint64_t* rays, int reset);
+int svo_stage_times(svo_ctx* ctx, float* ms, int n);
+const char* svo_stage_name(int stage);
+
+/* Ray-kernel tuning (autotuner knobs): work-queue chunk (rays per refill
+ * threshold), workgroups per CU for the persistent launch, iterations between
+ * refills.  0 keeps the current value. */
+int svo_set_tuning(svo_ctx* ctx, int refill_threshold, int wg_per_cu, int iters_per_refill);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SVO_H */

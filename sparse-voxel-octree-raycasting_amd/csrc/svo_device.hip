@@ -1,0 +1,1069 @@
+// svo_device.hip -- gfx950 kernels and the C-ABI runtime of libsvo.
+//
+// Kernels (all hand-written for CDNA wave64):
+//   k_rays          persistent ray traversal: every wave pulls rays from a
+//                   device work queue (hole list then refresh tile), refills
+//                   idle lanes with a ballot-ranked atomicAdd, keeps its
+//                   per-ray node stack in LDS (one column per lane);
+//                   restates kernel.cl:32-214 (fetchChildOffset, fetchColor,
+//                   CAST_RAY) with the same float op order.
+//   k_proj_scatter  warp of cache planes (kernel.cl:472-592): one 64-bit
+//                   atomicMin per source of key = depth24|source index|colour,
+//                   so the first source with the smallest depth key wins.
+//   k_proj_resolve  writes winners' screen word and coords, clears the key,
+//                   fills the rest of plane 0 with the hole word (fuses the
+//                   per-frame ocl_memset of plane 0).
+//   k_counthole /
+//   k_writeids      one wave per 16x16 block, one lane per 2x2 quad, ballot
+//                   of the hole quads (kernel.cl:234-274, :298-340).
+//   k_sumids        single-workgroup scan, total stays on the device in
+//                   ids[0] (kernel.cl:276-296; no host readback).
+//   k_copy_plane    plane 0 -> plane k (screen + coords), 16-byte accesses.
+//   k_fill_color    fillhole2 on a pre-filter snapshot fused with colorize
+//                   (kernel.cl:404-470, :944-974).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "svo_internal.h"
+
+#define HOLE 0xffffff00u
+
+namespace {
+
+constexpr int RAY_BLOCK = 256;
+constexpr int STACK_SLOTS = 13;  // slots 1..D-1 for D <= 13
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t bc8(uint32_t v) { return (uint32_t)__popc(v & 0xffu); }
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return (ax * bx + ay * by) + az * bz;
+}
+
+__device__ __forceinline__ float len3(float x, float y, float z)
+{
+    return sqrtf((x * x + y * y) + z * z);
+}
+
+// OpenCL sign(): +-1 for nonzero values, the zero itself for +-0
+__device__ __forceinline__ float cl_sign(float v)
+{
+    return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : v);
+}
+
+// kernel.cl:32-62
+__device__ __forceinline__ uint32_t fetch_child(const uint32_t* __restrict__ oct, uint32_t a_node,
+                                                uint32_t a_node_before, uint32_t& local_root,
+                                                uint32_t a_child, uint32_t child_test, int rek)
+{
+    uint32_t n = a_node >> 9;
+    uint32_t nadd = a_child;
+    if (a_node & 256u) {
+        if (!(a_node_before & 256u)) {
+            local_root = n << 6;
+            n = 0;
+        }
+        n += local_root;
+        nadd = bc8(a_node & ((child_test << 1) - 1));
+        if (rek == 2) {
+            uint32_t w = oct[n + (nadd >> 2)];
+            return ((w >> ((nadd & 3) << 3)) & 255u) + 256u + (nadd << 9);
+        }
+    }
+    return oct[n + nadd];
+}
+
+// kernel.cl:64-112
+__device__ uint32_t fetch_color(const uint32_t* __restrict__ oct, uint32_t a_node, uint32_t a_node_before,
+                                uint32_t a_node_before2, uint32_t local_root, int rek, uint32_t child_test)
+{
+    uint32_t n = a_node >> 9;
+    uint32_t nadd = 8;
+    if (rek == 1) {
+        n = (a_node_before >> 9) & 15u;
+        uint32_t ofs = (a_node_before2 >> 9) + local_root;
+        nadd = bc8(a_node_before2 & 255u);
+        for (uint32_t i = 1; i < n; i++) nadd += bc8((oct[ofs + (i >> 2)] >> ((i & 3) << 3)) & 255u);
+        nadd += bc8(a_node_before & ((child_test << 1) - 1));
+        return oct[ofs + (nadd >> 2)] >> ((nadd & 3) << 3);
+    }
+    if (a_node & 256u) {
+        if (!(a_node_before & 256u)) {
+            local_root = n << 6;
+            n = 0;
+        }
+        nadd = local_root;
+        if (rek == 2) {
+            uint32_t ofs = (a_node_before >> 9) + local_root;
+            nadd = 1 + bc8(a_node_before & 255u);
+            for (uint32_t i = 1; i < n; i++) nadd += bc8((oct[ofs + (i >> 2)] >> ((i & 3) << 3)) & 255u);
+            return oct[ofs + (nadd >> 2)] >> ((nadd & 3) << 3);
+        }
+    }
+    return oct[n + nadd];
+}
+
+struct RayArgs {
+    const uint32_t* oct;
+    uint32_t root;
+    int depth;
+    uint32_t* screen;  // plane 0
+    float* coords;     // plane 0, float4 per pixel
+    uint32_t* image;   // optional: colorized output (full-frame mode)
+    const uint32_t* ids;
+    uint32_t S;
+    int use_holes;
+    int add_x, add_y, ext_x, ext_y, tiles_x;
+    uint32_t n_rect;
+    int res_x, res_y;
+    float m0x, m0y, m0z;
+    float cx0, cx1, cx2, cy0, cy1, cy2, cz0, cz1, cz2;
+    float fovx, fovy;
+    uint32_t* queue;
+    int slot;
+    int refill;
+    int iters;
+    unsigned long long* ray_count;  // rays traced, accumulated across launches
+};
+
+struct Ray {
+    float px, py, pz;
+    float g0y, g0z, g1x, g1z, g2x, g2y;
+    float lx, ly, lz;
+    float dist, lodsw;
+    int ix, iy, iz;
+    int rek, lod, sxyz;
+    uint32_t node, before, lroot, test;
+    uint32_t pix;
+    uint32_t guard;
+};
+
+__device__ __forceinline__ uint32_t colorize_word(uint32_t word)
+{
+    // kernel.cl:959-971
+    const int a = (int)word;
+    const int c = a & 3;
+    const float i = (float)(a & (255 - 7));
+    float tr = 1.0f, tg = 1.0f, tb = 1.0f;
+    if (c == 1) { tr = 1.0f; tg = 0.7f; tb = 0.3f; }
+    else if (c == 2) { tr = 1.5f; tg = 0.8f; tb = 0.1f; }
+    else if (c == 3) { tr = 0.2f; tg = 0.8f; tb = 0.2f; }
+    int r = (int)(i * tr); if (r > 255) r = 255;
+    int g = (int)(i * tg); if (g > 255) g = 255;
+    int b = (int)(i * tb); if (b > 255) b = 255;
+    return (uint32_t)(b + g * 256 + r * 65536);
+}
+
+// Ray set-up: kernel.cl:883-908 and CAST_RAY prologue :115-150.
+__device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, uint32_t H, const RayArgs& a)
+{
+    int x, y;
+    if (k < H) {
+        uint32_t v = a.ids[2u * a.S + k];
+        x = (int)(v & 0xffffu);
+        y = (int)(v >> 16);
+    } else {
+        k -= H;
+        uint32_t t = k >> 6, w = k & 63u;
+        int tx = (int)(t % (uint32_t)a.tiles_x), ty = (int)(t / (uint32_t)a.tiles_x);
+        int lx = tx * 8 + (int)(w & 7u), ly = ty * 8 + (int)(w >> 3);
+        if (lx >= a.ext_x || ly >= a.ext_y) return false;
+        x = a.add_x + lx;
+        y = a.add_y + ly;
+    }
+    if (x >= a.res_x || y >= a.res_y) return false;
+    r.pix = (uint32_t)(y * a.res_x + x);
+
+    const int D = a.depth;
+    const float SCALE_MAX = (float)(1 << (D + 1));
+    float d1x = ((float)(x - a.res_x / 2) + 0.5f) * a.fovx / (float)a.res_y;
+    float d1y = ((float)(y - a.res_y / 2) + 0.5f) * a.fovy / (float)a.res_y;
+    float dx = dot3(d1x, d1y, 1.0f, a.cx0, a.cx1, a.cx2);
+    float dy = dot3(d1x, d1y, 1.0f, a.cy0, a.cy1, a.cy2);
+    float dz = dot3(d1x, d1y, 1.0f, a.cz0, a.cz1, a.cz2);
+    float px = a.m0x * 16.0f, py = a.m0y * 16.0f, pz = a.m0z * 16.0f;
+    const float eps = 0x1p-20f;
+    if (fabsf(dx) < eps) dx = cl_sign(dx) * eps;
+    if (fabsf(dy) < eps) dy = cl_sign(dy) * eps;
+    if (fabsf(dz) < eps) dz = cl_sign(dz) * eps;
+    int s = 0;
+    if (dx < 0) { s |= 1; px = SCALE_MAX - px; dx = -dx; }
+    if (dy < 0) { s |= 2; py = SCALE_MAX - py; dy = -dy; }
+    if (dz < 0) { s |= 4; pz = SCALE_MAX - pz; dz = -dz; }
+    r.g0y = dy / dx; r.g0z = dz / dx;
+    r.g1x = dx / dy; r.g1z = dz / dy;
+    r.g2x = dx / dz; r.g2y = dy / dz;
+    r.lx = len3(1.0f, r.g0y, r.g0z);
+    r.ly = len3(r.g1x, 1.0f, r.g1z);
+    r.lz = len3(r.g2x, r.g2y, 1.0f);
+    r.px = px; r.py = py; r.pz = pz;
+    r.ix = (int)px; r.iy = (int)py; r.iz = (int)pz;
+    r.sxyz = s;
+    r.rek = D;
+    r.lod = 1;
+    r.dist = 0.0f;
+    r.lodsw = (float)(a.res_x * 1 * 2);
+    r.node = a.root;
+    r.before = a.root;
+    r.lroot = 0;
+    r.test = 0;
+    r.guard = 0;
+    return true;
+}
+
+// One trip of the CAST_RAY do-while loop (kernel.cl:152-211).  Returns true
+// when the loop exits.
+__device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, uint32_t (*stk)[RAY_BLOCK], int tid)
+{
+    const int D = a.depth;
+    const int DAND = (1 << D) - 1;
+    if (++r.guard > SVO_MAX_RAY_ITERS) return true;
+    const int p0x = r.ix >> r.rek, p0y = r.iy >> r.rek, p0z = r.iz >> r.rek;
+    const int ni = ((p0x & 1) | ((p0y & 1) << 1) | ((p0z & 1) << 2)) ^ r.sxyz;
+    const uint32_t test = r.node & (1u << ni);
+    r.test = test;
+    if (test) {
+        // descend (kernel.cl:160-179).  At rek == 1 the fetched word is never
+        // consumed (the hit exits and fetchColor reads only node_before), so
+        // the load is skipped.
+        uint32_t child = r.node;
+        if (r.rek >= 2) child = fetch_child(a.oct, r.node, r.before, r.lroot, (uint32_t)ni, test, r.rek);
+        r.before = r.node;
+        r.node = child;
+        if (r.rek <= r.lod) return true;
+        r.rek--;
+        stk[r.rek][tid] = child;
+        return false;
+    }
+    // DDA step to the nearest cell face (kernel.cl:181-194)
+    const float mx = (float)(int)((uint32_t)(p0x + 1) << r.rek) - r.px;
+    const float my = (float)(int)((uint32_t)(p0y + 1) << r.rek) - r.py;
+    const float mz = (float)(int)((uint32_t)(p0z + 1) << r.rek) - r.pz;
+    float dist = mx * r.lx;
+    const float ddy = my * r.ly, ddz = mz * r.lz;
+    float sx = mx, sy = r.g0y * mx, sz = r.g0z * mx;
+    if (ddy < dist) { dist = ddy; sx = r.g1x * my; sy = my; sz = r.g1z * my; }
+    if (ddz < dist) { dist = ddz; sx = r.g2x * mz; sy = r.g2y * mz; sz = mz; }
+    r.px += sx; r.py += sy; r.pz += sz;
+    const int nx = (int)r.px, ny = (int)r.py, nz = (int)r.pz;
+    r.dist += dist;
+    // pop level (kernel.cl:197-208)
+    const int x0ry = r.iy ^ ny;
+    const int x0r = (r.ix ^ nx) ^ x0ry ^ (r.iz ^ nz);
+    r.ix = nx; r.iy = ny; r.iz = nz;
+    const uint32_t m = (uint32_t)(x0r & DAND);
+    const int rek_new = m ? 31 - __clz((int)m) : -1;
+    if (r.dist > 400000.0f) return true;
+    if (rek_new >= r.rek) {
+        r.rek = rek_new + 1;
+        if (r.rek >= D) {
+            r.node = a.root;
+            r.before = a.root;
+        } else {
+            r.node = stk[r.rek][tid];
+            // slot D belongs to no node of this ray; the reference reads a
+            // neighbour's slot there, a value consulted only for block
+            // nodes, and depth-1 nodes are normal nodes for D >= 8
+            r.before = (r.rek + 1 <= D - 1) ? stk[r.rek + 1][tid] : a.root;
+        }
+        if (r.dist > r.lodsw) { r.lodsw *= 2.0f; ++r.lod; }
+    }
+    return (x0ry & (2 * DAND + 2)) != 0;
+}
+
+__device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, uint32_t (*stk)[RAY_BLOCK], int tid)
+{
+    const float SCALE_MAX = (float)(1 << (a.depth + 1));
+    float px = r.px, py = r.py, pz = r.pz;
+    if (r.sxyz & 1) px = SCALE_MAX - px;
+    if (r.sxyz & 2) py = SCALE_MAX - py;
+    if (r.sxyz & 4) pz = SCALE_MAX - pz;
+    uint32_t b2 = (r.rek == 1) ? stk[2][tid] : 0u;
+    uint32_t col = fetch_color(a.oct, r.node, r.before, b2, r.lroot, r.rek, r.test);
+    uint32_t word = 0xff000000u + col;
+    a.screen[r.pix] = word;
+    float* c = a.coords + 4u * r.pix;
+    c[0] = px / 16.0f;
+    c[1] = py / 16.0f;
+    c[2] = pz / 16.0f;
+    if (a.image) a.image[r.pix] = colorize_word(word);
+}
+
+__global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
+{
+    __shared__ uint32_t stk[STACK_SLOTS][RAY_BLOCK];
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
+    const uint32_t H = a.use_holes ? min(a.ids[0], N) : 0u;  // the id list holds at most N entries
+    const uint32_t total = H + a.n_rect;
+    if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
+
+    Ray r;
+    bool active = false, exhausted = false;
+    uint32_t traced = 0;  // lane 0 of each wave: rays started by the wave
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        const int n_idle = __popcll(idle);
+        if (!exhausted && (n_idle >= a.refill || n_idle == 64)) {
+            const int leader = __ffsll((long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&a.queue[a.slot], (uint32_t)n_idle);
+            base = __shfl(base, leader);
+            if (base + (uint32_t)n_idle >= total) exhausted = true;
+            if (!active) {
+                const int rank = __popcll(idle & ((1ull << lane) - 1ull));
+                const uint32_t k = base + (uint32_t)rank;
+                if (k < total) active = init_ray(r, k, H, a);
+            }
+            traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)(64 - n_idle);
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) {
+            for (int it = 0; it < a.iters; ++it) {
+                if (step_ray(r, a, stk, tid)) {
+                    finish_ray(r, a, stk, tid);
+                    active = false;
+                    break;
+                }
+            }
+        }
+    }
+    if (lane == 0 && traced) atomicAdd(a.ray_count, (unsigned long long)traced);
+}
+
+// ---------------------------------------------------------------- warp
+__global__ __launch_bounds__(256) void k_memset(uint32_t* __restrict__ dst, uint32_t val, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n; i += stride) dst[i] = val;
+}
+
+struct ProjArgs {
+    uint32_t* screen;
+    const float4* coords;
+    unsigned long long* keys;
+    int res_x, res_y;
+    int plane_lo;
+    float m0x, m0y, m0z;
+    float r00, r01, r02, r10, r11, r12, r20, r21, r22;
+};
+
+// kernel.cl:498-572 per source pixel; blockIdx.y selects the source plane.
+__global__ __launch_bounds__(256) void k_proj_scatter(ProjArgs a)
+{
+    const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= N) return;
+    const uint32_t src = (uint32_t)(a.plane_lo + (int)blockIdx.y) * N + pix;
+    const uint32_t col = a.screen[src];
+    if (col == HOLE) return;
+    const float4 c = a.coords[src];
+    const float qx = c.x - a.m0x, qy = c.y - a.m0y, qz = c.z - a.m0z;
+    const float hx = dot3(qx, qy, qz, a.r00, a.r01, a.r02);
+    const float hy = dot3(qx, qy, qz, a.r10, a.r11, a.r12);
+    const float hz = dot3(qx, qy, qz, a.r20, a.r21, a.r22);
+    if (hz < 0.05f) { a.screen[src] = HOLE; return; }
+    const float sfx = (hx * (float)a.res_y + 0.0f) / hz + (float)a.res_x / 2 - 0.0f;
+    const float sfy = (hy * (float)a.res_y + 0.0f) / hz + (float)a.res_y / 2 - 0.0f;
+    const int sxi = (int)sfx, syi = (int)sfy;
+    if (sxi >= a.res_x - 1 || sxi < 0 || syi >= a.res_y - 1 || syi < 0) { a.screen[src] = HOLE; return; }
+    const uint32_t t = (uint32_t)syi * (uint32_t)a.res_x + (uint32_t)sxi;
+    const uint32_t depth = (uint32_t)(int)(hz * 1000.0f) & 0xffffffu;
+    const unsigned long long key = ((unsigned long long)depth << 40) |
+                                   ((unsigned long long)src << 8) | (unsigned long long)(col & 255u);
+    if (a.keys[t] > key) atomicMin(&a.keys[t], key);
+}
+
+// Resolve: winner words/coords into plane 0 (kernel.cl:571-585 semantics
+// against the plane-0 word w0), key reset; clear=1 also writes the hole word
+// to every pixel without a winner (the per-frame memset of plane 0).
+__global__ __launch_bounds__(256) void k_proj_resolve(ProjArgs a, int clear)
+{
+    const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    const unsigned long long key = a.keys[i];
+    if (key != ~0ull) {
+        a.keys[i] = ~0ull;
+        const uint32_t w0 = clear ? HOLE : a.screen[i];
+        const uint32_t depth = (uint32_t)(key >> 40);
+        if (depth < (w0 >> 8)) {
+            const uint32_t src = (uint32_t)(key >> 8);
+            const float4 c = a.coords[src];
+            const float qx = c.x - a.m0x, qy = c.y - a.m0y, qz = c.z - a.m0z;
+            const float hz = dot3(qx, qy, qz, a.r20, a.r21, a.r22);
+            a.screen[i] = (depth << 8) + (uint32_t)(key & 255u);
+            ((float4*)a.coords)[i] = make_float4(c.x, c.y, c.z, hz);
+            return;
+        }
+    }
+    if (clear) a.screen[i] = HOLE;
+}
+
+// ---------------------------------------------------------------- compaction
+// one wave per 16x16 block, lane q = 2x2 quad (q&7, q>>3): kernel.cl:262-272
+__device__ __forceinline__ bool quad_is_hole(const uint32_t* __restrict__ s, int res_x, int x, int y)
+{
+    const uint32_t o = (uint32_t)y * (uint32_t)res_x + (uint32_t)x;
+    int h = (s[o] == HOLE) + (s[o + 1] == HOLE) + (s[o + 1 + res_x] == HOLE) + (s[o + res_x] == HOLE);
+    return h >= 4;
+}
+
+__global__ __launch_bounds__(64) void k_counthole(const uint32_t* __restrict__ screen, uint32_t* ids, int res_x, int res_y)
+{
+    const int bxn = res_x / 16;
+    const int b = blockIdx.x;
+    const int x = (b % bxn) * 16 + 2 * (threadIdx.x & 7), y = (b / bxn) * 16 + 2 * (threadIdx.x >> 3);
+    const unsigned long long m = __ballot(quad_is_hole(screen, res_x, x, y));
+    if (threadIdx.x == 0) ids[b] = 4u * (uint32_t)__popcll(m);
+}
+
+__global__ __launch_bounds__(64) void k_writeids(const uint32_t* __restrict__ screen, uint32_t* ids, int res_x, int res_y)
+{
+    const int bxn = res_x / 16;
+    const uint32_t S = (uint32_t)bxn * (uint32_t)(res_y / 16);
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int x = (b % bxn) * 16 + 2 * (lane & 7), y = (b / bxn) * 16 + 2 * (lane >> 3);
+    const bool hole = quad_is_hole(screen, res_x, x, y);
+    const unsigned long long m = __ballot(hole);
+    if (!hole) return;
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const uint32_t dst = 2u * S + ids[S + b] + 4u * rank;
+    const uint32_t val = (uint32_t)x + ((uint32_t)y << 16);
+    ids[dst + 0] = val;
+    ids[dst + 1] = val + 1u;
+    ids[dst + 2] = val + 1u + (1u << 16);
+    ids[dst + 3] = val + (1u << 16);
+}
+
+// exclusive scan of ids[0..S) into ids[S..2S), total -> ids[0]
+__global__ __launch_bounds__(1024) void k_sumids(uint32_t* ids, uint32_t S)
+{
+    __shared__ uint32_t wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < S; base += 1024) {
+        const uint32_t i = base + (uint32_t)tid;
+        const uint32_t v = (i < S) ? ids[i] : 0u;
+        uint32_t incl = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t t = __shfl_up(incl, d);
+            if (lane >= d) incl += t;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        if (w == 0) {
+            uint32_t s = (lane < 16) ? wsum[lane] : 0u;
+            for (int d = 1; d < 16; d <<= 1) {
+                uint32_t t = __shfl_up(s, d);
+                if (lane >= d) s += t;
+            }
+            if (lane < 16) wsum[lane] = s;  // inclusive wave totals
+        }
+        __syncthreads();
+        const uint32_t wpre = (w > 0) ? wsum[w - 1] : 0u;
+        if (i < S) ids[S + i] = carry + wpre + incl - v;
+        carry += wsum[15];
+        __syncthreads();
+    }
+    if (tid == 0) ids[0] = carry;
+}
+
+// ---------------------------------------------------------------- copy / filter / colorize
+__global__ __launch_bounds__(256) void k_copy_plane(uint32_t* screen, float4* coords, uint32_t N, uint32_t dst_plane)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    const size_t d = (size_t)dst_plane * N + i;
+    screen[d] = screen[i];
+    coords[d] = coords[i];
+}
+
+__device__ __forceinline__ uint32_t rs(const uint32_t* __restrict__ s, long o, long limit)
+{
+    return o < limit ? s[o] : HOLE;
+}
+
+// kernel.cl:423-469 for one hole pixel, neighbours from the snapshot
+__device__ uint32_t fill_pixel(const uint32_t* __restrict__ src, long ofs, int res_x, long limit)
+{
+    uint32_t col = HOLE;
+    const uint32_t c1 = rs(src, ofs + 1, limit), c2 = rs(src, ofs - 1, limit);
+    const uint32_t c3 = rs(src, ofs + res_x, limit), c4 = rs(src, ofs - res_x, limit);
+    if (c1 != HOLE && c2 != HOLE && c3 != HOLE && c4 != HOLE)
+        return (c1 & 3u) + ((((c1 & 0xfcu) + (c2 & 0xfcu) + (c3 & 0xfcu) + (c4 & 0xfcu)) >> 2) & 0xfcu);
+    if (c1 != HOLE && c2 != HOLE) return (c1 & 3u) + ((((c1 & 0xfcu) + (c2 & 0xfcu)) >> 1) & 0xfcu);
+    if (c3 != HOLE && c4 != HOLE) return (c3 & 3u) + ((((c3 & 0xfcu) + (c4 & 0xfcu)) >> 1) & 0xfcu);
+    // 2x2: (0,0) is the hole itself, then (1,0), (0,1), (1,1)
+    col = rs(src, ofs + 1, limit);
+    if (col == HOLE) col = rs(src, ofs + res_x, limit);
+    if (col == HOLE) col = rs(src, ofs + 1 + res_x, limit);
+    if (col == HOLE) {
+        // 5x5, x offset outer, y offset inner; first non-hole wins
+        for (int i = -2; i < 3 && col == HOLE; i++)
+            for (int j = -2; j < 3 && col == HOLE; j++) col = rs(src, ofs + i + (long)j * res_x, limit);
+    }
+    return col;
+}
+
+// snapshot -> dst (plane 0) for holes inside the filter window; optional
+// colorize of the filtered word.
+__global__ __launch_bounds__(256) void k_fill_color(const uint32_t* __restrict__ snap, uint32_t* dst,
+                                                    uint32_t* image, int res_x, int res_y, int do_fill)
+{
+    const uint32_t N = (uint32_t)res_x * (uint32_t)res_y;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    const int x = (int)(i % (uint32_t)res_x), y = (int)(i / (uint32_t)res_x);
+    uint32_t v = snap[i];
+    if (do_fill && v == HOLE && x >= 2 && y >= 2 && x < res_x - 1 && y < res_y - 1) {
+        v = fill_pixel(snap, (long)i, res_x, (long)N);
+        dst[i] = v;
+    }
+    if (image) image[i] = colorize_word(v);
+}
+
+}  // namespace
+
+// ====================================================================== runtime
+#define HIPCHECK(expr)                                                                    \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return svo_fail(SVO_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+enum Stage { ST_CLEAR, ST_SCATTER, ST_RESOLVE, ST_COUNT, ST_SCAN, ST_WRITE, ST_RAYS, ST_COPY, ST_FILTER, ST_N };
+static const char* kStageNames[ST_N] = {"clear", "proj_scatter", "proj_resolve", "counthole", "sumids",
+                                        "writeids",  "rays",        "copy",         "fill_colorize"};
+
+struct svo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    // octree
+    uint32_t* d_oct = nullptr;
+    size_t n_words = 0;
+    uint32_t root = 0;
+    int depth = 0;
+    // frame buffers
+    int res_x = 0, res_y = 0;
+    size_t N = 0, S = 0;
+    uint32_t* d_screen = nullptr;
+    float4* d_coords = nullptr;
+    uint32_t* d_ids = nullptr;
+    uint32_t* d_image = nullptr;
+    unsigned long long* d_keys = nullptr;
+    uint32_t* d_snap = nullptr;
+    uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [8..9] ray count (u64)
+    int launch_seq = 0;
+    bool plane1_holes = false;
+    // tuning
+    int refill = 16, wg_per_cu = 8, iters = 16;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[ST_N + 1] = {};
+    float stage_ms[ST_N] = {};
+    bool stage_used[ST_N] = {};
+};
+
+static int free_frame(svo_ctx* c)
+{
+    hipFree(c->d_screen);
+    hipFree(c->d_coords);
+    hipFree(c->d_ids);
+    hipFree(c->d_image);
+    hipFree(c->d_keys);
+    hipFree(c->d_snap);
+    c->d_screen = nullptr;
+    c->d_coords = nullptr;
+    c->d_ids = nullptr;
+    c->d_image = nullptr;
+    c->d_keys = nullptr;
+    c->d_snap = nullptr;
+    c->res_x = c->res_y = 0;
+    c->N = c->S = 0;
+    return SVO_OK;
+}
+
+extern "C" int svo_create(svo_ctx** out, int device)
+{
+    if (!out) return svo_fail(SVO_ERR_ARG, "null context pointer");
+    *out = nullptr;
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return svo_fail(SVO_ERR_ARG, "device %d not present (%d devices)", device, n);
+    HIPCHECK(hipSetDevice(device));
+    svo_ctx* c = new svo_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->d_queue, 64) != hipSuccess || hipMemset(c->d_queue, 0, 64) != hipSuccess) {
+        delete c;
+        return svo_fail(SVO_ERR_HIP, "context set-up failed on device %d", device);
+    }
+    for (int i = 0; i <= ST_N; i++) hipEventCreate(&c->ev[i]);
+    *out = c;
+    return SVO_OK;
+}
+
+extern "C" int svo_destroy(svo_ctx* c)
+{
+    if (!c) return SVO_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    free_frame(c);
+    hipFree(c->d_oct);
+    hipFree(c->d_queue);
+    for (int i = 0; i <= ST_N; i++)
+        if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return SVO_OK;
+}
+
+// Pathological fetchColor loops (kernel.cl:104-106, reached only by rays that
+// stop at a byte node without a hit) may read up to 2^21 words past a valid
+// offset; the device copy carries that many zero words after the array.
+static const size_t kOctPad = (1u << 21) + 256;
+
+extern "C" int svo_upload_octree(svo_ctx* c, const uint32_t* words, size_t n_words, uint32_t root_ref, int depth)
+{
+    if (!c || !words) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (depth < 8 || depth > 13) return svo_fail(SVO_ERR_ARG, "depth %d outside 8..13", depth);
+    if (n_words < 10 || n_words > (1ull << 29)) return svo_fail(SVO_ERR_ARG, "word count %zu out of range", n_words);
+    if ((root_ref & 256u) || (size_t)(root_ref >> 9) + 10 > n_words)
+        return svo_fail(SVO_ERR_ARG, "root ref 0x%x is not a normal node inside the array", root_ref);
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    hipFree(c->d_oct);
+    c->d_oct = nullptr;
+    HIPCHECK(hipMalloc(&c->d_oct, (n_words + kOctPad) * 4));
+    HIPCHECK(hipMemcpy(c->d_oct, words, n_words * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(c->d_oct + n_words, 0, kOctPad * 4));
+    c->n_words = n_words;
+    c->root = root_ref;
+    c->depth = depth;
+    return SVO_OK;
+}
+
+extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (res_x < 16 || res_y < 16 || res_x > 8192 || res_y > 8192)
+        return svo_fail(SVO_ERR_ARG, "resolution %dx%d outside 16..8192", res_x, res_y);
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    free_frame(c);
+    const size_t N = (size_t)res_x * res_y, S = (size_t)(res_x / 16) * (res_y / 16);
+    const size_t ids_words = 2 * S + N + 64;
+    if (hipMalloc(&c->d_screen, 4 * N * 4) != hipSuccess || hipMalloc(&c->d_coords, 4 * N * 16) != hipSuccess ||
+        hipMalloc(&c->d_ids, ids_words * 4) != hipSuccess || hipMalloc(&c->d_image, N * 4) != hipSuccess ||
+        hipMalloc(&c->d_keys, N * 8) != hipSuccess || hipMalloc(&c->d_snap, N * 4) != hipSuccess) {
+        free_frame(c);
+        return svo_fail(SVO_ERR_NOMEM, "frame buffers for %dx%d", res_x, res_y);
+    }
+    HIPCHECK(hipMemset(c->d_screen, 0, 4 * N * 4));
+    HIPCHECK(hipMemset(c->d_coords, 0, 4 * N * 16));
+    HIPCHECK(hipMemset(c->d_ids, 0, ids_words * 4));
+    HIPCHECK(hipMemset(c->d_image, 0, N * 4));
+    HIPCHECK(hipMemset(c->d_keys, 0xff, N * 8));
+    HIPCHECK(hipMemset(c->d_snap, 0, N * 4));
+    HIPCHECK(hipDeviceSynchronize());
+    c->res_x = res_x;
+    c->res_y = res_y;
+    c->N = N;
+    c->S = S;
+    c->plane1_holes = false;
+    return SVO_OK;
+}
+
+static int need_frame(svo_ctx* c)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (!c->d_screen) return svo_fail(SVO_ERR_STATE, "svo_resize has not been called");
+    return SVO_OK;
+}
+static int need_octree(svo_ctx* c)
+{
+    int rc = need_frame(c);
+    if (rc) return rc;
+    if (!c->d_oct) return svo_fail(SVO_ERR_STATE, "no octree uploaded");
+    return SVO_OK;
+}
+
+static inline void mark(svo_ctx* c, int st)
+{
+    if (c->timing) {
+        hipEventRecord(c->ev[st], c->stream);
+        c->stage_used[st] = true;
+    }
+}
+
+static ProjArgs proj_args(svo_ctx* c, const svo_frame_params* p)
+{
+    ProjArgs a;
+    a.screen = c->d_screen;
+    a.coords = c->d_coords;
+    a.keys = c->d_keys;
+    a.res_x = c->res_x;
+    a.res_y = c->res_y;
+    a.plane_lo = 1;
+    a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
+    a.r00 = p->row0[0]; a.r01 = p->row0[1]; a.r02 = p->row0[2];
+    a.r10 = p->row1[0]; a.r11 = p->row1[1]; a.r12 = p->row1[2];
+    a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
+    return a;
+}
+
+static int launch_memset(svo_ctx* c, uint32_t* dst, uint32_t val, size_t n)
+{
+    if (n == 0) return SVO_OK;
+    size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)c->n_cu * 16);
+    hipLaunchKernelGGL(k_memset, dim3((unsigned)blocks), dim3(256), 0, c->stream, dst, val, n);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+static int launch_scatter_resolve(svo_ctx* c, const svo_frame_params* p, int plane_lo, int n_planes, int clear)
+{
+    ProjArgs a = proj_args(c, p);
+    a.plane_lo = plane_lo;
+    const unsigned gx = (unsigned)((c->N + 255) / 256);
+    mark(c, ST_SCATTER);
+    if (n_planes > 0) {
+        hipLaunchKernelGGL(k_proj_scatter, dim3(gx, (unsigned)n_planes), dim3(256), 0, c->stream, a);
+        HIPCHECK(hipGetLastError());
+    }
+    mark(c, ST_RESOLVE);
+    hipLaunchKernelGGL(k_proj_resolve, dim3(gx), dim3(256), 0, c->stream, a, clear);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+static int launch_rays(svo_ctx* c, const svo_frame_params* p, int use_holes, int add_x, int add_y, int ext_x,
+                       int ext_y, uint32_t* image)
+{
+    RayArgs a;
+    a.oct = c->d_oct;
+    a.root = c->root;
+    a.depth = c->depth;
+    a.screen = c->d_screen;
+    a.coords = (float*)c->d_coords;
+    a.image = image;
+    a.ids = c->d_ids;
+    a.S = (uint32_t)c->S;
+    a.use_holes = use_holes;
+    a.add_x = add_x;
+    a.add_y = add_y;
+    a.ext_x = std::max(0, ext_x);
+    a.ext_y = std::max(0, ext_y);
+    a.tiles_x = std::max(1, (a.ext_x + 7) / 8);
+    a.n_rect = (uint32_t)(((a.ext_x + 7) / 8) * ((a.ext_y + 7) / 8) * 64);
+    a.res_x = c->res_x;
+    a.res_y = c->res_y;
+    a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
+    a.cx0 = p->col0[0]; a.cx1 = p->col0[1]; a.cx2 = p->col0[2];
+    a.cy0 = p->col1[0]; a.cy1 = p->col1[1]; a.cy2 = p->col1[2];
+    a.cz0 = p->col2[0]; a.cz1 = p->col2[1]; a.cz2 = p->col2[2];
+    a.fovx = p->fovx;
+    a.fovy = p->fovy;
+    a.queue = c->d_queue;
+    a.slot = c->launch_seq & 1;
+    c->launch_seq++;
+    a.refill = c->refill;
+    a.iters = c->iters;
+    a.ray_count = (unsigned long long*)(c->d_queue + 8);
+    const unsigned grid = (unsigned)(c->n_cu * c->wg_per_cu);
+    hipLaunchKernelGGL(k_rays, dim3(grid), dim3(RAY_BLOCK), 0, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+static int launch_count(svo_ctx* c)
+{
+    if (c->S == 0) return SVO_OK;
+    hipLaunchKernelGGL(k_counthole, dim3((unsigned)c->S), dim3(64), 0, c->stream, c->d_screen, c->d_ids, c->res_x,
+                       c->res_y);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_sum(svo_ctx* c)
+{
+    hipLaunchKernelGGL(k_sumids, dim3(1), dim3(1024), 0, c->stream, c->d_ids, (uint32_t)c->S);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_write(svo_ctx* c)
+{
+    if (c->S == 0) return SVO_OK;
+    hipLaunchKernelGGL(k_writeids, dim3((unsigned)c->S), dim3(64), 0, c->stream, c->d_screen, c->d_ids, c->res_x,
+                       c->res_y);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_copy(svo_ctx* c, int dst_plane)
+{
+    hipLaunchKernelGGL(k_copy_plane, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, c->d_screen,
+                       c->d_coords, (uint32_t)c->N, (uint32_t)dst_plane);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_fill(svo_ctx* c, const uint32_t* snap, uint32_t* image, int do_fill)
+{
+    hipLaunchKernelGGL(k_fill_color, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, snap,
+                       c->d_screen, image, c->res_x, c->res_y, do_fill);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+#define RC(x)              \
+    do {                   \
+        int _r = (x);      \
+        if (_r) return _r; \
+    } while (0)
+
+extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
+{
+    RC(need_octree(c));
+    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
+    HIPCHECK(hipSetDevice(c->device));
+    for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
+    mark(c, ST_CLEAR);
+    if (p->frame < 2) {  // raycast.h:150-154
+        RC(launch_memset(c, c->d_screen, HOLE, 4 * c->N));
+        c->plane1_holes = true;
+    }
+    // planes 1 and 2 (raycast.h:176-198); plane 1 is skipped while the
+    // library knows it holds only hole words (then it contributes nothing)
+    const int lo = c->plane1_holes ? 2 : 1;
+    RC(launch_scatter_resolve(c, p, lo, 3 - lo, 1));
+    mark(c, ST_COUNT);
+    RC(launch_count(c));
+    mark(c, ST_SCAN);
+    RC(launch_sum(c));
+    mark(c, ST_WRITE);
+    RC(launch_write(c));
+    int ax, ay, ex, ey;
+    svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
+    mark(c, ST_RAYS);
+    RC(launch_rays(c, p, 1, ax, ay, ex, ey, nullptr));
+    mark(c, ST_COPY);
+    RC(launch_copy(c, 2));
+    mark(c, ST_FILTER);
+    RC(launch_fill(c, c->d_screen + 2 * c->N, c->d_image, 1));
+    if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
+    return SVO_OK;
+}
+
+extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
+{
+    RC(need_octree(c));
+    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
+    HIPCHECK(hipSetDevice(c->device));
+    for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
+    mark(c, ST_RAYS);
+    RC(launch_rays(c, p, 0, 0, 0, c->res_x, c->res_y, c->d_image));
+    if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
+    return SVO_OK;
+}
+
+extern "C" int svo_memset(svo_ctx* c, size_t byte_ofs, uint32_t val, size_t bytes)
+{
+    RC(need_frame(c));
+    if ((byte_ofs | bytes) & 3) return svo_fail(SVO_ERR_ARG, "memset offset/size must be multiples of 4");
+    if (byte_ofs + bytes > 16 * c->N) return svo_fail(SVO_ERR_ARG, "memset past the screen buffer");
+    HIPCHECK(hipSetDevice(c->device));
+    RC(launch_memset(c, c->d_screen + byte_ofs / 4, val, bytes / 4));
+    const size_t p1b = 4 * c->N, p1e = 8 * c->N;
+    if (byte_ofs <= p1b && byte_ofs + bytes >= p1e) c->plane1_holes = (val == HOLE);
+    else if (byte_ofs < p1e && byte_ofs + bytes > p1b && val != HOLE) c->plane1_holes = false;
+    return SVO_OK;
+}
+
+extern "C" int svo_proj(svo_ctx* c, int plane, const svo_frame_params* p)
+{
+    RC(need_frame(c));
+    if (!p || plane < 1 || plane > 3) return svo_fail(SVO_ERR_ARG, "plane %d outside 1..3", plane);
+    HIPCHECK(hipSetDevice(c->device));
+    return launch_scatter_resolve(c, p, plane, 1, 0);
+}
+
+extern "C" int svo_counthole(svo_ctx* c)
+{
+    RC(need_frame(c));
+    HIPCHECK(hipSetDevice(c->device));
+    return launch_count(c);
+}
+extern "C" int svo_sumids(svo_ctx* c)
+{
+    RC(need_frame(c));
+    HIPCHECK(hipSetDevice(c->device));
+    return launch_sum(c);
+}
+extern "C" int svo_writeids(svo_ctx* c)
+{
+    RC(need_frame(c));
+    HIPCHECK(hipSetDevice(c->device));
+    return launch_write(c);
+}
+extern "C" int svo_raycast_holes(svo_ctx* c, const svo_frame_params* p)
+{
+    RC(need_octree(c));
+    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
+    HIPCHECK(hipSetDevice(c->device));
+    return launch_rays(c, p, 1, 0, 0, 0, 0, nullptr);
+}
+extern "C" int svo_raycast_rect(svo_ctx* c, const svo_frame_params* p, int add_x, int add_y, int ext_x, int ext_y)
+{
+    RC(need_octree(c));
+    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
+    if (add_x < 0 || add_y < 0 || ext_x < 0 || ext_y < 0 || ext_x > 65536 || ext_y > 65536)
+        return svo_fail(SVO_ERR_ARG, "bad rectangle");
+    HIPCHECK(hipSetDevice(c->device));
+    return launch_rays(c, p, 0, add_x, add_y, ext_x, ext_y, nullptr);
+}
+extern "C" int svo_copy_cache(svo_ctx* c, int dst_plane)
+{
+    RC(need_frame(c));
+    if (dst_plane < 1 || dst_plane > 3) return svo_fail(SVO_ERR_ARG, "plane %d outside 1..3", dst_plane);
+    HIPCHECK(hipSetDevice(c->device));
+    if (dst_plane == 1) c->plane1_holes = false;
+    return launch_copy(c, dst_plane);
+}
+extern "C" int svo_fillhole(svo_ctx* c)
+{
+    RC(need_frame(c));
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipMemcpyAsync(c->d_snap, c->d_screen, c->N * 4, hipMemcpyDeviceToDevice, c->stream));
+    return launch_fill(c, c->d_snap, nullptr, 1);
+}
+extern "C" int svo_colorize(svo_ctx* c)
+{
+    RC(need_frame(c));
+    HIPCHECK(hipSetDevice(c->device));
+    return launch_fill(c, c->d_screen, c->d_image, 0);
+}
+
+static int buf_ptr(svo_ctx* c, int buf, size_t byte_ofs, size_t bytes, char** p)
+{
+    size_t cap = 0;
+    char* base = nullptr;
+    switch (buf) {
+    case SVO_BUF_SCREEN: base = (char*)c->d_screen; cap = 16 * c->N; break;
+    case SVO_BUF_COORDS: base = (char*)c->d_coords; cap = 64 * c->N; break;
+    case SVO_BUF_IDS: base = (char*)c->d_ids; cap = 4 * (2 * c->S + c->N); break;
+    case SVO_BUF_IMAGE: base = (char*)c->d_image; cap = 4 * c->N; break;
+    case SVO_BUF_OCTREE: base = (char*)c->d_oct; cap = 4 * c->n_words; break;
+    default: return svo_fail(SVO_ERR_ARG, "unknown buffer %d", buf);
+    }
+    if (!base) return svo_fail(SVO_ERR_STATE, "buffer %d not allocated", buf);
+    if (byte_ofs > cap || bytes > cap - byte_ofs) return svo_fail(SVO_ERR_ARG, "range outside buffer %d", buf);
+    *p = base + byte_ofs;
+    return SVO_OK;
+}
+
+extern "C" int svo_read(svo_ctx* c, int buf, size_t byte_ofs, size_t bytes, void* dst)
+{
+    if (!c || (!dst && bytes)) return svo_fail(SVO_ERR_ARG, "null argument");
+    char* p;
+    RC(buf_ptr(c, buf, byte_ofs, bytes, &p));
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return SVO_OK;
+}
+
+extern "C" int svo_write(svo_ctx* c, int buf, size_t byte_ofs, size_t bytes, const void* src)
+{
+    if (!c || (!src && bytes)) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (buf == SVO_BUF_OCTREE) return svo_fail(SVO_ERR_ARG, "use svo_upload_octree for the octree");
+    char* p;
+    RC(buf_ptr(c, buf, byte_ofs, bytes, &p));
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    if (buf == SVO_BUF_SCREEN && byte_ofs < 8 * c->N && byte_ofs + bytes > 4 * c->N) c->plane1_holes = false;
+    return SVO_OK;
+}
+
+extern "C" int svo_sync(svo_ctx* c)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        int last = ST_N;
+        for (int i = ST_N - 1; i >= 0; i--) {
+            c->stage_ms[i] = 0.0f;
+            if (!c->stage_used[i]) continue;
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, c->ev[i], c->ev[last]);
+            c->stage_ms[i] = ms;
+            last = i;
+        }
+    }
+    return SVO_OK;
+}
+
+extern "C" int svo_hole_count(svo_ctx* c, uint32_t* holes)
+{
+    if (!holes) return svo_fail(SVO_ERR_ARG, "null argument");
+    return svo_read(c, SVO_BUF_IDS, 0, 4, holes);
+}
+
+extern "C" int svo_ray_count(svo_ctx* c, uint64_t* rays, int reset)
+{
+    if (!c || !rays) return svo_fail(SVO_ERR_ARG, "null argument");
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(rays, c->d_queue + 8, 8, hipMemcpyDeviceToHost));
+    if (reset) HIPCHECK(hipMemset(c->d_queue + 8, 0, 8));
+    return SVO_OK;
+}
+
+extern "C" int svo_enable_timing(svo_ctx* c, int enable)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    c->timing = enable != 0;
+    return SVO_OK;
+}
+
+extern "C" int svo_stage_times(svo_ctx* c, float* ms, int n)
+{
+    if (!c || (!ms && n)) return svo_fail(SVO_ERR_ARG, "null argument");
+    int k = std::min(n, (int)ST_N);
+    for (int i = 0; i < k; i++) ms[i] = c->stage_ms[i];
+    return k;
+}
+
+extern "C" const char* svo_stage_name(int stage)
+{
+    return (stage >= 0 && stage < ST_N) ? kStageNames[stage] : "";
+}
+
+extern "C" int svo_set_tuning(svo_ctx* c, int refill_threshold, int wg_per_cu, int iters_per_refill)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (refill_threshold < 0 || refill_threshold > 64 || wg_per_cu < 0 || wg_per_cu > 32 || iters_per_refill < 0 ||
+        iters_per_refill > 4096)
+        return svo_fail(SVO_ERR_ARG, "tuning value out of range");
+    if (refill_threshold) c->refill = refill_threshold;
+    if (wg_per_cu) c->wg_per_cu = wg_per_cu;
+    if (iters_per_refill) c->iters = iters_per_refill;
+    return SVO_OK;
+}

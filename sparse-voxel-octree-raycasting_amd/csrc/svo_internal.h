@@ -1,0 +1,18 @@
+// svo_internal.h -- shared declarations between the host (.cpp) and device
+// (.hip) halves of libsvo.
+#pragma once
+#include <cstdint>
+
+#include "../../include/svo.h"
+
+// Records the message for svo_last_error() and returns code.
+int svo_fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Host worker threads used by the builder and scene generators.
+int svo_host_threads();
+
+// Hard bound on traversal iterations per ray (a ray that has not exited
+// after this many loop trips is finished as a miss at its current node).
+// Far above what any ray of a valid scene needs: every step advances the
+// ray to a cell face and rays stop at 400000 ray units.
+#define SVO_MAX_RAY_ITERS (1u << 22)

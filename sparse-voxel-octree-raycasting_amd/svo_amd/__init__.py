@@ -1,0 +1,350 @@
+"""ctypes binding of libsvo.so -- the MI355X SVO raycast + image-warp pipeline.
+
+Host-side mirror of the reference's per-frame interface (src/raycast.h driving
+kernel/kernel.cl through src/ocl.h).  Every call goes through the C ABI
+declared in include/svo.h; nothing here computes pixels on the CPU.  If the
+compiled library is missing, importing this module raises ImportError: there
+is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("SVO_LIB", os.path.join(_HERE, "libsvo.so"))
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libsvo.so not found at {LIB_PATH}; build it with "
+        "`python sparse-voxel-octree-raycasting_amd/build.py`")
+_lib = C.CDLL(LIB_PATH)
+
+HOLE = 0xFFFFFF00
+BUF_SCREEN, BUF_COORDS, BUF_IDS, BUF_IMAGE, BUF_OCTREE = range(5)
+
+_u32p = C.POINTER(C.c_uint32)
+_u16p = C.POINTER(C.c_uint16)
+_vp = C.c_void_p
+
+
+class FrameParams(C.Structure):
+    """svo_frame_params: kernel arguments of one frame (kernel.cl:479-482,
+    :609-614, :858-863)."""
+    _fields_ = [("m0", C.c_float * 4),
+                ("row0", C.c_float * 4), ("row1", C.c_float * 4), ("row2", C.c_float * 4),
+                ("col0", C.c_float * 4), ("col1", C.c_float * 4), ("col2", C.c_float * 4),
+                ("fovx", C.c_float), ("fovy", C.c_float),
+                ("frame", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SvoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"svo error {code}: {msg}")
+        self.code = code
+
+
+def _sig(name, restype, *argtypes):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    return f
+
+
+_last_error = _sig("svo_last_error", C.c_char_p)
+
+
+def _check(rc):
+    if rc < 0:
+        raise SvoError(rc, _last_error().decode(errors="replace"))
+    return rc
+
+
+_ctxp = C.c_void_p
+_f = {
+    "create": _sig("svo_create", C.c_int, C.POINTER(_ctxp), C.c_int),
+    "destroy": _sig("svo_destroy", C.c_int, _ctxp),
+    "upload_octree": _sig("svo_upload_octree", C.c_int, _ctxp, _u32p, C.c_size_t, C.c_uint32, C.c_int),
+    "resize": _sig("svo_resize", C.c_int, _ctxp, C.c_int, C.c_int),
+    "frame_warp": _sig("svo_frame_warp", C.c_int, _ctxp, C.POINTER(FrameParams)),
+    "frame_full": _sig("svo_frame_full", C.c_int, _ctxp, C.POINTER(FrameParams)),
+    "memset": _sig("svo_memset", C.c_int, _ctxp, C.c_size_t, C.c_uint32, C.c_size_t),
+    "proj": _sig("svo_proj", C.c_int, _ctxp, C.c_int, C.POINTER(FrameParams)),
+    "counthole": _sig("svo_counthole", C.c_int, _ctxp),
+    "sumids": _sig("svo_sumids", C.c_int, _ctxp),
+    "writeids": _sig("svo_writeids", C.c_int, _ctxp),
+    "raycast_holes": _sig("svo_raycast_holes", C.c_int, _ctxp, C.POINTER(FrameParams)),
+    "raycast_rect": _sig("svo_raycast_rect", C.c_int, _ctxp, C.POINTER(FrameParams),
+                         C.c_int, C.c_int, C.c_int, C.c_int),
+    "copy_cache": _sig("svo_copy_cache", C.c_int, _ctxp, C.c_int),
+    "fillhole": _sig("svo_fillhole", C.c_int, _ctxp),
+    "colorize": _sig("svo_colorize", C.c_int, _ctxp),
+    "read": _sig("svo_read", C.c_int, _ctxp, C.c_int, C.c_size_t, C.c_size_t, _vp),
+    "write": _sig("svo_write", C.c_int, _ctxp, C.c_int, C.c_size_t, C.c_size_t, _vp),
+    "sync": _sig("svo_sync", C.c_int, _ctxp),
+    "hole_count": _sig("svo_hole_count", C.c_int, _ctxp, _u32p),
+    "ray_count": _sig("svo_ray_count", C.c_int, _ctxp, C.POINTER(C.c_uint64), C.c_int),
+    "enable_timing": _sig("svo_enable_timing", C.c_int, _ctxp, C.c_int),
+    "stage_times": _sig("svo_stage_times", C.c_int, _ctxp, C.POINTER(C.c_float), C.c_int),
+    "stage_name": _sig("svo_stage_name", C.c_char_p, C.c_int),
+    "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int),
+    "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
+                   C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
+    "tile_schedule": _sig("svo_tile_schedule", C.c_int, C.c_int, C.c_int, C.c_int,
+                          C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                          C.POINTER(C.c_int)),
+    "build_octree": _sig("svo_build_octree", C.c_int, _u16p, _u32p, C.c_size_t, C.c_int,
+                         C.POINTER(_u32p), C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)),
+    "free": _sig("svo_free", None, _vp),
+    "octree_save": _sig("svo_octree_save", C.c_int, C.c_char_p, _u32p, C.c_size_t, C.c_uint32, C.c_int),
+    "octree_load": _sig("svo_octree_load", C.c_int, C.c_char_p, C.POINTER(_u32p),
+                        C.POINTER(C.c_size_t), C.POINTER(C.c_uint32), C.POINTER(C.c_int)),
+    "scene_generate": _sig("svo_scene_generate", C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
+                           C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
+    "camera_path": _sig("svo_camera_path", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_float,
+                        C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)),
+}
+
+# every symbol declared in include/svo.h
+EXPORTED = [
+    "svo_last_error", "svo_create", "svo_destroy", "svo_upload_octree", "svo_resize",
+    "svo_frame_warp", "svo_frame_full", "svo_memset", "svo_proj", "svo_counthole", "svo_sumids",
+    "svo_writeids", "svo_raycast_holes", "svo_raycast_rect", "svo_copy_cache", "svo_fillhole",
+    "svo_colorize", "svo_read", "svo_write", "svo_sync", "svo_hole_count", "svo_camera",
+    "svo_tile_schedule", "svo_build_octree", "svo_free", "svo_octree_save", "svo_octree_load",
+    "svo_scene_generate", "svo_camera_path", "svo_enable_timing", "svo_stage_times",
+    "svo_stage_name", "svo_set_tuning", "svo_ray_count",
+]
+
+
+def _take_u32(p, n):
+    a = np.ctypeslib.as_array(p, (n,)).copy()
+    _f["free"](C.cast(p, _vp))
+    return a
+
+
+# ------------------------------------------------------------------ host helpers
+def camera(yaw, pitch, roll, pos, depth, frame):
+    """Camera vectors for one frame (raycast.h:113-162, :322-326).
+    Returns (FrameParams, wrapped position)."""
+    p = FrameParams()
+    pos_c = (C.c_float * 3)(*[float(v) for v in pos])
+    wrapped = (C.c_float * 3)()
+    _check(_f["camera"](float(yaw), float(pitch), float(roll), pos_c, int(depth), int(frame),
+                        C.byref(p), wrapped))
+    return p, tuple(wrapped)
+
+
+def tile_schedule(res_x, res_y, frame):
+    """(add_x, add_y, ext_x, ext_y) of the 8x4 refresh tile (raycast.h:363-366)."""
+    v = [C.c_int() for _ in range(4)]
+    _check(_f["tile_schedule"](res_x, res_y, frame, *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def build_octree(xyz, rgba, depth):
+    """set_voxel stream -> compact word array + root ref (octree.h:32-293)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.uint16).reshape(-1, 3)
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint32).reshape(-1)
+    if len(xyz) != len(rgba):
+        raise ValueError("xyz and rgba lengths differ")
+    out = _u32p()
+    n = C.c_size_t()
+    root = C.c_uint32()
+    _check(_f["build_octree"](xyz.ctypes.data_as(_u16p), rgba.ctypes.data_as(_u32p), len(rgba),
+                              int(depth), C.byref(out), C.byref(n), C.byref(root)))
+    return _take_u32(out, n.value), root.value
+
+
+def octree_save(path, words, root, depth):
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    _check(_f["octree_save"](os.fsencode(path), words.ctypes.data_as(_u32p), len(words), root, depth))
+
+
+def octree_load(path):
+    out = _u32p()
+    n = C.c_size_t()
+    root = C.c_uint32()
+    depth = C.c_int()
+    _check(_f["octree_load"](os.fsencode(path), C.byref(out), C.byref(n), C.byref(root), C.byref(depth)))
+    return _take_u32(out, n.value), root.value, depth.value
+
+
+def scene(kind, seed, depth, n_spheres=0):
+    """Seeded synthetic voxel stream: returns (xyz uint16 [n,3], rgba uint32 [n])."""
+    xyz = _u16p()
+    rgba = _u32p()
+    n = C.c_size_t()
+    _check(_f["scene_generate"](int(kind), int(seed), int(depth), int(n_spheres), C.byref(xyz),
+                                C.byref(rgba), C.byref(n)))
+    nv = n.value
+    a = np.ctypeslib.as_array(xyz, (nv * 3,)).copy().reshape(-1, 3) if nv else np.zeros((0, 3), np.uint16)
+    b = np.ctypeslib.as_array(rgba, (nv,)).copy() if nv else np.zeros(0, np.uint32)
+    _f["free"](C.cast(xyz, _vp))
+    _f["free"](C.cast(rgba, _vp))
+    return a, b
+
+
+def camera_path(seed, depth, n_frames, max_voxels_per_frame, max_yaw_deg_per_frame,
+                height_voxels, pitch):
+    """Seeded Catmull-Rom flythrough: float32 [n_frames, 6] rows
+    (yaw, pitch, roll, x, y, z) in world units."""
+    out = np.zeros((n_frames, 6), np.float32)
+    _check(_f["camera_path"](int(seed), int(depth), int(n_frames), float(max_voxels_per_frame),
+                             float(max_yaw_deg_per_frame), float(height_voxels), float(pitch),
+                             out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One GPU context: octree, frame buffers and a HIP stream
+    (the reference's ocl_init + raycast_init state)."""
+
+    def __init__(self, device=0):
+        self._h = _ctxp()
+        _check(_f["create"](C.byref(self._h), int(device)))
+        self.res_x = self.res_y = 0
+        self.depth = 0
+
+    # lifecycle
+    def close(self):
+        if self._h:
+            _f["destroy"](self._h)
+            self._h = _ctxp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def N(self):
+        return self.res_x * self.res_y
+
+    @property
+    def S(self):
+        return (self.res_x // 16) * (self.res_y // 16)
+
+    def upload_octree(self, words, root, depth):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        _check(_f["upload_octree"](self._h, words.ctypes.data_as(_u32p), len(words), int(root), int(depth)))
+        self.depth = int(depth)
+
+    def resize(self, res_x, res_y):
+        _check(_f["resize"](self._h, int(res_x), int(res_y)))
+        self.res_x, self.res_y = int(res_x), int(res_y)
+
+    # frames
+    def frame_warp(self, p):
+        _check(_f["frame_warp"](self._h, C.byref(p)))
+
+    def frame_full(self, p):
+        _check(_f["frame_full"](self._h, C.byref(p)))
+
+    # per-stage (reference kernel names)
+    def memset(self, byte_ofs, val, nbytes):
+        _check(_f["memset"](self._h, int(byte_ofs), int(val) & 0xFFFFFFFF, int(nbytes)))
+
+    def proj(self, plane, p):
+        _check(_f["proj"](self._h, int(plane), C.byref(p)))
+
+    def counthole(self):
+        _check(_f["counthole"](self._h))
+
+    def sumids(self):
+        _check(_f["sumids"](self._h))
+
+    def writeids(self):
+        _check(_f["writeids"](self._h))
+
+    def raycast_holes(self, p):
+        _check(_f["raycast_holes"](self._h, C.byref(p)))
+
+    def raycast_rect(self, p, add_x, add_y, ext_x, ext_y):
+        _check(_f["raycast_rect"](self._h, C.byref(p), int(add_x), int(add_y), int(ext_x), int(ext_y)))
+
+    def copy_cache(self, dst_plane):
+        _check(_f["copy_cache"](self._h, int(dst_plane)))
+
+    def fillhole(self):
+        _check(_f["fillhole"](self._h))
+
+    def colorize(self):
+        _check(_f["colorize"](self._h))
+
+    def sync(self):
+        _check(_f["sync"](self._h))
+
+    def hole_count(self):
+        v = C.c_uint32()
+        _check(_f["hole_count"](self._h, C.byref(v)))
+        return v.value
+
+    def set_tuning(self, refill=0, wg_per_cu=0, iters=0):
+        _check(_f["set_tuning"](self._h, int(refill), int(wg_per_cu), int(iters)))
+
+    def ray_count(self, reset=False):
+        """Rays traced by the ray kernels since the last reset."""
+        v = C.c_uint64()
+        _check(_f["ray_count"](self._h, C.byref(v), 1 if reset else 0))
+        return v.value
+
+    def enable_timing(self, on=True):
+        _check(_f["enable_timing"](self._h, 1 if on else 0))
+
+    def stage_times(self):
+        buf = (C.c_float * 16)()
+        n = _check(_f["stage_times"](self._h, buf, 16))
+        return {_f["stage_name"](i).decode(): buf[i] for i in range(n)}
+
+    # transfers
+    def read(self, buf, byte_ofs, nbytes, dtype=np.uint32):
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(_f["read"](self._h, int(buf), int(byte_ofs), int(nbytes), out.ctypes.data_as(_vp)))
+        return out
+
+    def write(self, buf, byte_ofs, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(_f["write"](self._h, int(buf), int(byte_ofs), arr.nbytes, arr.ctypes.data_as(_vp)))
+
+    def screen(self, plane=None):
+        """uint32 words: [4, res_y, res_x] or one plane [res_y, res_x]."""
+        if plane is None:
+            return self.read(BUF_SCREEN, 0, 16 * self.N).reshape(4, self.res_y, self.res_x)
+        return self.read(BUF_SCREEN, 4 * self.N * plane, 4 * self.N).reshape(self.res_y, self.res_x)
+
+    def coords(self, plane=None):
+        """float32 [.., res_y, res_x, 4] world hit coordinates."""
+        if plane is None:
+            return self.read(BUF_COORDS, 0, 64 * self.N, np.float32).reshape(4, self.res_y, self.res_x, 4)
+        return self.read(BUF_COORDS, 16 * self.N * plane, 16 * self.N, np.float32).reshape(
+            self.res_y, self.res_x, 4)
+
+    def ids(self):
+        return self.read(BUF_IDS, 0, 4 * (2 * self.S + self.N))
+
+    def image(self):
+        return self.read(BUF_IMAGE, 0, 4 * self.N).reshape(self.res_y, self.res_x)
+
+
+def image_rgb(img):
+    """BGRX words (b + g<<8 + r<<16) -> uint8 [h, w, 3] RGB."""
+    img = np.asarray(img, dtype=np.uint32)
+    return np.stack([(img >> 16) & 255, (img >> 8) & 255, img & 255], axis=-1).astype(np.uint8)
+
+
+def psnr(a, b):
+    """PSNR in dB between two uint8 RGB images (inf when identical)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    mse = np.mean((a - b) ** 2)
+    return float("inf") if mse == 0 else 10.0 * np.log10(255.0 ** 2 / mse)

@@ -1,0 +1,61 @@
+"""GPU parity: the HIP path through the C ABI vs the CPU oracle, bit-exact
+on integer words and on the FP32 coordinate bits (identical op order)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import svo_amd as S
+from helpers import CONFIG1, assert_equal_f32_bits, assert_equal_u32, params, path, scene_octree
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scene1():
+    return scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def test_full_frame_config1_bitexact(ctx, scene1):
+    words, root = scene1
+    rx, ry = CONFIG1["res"]
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    pr = path(CONFIG1, 4)
+    for f in (0, 3):
+        p = params(CONFIG1, pr, f)
+        print(f"full frame {f}", flush=True)
+        ctx.frame_full(p)
+        ctx.sync()
+        o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+        o.frame_full(O.params_from(p))
+        assert_equal_u32(ctx.screen(0), o.screen[0], "screen plane 0")
+        assert_equal_f32_bits(ctx.coords(0)[..., :3], o.coords[0][..., :3], "coords xyz")
+        assert_equal_u32(ctx.image(), o.image, "image")
+        o.close()
+
+
+def test_warp_sequence_config1_bitexact(ctx, scene1):
+    words, root = scene1
+    rx, ry = CONFIG1["res"]
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+    pr = path(CONFIG1, 12)
+    for f in range(12):
+        p = params(CONFIG1, pr, f)
+        print(f"warp frame {f}", flush=True)
+        ctx.frame_warp(p)
+        ctx.sync()
+        o.frame_warp(O.params_from(p))
+        assert_equal_u32(ctx.screen(), o.screen, f"frame {f} screen planes")
+        assert_equal_f32_bits(ctx.coords(), o.coords, f"frame {f} coords planes")
+        assert_equal_u32(ctx.ids(), o.ids, f"frame {f} id buffer")
+        assert_equal_u32(ctx.image(), o.image, f"frame {f} image")
+    o.close()

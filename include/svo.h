@@ -181,10 +181,15 @@ int svo_ray_count(svo_ctx* ctx, uint64_t* rays, int reset);
 int svo_stage_times(svo_ctx* ctx, float* ms, int n);
 const char* svo_stage_name(int stage);
 
-/* Ray-kernel tuning (autotuner knobs): work-queue chunk (rays per refill
- * threshold), workgroups per CU for the persistent launch, iterations between
- * refills.  0 keeps the current value. */
-int svo_set_tuning(svo_ctx* ctx, int refill_threshold, int wg_per_cu, int iters_per_refill);
+/* Ray-kernel tuning (the autotuner's knobs); a negative value keeps the
+ * current setting.
+ *   refill_threshold  idle lanes (1..64) that trigger a work-queue refill
+ *   wg_per_cu         persistent workgroups per compute unit (1..32)
+ *   iters_per_refill  traversal steps between refill checks (1..4096)
+ *   lds_cache_words   octree words (top levels, breadth-first) kept in LDS
+ *                     per workgroup (0..12288; 0 disables the cache) */
+int svo_set_tuning(svo_ctx* ctx, int refill_threshold, int wg_per_cu,
+                   int iters_per_refill, int lds_cache_words);
 
 #ifdef __cplusplus
 }

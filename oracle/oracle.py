@@ -30,7 +30,8 @@ class Params(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("iters", C.c_uint64), ("descends", C.c_uint64),
-                ("steps", C.c_uint64), ("pops", C.c_uint64), ("fetch_words", C.c_uint64)]
+                ("steps", C.c_uint64), ("pops", C.c_uint64), ("fetch_words", C.c_uint64),
+                ("desc_by_rek", C.c_uint64 * 16)]
 
 
 class State(C.Structure):
@@ -201,7 +202,9 @@ class Frame:
     @property
     def stats(self):
         st = self.s.st
-        return {k: getattr(st, k) for k, _ in Stats._fields_}
+        d = {k: getattr(st, k) for k, _ in Stats._fields_ if k != "desc_by_rek"}
+        d["desc_by_rek"] = list(st.desc_by_rek)
+        return d
 
     # kernels
     def memset(self, word_ofs, val, n_words):

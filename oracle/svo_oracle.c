@@ -422,7 +422,7 @@ uint32_t orc_trace_pixel(const uint32_t* oct, size_t n_words, uint32_t root,
         node_test = nodeid & (1u << node_index);
         if (node_test) {
             uint32_t tmp = nodeid;
-            if (st) st->descends++;
+            if (st) { st->descends++; st->desc_by_rek[rek & 15]++; }
             nodeid = fetch_child(&o, nodeid, nodeid_before, &local_root,
                                  (uint32_t)node_index, node_test, rek);
             nodeid_before = tmp;
@@ -619,6 +619,7 @@ static void add_stats(orc_stats* a, const orc_stats* b)
 {
     a->rays += b->rays; a->iters += b->iters; a->descends += b->descends;
     a->steps += b->steps; a->pops += b->pops; a->fetch_words += b->fetch_words;
+    for (int i = 0; i < 16; i++) a->desc_by_rek[i] += b->desc_by_rek[i];
 }
 
 /* kernel.cl:594-694: one ray per id in ids[2S .. 2S+ids[0]) */

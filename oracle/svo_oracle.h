@@ -69,6 +69,7 @@ void orc_free(void* p);
 /* ---- ray traversal (kernel.cl:23-214, :594-694, :846-942) ---- */
 typedef struct {
     uint64_t rays, iters, descends, steps, pops, fetch_words;
+    uint64_t desc_by_rek[16];   /* descends by level (rekursion) */
 } orc_stats;
 
 /* One primary ray for pixel (idx,idy); returns the screen word

@@ -36,7 +36,6 @@
 namespace {
 
 constexpr int RAY_BLOCK = 256;
-constexpr int STACK_SLOTS = 13;  // slots 1..D-1 for D <= 13
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t bc8(uint32_t v) { return (uint32_t)__popc(v & 0xffu); }
@@ -57,31 +56,30 @@ __device__ __forceinline__ float cl_sign(float v)
     return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : v);
 }
 
-// kernel.cl:32-62
-__device__ __forceinline__ uint32_t fetch_child(const uint32_t* __restrict__ oct, uint32_t a_node,
-                                                uint32_t a_node_before, uint32_t& local_root,
-                                                uint32_t a_child, uint32_t child_test, int rek)
+// Octree reads go through a raw buffer resource (buffer_load_dword with an
+// SGPR descriptor): 32-bit offsets, and reads at or past num_records return
+// 0 in hardware.  Dword 3 = 0x00020000 is the gfx9-family raw-buffer format.
+typedef __amdgpu_buffer_rsrc_t OctBuf;
+__device__ __forceinline__ OctBuf oct_buffer(const uint32_t* oct, uint32_t bytes)
 {
-    uint32_t n = a_node >> 9;
-    uint32_t nadd = a_child;
-    if (a_node & 256u) {
-        if (!(a_node_before & 256u)) {
-            local_root = n << 6;
-            n = 0;
-        }
-        n += local_root;
-        nadd = bc8(a_node & ((child_test << 1) - 1));
-        if (rek == 2) {
-            uint32_t w = oct[n + (nadd >> 2)];
-            return ((w >> ((nadd & 3) << 3)) & 255u) + 256u + (nadd << 9);
-        }
-    }
-    return oct[n + nadd];
+    return __builtin_amdgcn_make_buffer_rsrc((void*)oct, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t oct_load(OctBuf b, uint32_t i)
+{
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(b, (int)(i * 4u), 0, 0);
+}
+
+// Octree word i: the first cache_words words (the breadth-first normal
+// records of the top levels) come from the LDS copy, the rest from HBM.
+__device__ __forceinline__ uint32_t oct_word(OctBuf oct, const uint32_t* cache, uint32_t cache_words, uint32_t i)
+{
+    return i < cache_words ? cache[i] : oct_load(oct, i);
 }
 
 // kernel.cl:64-112
-__device__ uint32_t fetch_color(const uint32_t* __restrict__ oct, uint32_t a_node, uint32_t a_node_before,
-                                uint32_t a_node_before2, uint32_t local_root, int rek, uint32_t child_test)
+__device__ uint32_t fetch_color(OctBuf oct, const uint32_t* cache, uint32_t cache_words,
+                                uint32_t a_node, uint32_t a_node_before, uint32_t a_node_before2,
+                                uint32_t local_root, int rek, uint32_t child_test)
 {
     uint32_t n = a_node >> 9;
     uint32_t nadd = 8;
@@ -89,9 +87,9 @@ __device__ uint32_t fetch_color(const uint32_t* __restrict__ oct, uint32_t a_nod
         n = (a_node_before >> 9) & 15u;
         uint32_t ofs = (a_node_before2 >> 9) + local_root;
         nadd = bc8(a_node_before2 & 255u);
-        for (uint32_t i = 1; i < n; i++) nadd += bc8((oct[ofs + (i >> 2)] >> ((i & 3) << 3)) & 255u);
+        for (uint32_t i = 1; i < n; i++) nadd += bc8((oct_load(oct, ofs + (i >> 2)) >> ((i & 3) << 3)) & 255u);
         nadd += bc8(a_node_before & ((child_test << 1) - 1));
-        return oct[ofs + (nadd >> 2)] >> ((nadd & 3) << 3);
+        return oct_load(oct, ofs + (nadd >> 2)) >> ((nadd & 3) << 3);
     }
     if (a_node & 256u) {
         if (!(a_node_before & 256u)) {
@@ -102,11 +100,11 @@ __device__ uint32_t fetch_color(const uint32_t* __restrict__ oct, uint32_t a_nod
         if (rek == 2) {
             uint32_t ofs = (a_node_before >> 9) + local_root;
             nadd = 1 + bc8(a_node_before & 255u);
-            for (uint32_t i = 1; i < n; i++) nadd += bc8((oct[ofs + (i >> 2)] >> ((i & 3) << 3)) & 255u);
-            return oct[ofs + (nadd >> 2)] >> ((nadd & 3) << 3);
+            for (uint32_t i = 1; i < n; i++) nadd += bc8((oct_load(oct, ofs + (i >> 2)) >> ((i & 3) << 3)) & 255u);
+            return oct_load(oct, ofs + (nadd >> 2)) >> ((nadd & 3) << 3);
         }
     }
-    return oct[n + nadd];
+    return oct_word(oct, cache, cache_words, n + nadd);
 }
 
 struct RayArgs {
@@ -130,6 +128,9 @@ struct RayArgs {
     int refill;
     int iters;
     unsigned long long* ray_count;  // rays traced, accumulated across launches
+    uint32_t cache_words;           // octree prefix held in LDS
+    uint32_t cache_last;            // max(cache_words, 1) - 1
+    uint32_t oct_bytes;             // buffer extent of the octree (words + pad)
 };
 
 struct Ray {
@@ -217,9 +218,16 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, uint32_t H, const R
     return true;
 }
 
+// LDS: node stack, row `rek` (1..D-1) x RAY_BLOCK lanes, then the node cache
+#define STK(rek) stk[(rek) * RAY_BLOCK + tid]
+
 // One trip of the CAST_RAY do-while loop (kernel.cl:152-211).  Returns true
-// when the loop exits.
-__device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, uint32_t (*stk)[RAY_BLOCK], int tid)
+// when the loop exits.  The child fetch of descending lanes is issued first
+// (fetchChildOffset, kernel.cl:32-62, with its single word read hoisted), the
+// DDA step of the other lanes runs while that read is in flight, and the
+// descend completes afterwards.
+__device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, OctBuf ob, uint32_t* stk, const uint32_t* cache,
+                                         int tid)
 {
     const int D = a.depth;
     const int DAND = (1 << D) - 1;
@@ -228,64 +236,91 @@ __device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, uint32_t (*st
     const int ni = ((p0x & 1) | ((p0y & 1) << 1) | ((p0z & 1) << 2)) ^ r.sxyz;
     const uint32_t test = r.node & (1u << ni);
     r.test = test;
-    if (test) {
-        // descend (kernel.cl:160-179).  At rek == 1 the fetched word is never
-        // consumed (the hit exits and fetchColor reads only node_before), so
-        // the load is skipped.
-        uint32_t child = r.node;
-        if (r.rek >= 2) child = fetch_child(a.oct, r.node, r.before, r.lroot, (uint32_t)ni, test, r.rek);
-        r.before = r.node;
-        r.node = child;
-        if (r.rek <= r.lod) return true;
-        r.rek--;
-        stk[r.rek][tid] = child;
-        return false;
-    }
-    // DDA step to the nearest cell face (kernel.cl:181-194)
-    const float mx = (float)(int)((uint32_t)(p0x + 1) << r.rek) - r.px;
-    const float my = (float)(int)((uint32_t)(p0y + 1) << r.rek) - r.py;
-    const float mz = (float)(int)((uint32_t)(p0z + 1) << r.rek) - r.pz;
-    float dist = mx * r.lx;
-    const float ddy = my * r.ly, ddz = mz * r.lz;
-    float sx = mx, sy = r.g0y * mx, sz = r.g0z * mx;
-    if (ddy < dist) { dist = ddy; sx = r.g1x * my; sy = my; sz = r.g1z * my; }
-    if (ddz < dist) { dist = ddz; sx = r.g2x * mz; sy = r.g2y * mz; sz = mz; }
-    r.px += sx; r.py += sy; r.pz += sz;
-    const int nx = (int)r.px, ny = (int)r.py, nz = (int)r.pz;
-    r.dist += dist;
-    // pop level (kernel.cl:197-208)
-    const int x0ry = r.iy ^ ny;
-    const int x0r = (r.ix ^ nx) ^ x0ry ^ (r.iz ^ nz);
-    r.ix = nx; r.iy = ny; r.iz = nz;
-    const uint32_t m = (uint32_t)(x0r & DAND);
-    const int rek_new = m ? 31 - __clz((int)m) : -1;
-    if (r.dist > 400000.0f) return true;
-    if (rek_new >= r.rek) {
-        r.rek = rek_new + 1;
-        if (r.rek >= D) {
-            r.node = a.root;
-            r.before = a.root;
+
+    // ---- issue the child fetch.  At rek == 1 the fetched word is never
+    // consumed (the hit exits and fetchColor reads only node_before), so no
+    // read is made there.
+    uint32_t w = 0, nadd = 0;
+    bool synth = false;
+    if (test && r.rek >= 2) {
+        uint32_t n = r.node >> 9;
+        uint32_t idx;
+        if (r.node & 256u) {
+            if (!(r.before & 256u)) {
+                r.lroot = n << 6;
+                n = 0;
+            }
+            n += r.lroot;
+            nadd = bc8(r.node & ((test << 1) - 1));
+            synth = (r.rek == 2);
+            idx = synth ? n + (nadd >> 2) : n + nadd;
         } else {
-            r.node = stk[r.rek][tid];
-            // slot D belongs to no node of this ray; the reference reads a
-            // neighbour's slot there, a value consulted only for block
-            // nodes, and depth-1 nodes are normal nodes for D >= 8
-            r.before = (r.rek + 1 <= D - 1) ? stk[r.rek + 1][tid] : a.root;
+            idx = n + (uint32_t)ni;
         }
-        if (r.dist > r.lodsw) { r.lodsw *= 2.0f; ++r.lod; }
+        // normal records of the top levels sit in the LDS prefix
+        if (idx < a.cache_words)
+            w = cache[idx];
+        else
+            w = oct_load(ob, idx);
     }
-    return (x0ry & (2 * DAND + 2)) != 0;
+
+    if (!test) {
+        // DDA step to the nearest cell face (kernel.cl:181-194)
+        const float mx = (float)(int)((uint32_t)(p0x + 1) << r.rek) - r.px;
+        const float my = (float)(int)((uint32_t)(p0y + 1) << r.rek) - r.py;
+        const float mz = (float)(int)((uint32_t)(p0z + 1) << r.rek) - r.pz;
+        float dist = mx * r.lx;
+        const float ddy = my * r.ly, ddz = mz * r.lz;
+        float sx = mx, sy = r.g0y * mx, sz = r.g0z * mx;
+        if (ddy < dist) { dist = ddy; sx = r.g1x * my; sy = my; sz = r.g1z * my; }
+        if (ddz < dist) { dist = ddz; sx = r.g2x * mz; sy = r.g2y * mz; sz = mz; }
+        r.px += sx; r.py += sy; r.pz += sz;
+        const int nx = (int)r.px, ny = (int)r.py, nz = (int)r.pz;
+        r.dist += dist;
+        // pop level (kernel.cl:197-208)
+        const int x0ry = r.iy ^ ny;
+        const int x0r = (r.ix ^ nx) ^ x0ry ^ (r.iz ^ nz);
+        r.ix = nx; r.iy = ny; r.iz = nz;
+        const uint32_t m = (uint32_t)(x0r & DAND);
+        const int rek_new = m ? 31 - __clz((int)m) : -1;
+        if (r.dist > 400000.0f) return true;
+        if (rek_new >= r.rek) {
+            r.rek = rek_new + 1;
+            if (r.rek >= D) {
+                r.node = a.root;
+                r.before = a.root;
+            } else {
+                r.node = STK(r.rek);
+                // slot D belongs to no node of this ray; the reference reads a
+                // neighbour's slot there, a value consulted only for block
+                // nodes, and depth-1 nodes are normal nodes for D >= 8
+                r.before = (r.rek + 1 <= D - 1) ? STK(r.rek + 1) : a.root;
+            }
+            if (r.dist > r.lodsw) { r.lodsw *= 2.0f; ++r.lod; }
+        }
+        return (x0ry & (2 * DAND + 2)) != 0;
+    }
+
+    // ---- complete the descend (kernel.cl:162-178)
+    const uint32_t child = synth ? ((w >> ((nadd & 3) << 3)) & 255u) + 256u + (nadd << 9) : w;
+    r.before = r.node;
+    r.node = (r.rek >= 2) ? child : r.node;
+    if (r.rek <= r.lod) return true;
+    r.rek--;
+    STK(r.rek) = r.node;
+    return false;
 }
 
-__device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, uint32_t (*stk)[RAY_BLOCK], int tid)
+__device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, OctBuf ob, uint32_t* stk, const uint32_t* cache,
+                                           int tid)
 {
     const float SCALE_MAX = (float)(1 << (a.depth + 1));
     float px = r.px, py = r.py, pz = r.pz;
     if (r.sxyz & 1) px = SCALE_MAX - px;
     if (r.sxyz & 2) py = SCALE_MAX - py;
     if (r.sxyz & 4) pz = SCALE_MAX - pz;
-    uint32_t b2 = (r.rek == 1) ? stk[2][tid] : 0u;
-    uint32_t col = fetch_color(a.oct, r.node, r.before, b2, r.lroot, r.rek, r.test);
+    uint32_t b2 = (r.rek == 1) ? STK(2) : 0u;
+    uint32_t col = fetch_color(ob, cache, a.cache_words, r.node, r.before, b2, r.lroot, r.rek, r.test);
     uint32_t word = 0xff000000u + col;
     a.screen[r.pix] = word;
     float* c = a.coords + 4u * r.pix;
@@ -297,8 +332,13 @@ __device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, uint32_t (*
 
 __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
 {
-    __shared__ uint32_t stk[STACK_SLOTS][RAY_BLOCK];
+    extern __shared__ uint32_t smem[];
+    uint32_t* stk = smem;
+    uint32_t* cache = smem + a.depth * RAY_BLOCK;
     const int tid = threadIdx.x;
+    for (uint32_t i = (uint32_t)tid; i < a.cache_words; i += RAY_BLOCK) cache[i] = a.oct[i];
+    __syncthreads();
+    const OctBuf ob = oct_buffer(a.oct, a.oct_bytes);
     const int lane = __lane_id();
     const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
     const uint32_t H = a.use_holes ? min(a.ids[0], N) : 0u;  // the id list holds at most N entries
@@ -330,8 +370,8 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
         }
         if (active) {
             for (int it = 0; it < a.iters; ++it) {
-                if (step_ray(r, a, stk, tid)) {
-                    finish_ray(r, a, stk, tid);
+                if (step_ray(r, a, ob, stk, cache, tid)) {
+                    finish_ray(r, a, ob, stk, cache, tid);
                     active = false;
                     break;
                 }
@@ -538,6 +578,12 @@ __global__ __launch_bounds__(256) void k_fill_color(const uint32_t* __restrict__
 }  // namespace
 
 // ====================================================================== runtime
+#define RC(x)              \
+    do {                   \
+        int _r = (x);      \
+        if (_r) return _r; \
+    } while (0)
+
 #define HIPCHECK(expr)                                                                    \
     do {                                                                                  \
         hipError_t _e = (expr);                                                           \
@@ -554,8 +600,9 @@ struct svo_ctx {
     hipStream_t stream = nullptr;
     int n_cu = 256;
     // octree
-    uint32_t* d_oct = nullptr;
+    uint32_t* d_oct = nullptr;  // validated copy, normal records in breadth-first order
     size_t n_words = 0;
+    uint32_t normal_words = 0;  // size of the renumbered normal region
     uint32_t root = 0;
     int depth = 0;
     // frame buffers
@@ -572,6 +619,7 @@ struct svo_ctx {
     bool plane1_holes = false;
     // tuning
     int refill = 16, wg_per_cu = 8, iters = 16;
+    uint32_t cache_req = 1024;  // requested LDS node-cache words
     // timing
     bool timing = false;
     hipEvent_t ev[ST_N + 1] = {};
@@ -581,12 +629,12 @@ struct svo_ctx {
 
 static int free_frame(svo_ctx* c)
 {
-    hipFree(c->d_screen);
-    hipFree(c->d_coords);
-    hipFree(c->d_ids);
-    hipFree(c->d_image);
-    hipFree(c->d_keys);
-    hipFree(c->d_snap);
+    (void)hipFree(c->d_screen);
+    (void)hipFree(c->d_coords);
+    (void)hipFree(c->d_ids);
+    (void)hipFree(c->d_image);
+    (void)hipFree(c->d_keys);
+    (void)hipFree(c->d_snap);
     c->d_screen = nullptr;
     c->d_coords = nullptr;
     c->d_ids = nullptr;
@@ -611,12 +659,13 @@ extern "C" int svo_create(svo_ctx** out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->n_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&c->d_queue, 64) != hipSuccess || hipMemset(c->d_queue, 0, 64) != hipSuccess) {
-        delete c;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&c->d_queue, 64) == hipSuccess && hipMemset(c->d_queue, 0, 64) == hipSuccess;
+    for (int i = 0; ok && i <= ST_N; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
+    if (!ok) {
+        svo_destroy(c);
         return svo_fail(SVO_ERR_HIP, "context set-up failed on device %d", device);
     }
-    for (int i = 0; i <= ST_N; i++) hipEventCreate(&c->ev[i]);
     *out = c;
     return SVO_OK;
 }
@@ -624,14 +673,14 @@ extern "C" int svo_create(svo_ctx** out, int device)
 extern "C" int svo_destroy(svo_ctx* c)
 {
     if (!c) return SVO_OK;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     free_frame(c);
-    hipFree(c->d_oct);
-    hipFree(c->d_queue);
+    (void)hipFree(c->d_oct);
+    (void)hipFree(c->d_queue);
     for (int i = 0; i <= ST_N; i++)
-        if (c->ev[i]) hipEventDestroy(c->ev[i]);
-    hipStreamDestroy(c->stream);
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return SVO_OK;
 }
@@ -646,18 +695,20 @@ extern "C" int svo_upload_octree(svo_ctx* c, const uint32_t* words, size_t n_wor
     if (!c || !words) return svo_fail(SVO_ERR_ARG, "null argument");
     if (depth < 8 || depth > 13) return svo_fail(SVO_ERR_ARG, "depth %d outside 8..13", depth);
     if (n_words < 10 || n_words > (1ull << 29)) return svo_fail(SVO_ERR_ARG, "word count %zu out of range", n_words);
-    if ((root_ref & 256u) || (size_t)(root_ref >> 9) + 10 > n_words)
-        return svo_fail(SVO_ERR_ARG, "root ref 0x%x is not a normal node inside the array", root_ref);
+    std::vector<uint32_t> dev;
+    uint32_t new_root = 0, normal_words = 0;
+    RC(svo_device_layout(words, n_words, root_ref, depth, dev, &new_root, &normal_words));
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    hipFree(c->d_oct);
+    (void)hipFree(c->d_oct);
     c->d_oct = nullptr;
     HIPCHECK(hipMalloc(&c->d_oct, (n_words + kOctPad) * 4));
-    HIPCHECK(hipMemcpy(c->d_oct, words, n_words * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->d_oct, dev.data(), n_words * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(c->d_oct + n_words, 0, kOctPad * 4));
     c->n_words = n_words;
-    c->root = root_ref;
+    c->root = new_root;
     c->depth = depth;
+    c->normal_words = normal_words;
     return SVO_OK;
 }
 
@@ -709,7 +760,7 @@ static int need_octree(svo_ctx* c)
 static inline void mark(svo_ctx* c, int st)
 {
     if (c->timing) {
-        hipEventRecord(c->ev[st], c->stream);
+        (void)hipEventRecord(c->ev[st], c->stream);
         c->stage_used[st] = true;
     }
 }
@@ -788,8 +839,12 @@ static int launch_rays(svo_ctx* c, const svo_frame_params* p, int use_holes, int
     a.refill = c->refill;
     a.iters = c->iters;
     a.ray_count = (unsigned long long*)(c->d_queue + 8);
+    a.cache_words = std::min(c->cache_req, c->normal_words);
+    a.cache_last = a.cache_words ? a.cache_words - 1 : 0;
+    a.oct_bytes = (uint32_t)std::min<size_t>((c->n_words + kOctPad) * 4, 0xfffffffcull);
     const unsigned grid = (unsigned)(c->n_cu * c->wg_per_cu);
-    hipLaunchKernelGGL(k_rays, dim3(grid), dim3(RAY_BLOCK), 0, c->stream, a);
+    const size_t lds = ((size_t)c->depth * RAY_BLOCK + std::max(a.cache_words, 1u)) * 4;
+    hipLaunchKernelGGL(k_rays, dim3(grid), dim3(RAY_BLOCK), lds, c->stream, a);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }
@@ -830,12 +885,6 @@ static int launch_fill(svo_ctx* c, const uint32_t* snap, uint32_t* image, int do
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }
-
-#define RC(x)              \
-    do {                   \
-        int _r = (x);      \
-        if (_r) return _r; \
-    } while (0)
 
 extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
 {
@@ -1012,7 +1061,7 @@ extern "C" int svo_sync(svo_ctx* c)
             c->stage_ms[i] = 0.0f;
             if (!c->stage_used[i]) continue;
             float ms = 0.0f;
-            hipEventElapsedTime(&ms, c->ev[i], c->ev[last]);
+            (void)hipEventElapsedTime(&ms, c->ev[i], c->ev[last]);
             c->stage_ms[i] = ms;
             last = i;
         }
@@ -1056,14 +1105,16 @@ extern "C" const char* svo_stage_name(int stage)
     return (stage >= 0 && stage < ST_N) ? kStageNames[stage] : "";
 }
 
-extern "C" int svo_set_tuning(svo_ctx* c, int refill_threshold, int wg_per_cu, int iters_per_refill)
+extern "C" int svo_set_tuning(svo_ctx* c, int refill_threshold, int wg_per_cu, int iters_per_refill,
+                              int lds_cache_words)
 {
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
-    if (refill_threshold < 0 || refill_threshold > 64 || wg_per_cu < 0 || wg_per_cu > 32 || iters_per_refill < 0 ||
-        iters_per_refill > 4096)
+    if (refill_threshold == 0 || refill_threshold > 64 || wg_per_cu == 0 || wg_per_cu > 32 ||
+        iters_per_refill == 0 || iters_per_refill > 4096 || lds_cache_words > 12288)
         return svo_fail(SVO_ERR_ARG, "tuning value out of range");
-    if (refill_threshold) c->refill = refill_threshold;
-    if (wg_per_cu) c->wg_per_cu = wg_per_cu;
-    if (iters_per_refill) c->iters = iters_per_refill;
+    if (refill_threshold > 0) c->refill = refill_threshold;
+    if (wg_per_cu > 0) c->wg_per_cu = wg_per_cu;
+    if (iters_per_refill > 0) c->iters = iters_per_refill;
+    if (lds_cache_words >= 0) c->cache_req = (uint32_t)lds_cache_words;
     return SVO_OK;
 }

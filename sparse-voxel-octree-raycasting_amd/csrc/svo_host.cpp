@@ -331,6 +331,110 @@ extern "C" int svo_build_octree(const uint16_t* xyz, const uint32_t* rgba, size_
     return SVO_OK;
 }
 
+// ---------------------------------------------------------------- device layout
+// Validates every ref of a compact word array (octree.h:232-293 layout) and
+// renumbers its 10-word normal records in breadth-first order (root at word
+// 0, then depth 1, ...).  Normal-node refs are only ever used to address
+// their own records (kernel.cl:42,:61,:74,:111); the words a ray returns are
+// colour words and block-relative refs, which do not move, so every output is
+// unchanged while the top levels become a prefix of the array that the ray
+// kernel keeps in LDS.
+namespace {
+struct Validator {
+    const uint32_t* w;
+    size_t n;
+    int D;
+    bool ok = true;
+    std::string why;
+    bool in(size_t i) { return i < n; }
+    void fail(const char* m, size_t at)
+    {
+        if (ok) {
+            char b[160];
+            snprintf(b, sizeof(b), "%s (word %zu)", m, at);
+            why = b;
+        }
+        ok = false;
+    }
+    // compact node at depth d (D-6 .. D-3) at absolute word a, or byte node
+    void node(size_t local_root, uint32_t ref, int d)
+    {
+        if (!ok) return;
+        size_t a = local_root + (ref >> 9);
+        int mask = (int)(ref & 255u);
+        if (!(ref & 256u)) { fail("block-internal ref without the block bit", a); return; }
+        if (d == D - 2) {  // byte node: colour, masks, leaf colours
+            size_t nb = 1 + (size_t)popc8((uint32_t)mask);
+            if (!in(a + (nb + 3) / 4)) { fail("byte node past the array", a); return; }
+            size_t leaves = 0;
+            for (size_t i = 1; i < nb; i++) leaves += popc8((w[a + (i >> 2)] >> ((i & 3) * 8)) & 255u);
+            if (!in(a + (nb + leaves + 3) / 4 - 1)) fail("byte node leaves past the array", a);
+            return;
+        }
+        int k = popc8((uint32_t)mask);
+        if (!in(a + (size_t)k)) { fail("compact node past the array", a); return; }
+        for (int j = 0; j < k && ok; j++) node(local_root, w[a + 1 + j], d + 1);
+    }
+};
+}  // namespace
+
+int svo_device_layout(const uint32_t* words, size_t n_words, uint32_t root, int depth,
+                      std::vector<uint32_t>& out, uint32_t* new_root, uint32_t* normal_words)
+{
+    const size_t nrec_max = std::min<size_t>(n_words, 2097152) / 10;
+    std::vector<int32_t> newidx(nrec_max + 1, -1);
+    std::vector<uint32_t> order;
+    if ((root & 256u) || (root >> 9) % 10 != 0 || (root >> 9) / 10 >= nrec_max)
+        return svo_fail(SVO_ERR_ARG, "root ref 0x%x is not a normal record", root);
+    Validator V{words, n_words, depth};
+    order.push_back(root >> 9);
+    newidx[(root >> 9) / 10] = 0;
+    std::vector<int> level(1, 0);
+    for (size_t q = 0; q < order.size(); q++) {
+        uint32_t o = order[q];
+        for (int j = 0; j < 8; j++) {
+            uint32_t v = words[o + j];
+            if (!v) continue;
+            if (v & 256u) {
+                size_t lr = (size_t)(v >> 9) << 6;
+                if (level[q] + 1 != depth - 6)
+                    return svo_fail(SVO_ERR_ARG, "block ref at depth %d (expected %d)", level[q] + 1, depth - 6);
+                if (lr < (size_t)10 * nrec_max && lr < 2097152)
+                    return svo_fail(SVO_ERR_ARG, "block at word %zu overlaps the normal region", lr);
+                // block root record sits at relative offset 0
+                V.node(lr, (v & 255u) | 256u, depth - 6);
+                if (!V.ok) return svo_fail(SVO_ERR_ARG, "invalid octree: %s", V.why.c_str());
+                continue;
+            }
+            uint32_t c = v >> 9;
+            if (c % 10 != 0 || c / 10 >= nrec_max)
+                return svo_fail(SVO_ERR_ARG, "normal ref 0x%x out of range", v);
+            if (level[q] + 1 > depth - 7)
+                return svo_fail(SVO_ERR_ARG, "normal node below depth %d", depth - 7);
+            if (newidx[c / 10] < 0) {
+                newidx[c / 10] = (int32_t)order.size();
+                order.push_back(c);
+                level.push_back(level[q] + 1);
+            }
+        }
+    }
+    out.assign(words, words + n_words);
+    for (size_t q = 0; q < order.size(); q++) {
+        const uint32_t* src = words + order[q];
+        uint32_t* dst = out.data() + 10 * q;
+        for (int j = 0; j < 8; j++) {
+            uint32_t v = src[j];
+            if (v && !(v & 256u)) v = ((uint32_t)(10 * newidx[(v >> 9) / 10]) << 9) | (v & 255u);
+            dst[j] = v;
+        }
+        dst[8] = src[8];
+        dst[9] = src[9];
+    }
+    *new_root = root & 255u;  // record 0
+    *normal_words = (uint32_t)(10 * order.size());
+    return SVO_OK;
+}
+
 // ---------------------------------------------------------------- save/load
 static const char kMagic[8] = {'S', 'V', 'O', 'W', 'O', 'R', 'D', '1'};
 

@@ -11,6 +11,12 @@ int svo_fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)
 // Host worker threads used by the builder and scene generators.
 int svo_host_threads();
 
+#include <vector>
+// Validated device copy of a compact word array with the normal region in
+// breadth-first order; *normal_words = size of the renumbered region.
+int svo_device_layout(const uint32_t* words, size_t n_words, uint32_t root, int depth,
+                      std::vector<uint32_t>& out, uint32_t* new_root, uint32_t* normal_words);
+
 // Hard bound on traversal iterations per ray (a ray that has not exited
 // after this many loop trips is finished as a miss at its current node).
 // Far above what any ray of a valid scene needs: every step advances the

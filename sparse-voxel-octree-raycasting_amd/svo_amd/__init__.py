@@ -88,7 +88,7 @@ _f = {
     "enable_timing": _sig("svo_enable_timing", C.c_int, _ctxp, C.c_int),
     "stage_times": _sig("svo_stage_times", C.c_int, _ctxp, C.POINTER(C.c_float), C.c_int),
     "stage_name": _sig("svo_stage_name", C.c_char_p, C.c_int),
-    "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int),
+    "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int, C.c_int),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
     "tile_schedule": _sig("svo_tile_schedule", C.c_int, C.c_int, C.c_int, C.c_int,
@@ -289,8 +289,9 @@ class Context:
         _check(_f["hole_count"](self._h, C.byref(v)))
         return v.value
 
-    def set_tuning(self, refill=0, wg_per_cu=0, iters=0):
-        _check(_f["set_tuning"](self._h, int(refill), int(wg_per_cu), int(iters)))
+    def set_tuning(self, refill=-1, wg_per_cu=-1, iters=-1, lds_cache_words=-1):
+        """Ray-kernel knobs; -1 keeps the current value."""
+        _check(_f["set_tuning"](self._h, int(refill), int(wg_per_cu), int(iters), int(lds_cache_words)))
 
     def ray_count(self, reset=False):
         """Rays traced by the ray kernels since the last reset."""

@@ -1,0 +1,79 @@
+"""Autotuner for the ray kernel's launch knobs.
+
+Knobs (svo_set_tuning): work-queue refill threshold (idle lanes per refill),
+persistent workgroups per CU, traversal steps between refill checks, and the
+LDS node-cache depth (octree words of the breadth-first top levels kept in
+LDS).  Every candidate replays the same frame window from the same starting
+state (the window starts with the two clearing frames), so candidates see
+identical work; the score is the median GPU time per frame over the timed
+frames, measured with the library's HIP-event stage timer.  Results do not
+depend on the knobs (the tests check bit-exactness under several settings).
+"""
+from __future__ import annotations
+
+import itertools
+import statistics
+
+DEFAULT_SPACE = {
+    "refill": [8, 16, 32, 48],
+    "wg_per_cu": [4, 8, 16],
+    "iters": [8, 16, 64],
+    "lds_cache_words": [0, 96, 736, 2048, 4096],
+}
+
+
+def _frame_time(ctx, params, warmup, mode):
+    ctx.enable_timing(False)
+    for p in params[:warmup]:
+        (ctx.frame_warp if mode == "warp" else ctx.frame_full)(p)
+    ctx.sync()
+    ctx.enable_timing(True)
+    ts = []
+    for p in params[warmup:]:
+        (ctx.frame_warp if mode == "warp" else ctx.frame_full)(p)
+        ctx.sync()
+        ts.append(sum(ctx.stage_times().values()))
+    ctx.enable_timing(False)
+    return statistics.median(ts)
+
+
+def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):
+    """Pick the fastest knob setting for this context's scene/resolution.
+
+    params: FrameParams list (params[0] must have frame 0).  With
+    coordinate=True the knobs are tuned one at a time (coordinate descent,
+    two sweeps); otherwise the full grid is measured.  Applies and returns
+    (best_setting, best_ms, trials)."""
+    space = dict(DEFAULT_SPACE if space is None else space)
+    keys = list(space)
+    best = {k: space[k][min(1, len(space[k]) - 1)] for k in keys}
+    trials = []
+
+    def measure(cfg):
+        ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
+        ms = _frame_time(ctx, params, warmup, mode)
+        trials.append((dict(cfg), ms))
+        if log:
+            log(f"[autotune] {cfg} -> {ms:.4f} ms")
+        return ms
+
+    if coordinate:
+        best_ms = measure(best)
+        for _ in range(2):
+            for k in keys:
+                for v in space[k]:
+                    if v == best[k]:
+                        continue
+                    cfg = dict(best, **{k: v})
+                    ms = measure(cfg)
+                    if ms < best_ms:
+                        best, best_ms = cfg, ms
+    else:
+        best_ms = float("inf")
+        for vals in itertools.product(*[space[k] for k in keys]):
+            cfg = dict(zip(keys, vals))
+            ms = measure(cfg)
+            if ms < best_ms:
+                best, best_ms = cfg, ms
+    ctx.set_tuning(best["refill"], best["wg_per_cu"], best["iters"], best["lds_cache_words"])
+    return best, best_ms, trials

@@ -1,0 +1,172 @@
+"""GPU parity of the per-stage entry points (reference kernel names) and of
+edge cases, against the CPU oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+import oracle as O
+import svo_amd as S
+from helpers import CONFIG1, assert_equal_f32_bits, assert_equal_u32, params, path, scene_octree
+
+pytestmark = pytest.mark.gpu
+HOLE = 0xFFFFFF00
+
+
+@pytest.fixture(scope="module")
+def scene1():
+    return scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def _compare_all(c, o, what):
+    assert_equal_u32(c.screen(), o.screen, f"{what}: screen")
+    assert_equal_f32_bits(c.coords(), o.coords, f"{what}: coords")
+    assert_equal_u32(c.ids(), o.ids, f"{what}: ids")
+
+
+def test_stagewise_frames(ctx, scene1):
+    """raycast.h:147-438 issued stage by stage through the C ABI."""
+    words, root = scene1
+    rx, ry = 256, 128
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+    N = rx * ry
+    pr = path(CONFIG1, 6)
+    for f in range(6):
+        p = params(CONFIG1, pr, f)
+        op = O.params_from(p)
+        print(f"stagewise frame {f}", flush=True)
+        if f < 2:
+            ctx.memset(0, HOLE, 16 * N)
+            o.memset(0, HOLE, 4 * N)
+        ctx.memset(0, HOLE, 4 * N)
+        o.memset(0, HOLE, N)
+        for plane in (1, 2):
+            ctx.proj(plane, p)
+            o.proj(plane, op)
+            ctx.sync()
+            _compare_all(ctx, o, f"frame {f} proj {plane}")
+        ctx.counthole(); o.counthole()
+        ctx.sumids(); o.sumids()
+        ctx.sync()
+        assert ctx.hole_count() == o.ids[0]
+        ctx.writeids(); o.writeids()
+        ctx.sync()
+        _compare_all(ctx, o, f"frame {f} compaction")
+        ctx.raycast_holes(p); o.raycast_holes(op)
+        ax, ay, ex, ey = S.tile_schedule(rx, ry, f)
+        assert (ax, ay, ex, ey) == O.tile_schedule(rx, ry, f)
+        ctx.raycast_rect(p, ax, ay, ex, ey); o.raycast_rect(op, ax, ay, ex, ey)
+        ctx.sync()
+        _compare_all(ctx, o, f"frame {f} rays")
+        ctx.copy_cache(2); o.copy_plane(2)
+        ctx.fillhole(); o.fillhole_snapshot()
+        ctx.colorize(); o.colorize()
+        ctx.sync()
+        _compare_all(ctx, o, f"frame {f} filter")
+        assert_equal_u32(ctx.image(), o.image, f"frame {f} image")
+    o.close()
+
+
+def test_proj_against_populated_plane0(ctx, scene1):
+    """proj without the per-frame clear: sources must beat the existing
+    plane-0 depth key (kernel.cl:571) to win."""
+    words, root = scene1
+    rx, ry = 128, 96
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    o = O.Frame(rx, ry, words, root, 8)
+    pr = path(CONFIG1, 4)
+    p0 = params(CONFIG1, pr, 0)
+    ctx.frame_full(p0)
+    o.frame_full(O.params_from(p0))
+    ctx.copy_cache(2); o.copy_plane(2)
+    rng = np.random.default_rng(5)
+    plane0 = rng.integers(0, 2**32, size=(ry, rx), dtype=np.uint64).astype(np.uint32)
+    plane0[rng.random((ry, rx)) < 0.3] = HOLE
+    ctx.write(S.BUF_SCREEN, 0, plane0)
+    o.screen[0][:] = plane0
+    p2 = params(CONFIG1, pr, 3)
+    ctx.proj(2, p2)
+    o.proj(2, O.params_from(p2))
+    ctx.sync()
+    _compare_all(ctx, o, "populated proj")
+    o.close()
+
+
+@pytest.mark.parametrize("res", [(200, 120), (48, 40), (333, 77)])
+def test_ragged_resolution_warp(ctx, scene1, res):
+    """Resolutions that are not multiples of 16 (partial blocks are never
+    compacted, kernel.cl:243-244)."""
+    words, root = scene1
+    rx, ry = res
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+    pr = path(CONFIG1, 5)
+    for f in range(5):
+        p = params(CONFIG1, pr, f)
+        ctx.frame_warp(p)
+        o.frame_warp(O.params_from(p))
+        ctx.sync()
+        _compare_all(ctx, o, f"{res} frame {f}")
+        assert_equal_u32(ctx.image(), o.image, f"{res} frame {f} image")
+    o.close()
+
+
+def test_empty_scene(ctx):
+    """No voxels: every ray misses and returns the root colour word."""
+    words, root = S.build_octree(np.zeros((0, 3), np.uint16), np.zeros(0, np.uint32), 8)
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(64, 32)
+    p, _ = S.camera(0.3, 0.2, 0.0, (3.0, 10.0, 4.0), 8, 0)
+    ctx.frame_full(p)
+    ctx.sync()
+    o = O.Frame(64, 32, words, root, 8)
+    o.frame_full(O.params_from(p))
+    assert_equal_u32(ctx.screen(0), o.screen[0], "empty scene screen")
+    assert np.all(ctx.screen(0) == 0xFF000000)
+    o.close()
+
+
+def test_depth_variants(ctx):
+    """Depths 9 and 12 (block boundaries move with D) on small random scenes."""
+    rng = np.random.default_rng(11)
+    for D in (9, 12):
+        n = 40000
+        size = 1 << D
+        xyz = np.stack([rng.integers(0, size, n), rng.integers(0, size // 3, n),
+                        rng.integers(0, size, n)], 1).astype(np.uint16)
+        rgba = (rng.integers(1, 255, n) | (rng.integers(0, 2**24, n) << 8)).astype(np.uint32)
+        words, root = S.build_octree(xyz, rgba, D)
+        ctx.upload_octree(words, root, D)
+        ctx.resize(96, 64)
+        o = O.Frame(96, 64, words, root, D, nthreads=8)
+        for f in range(3):
+            p, _ = S.camera(0.4 + 0.01 * f, 0.3, 0.0, (size / 16 * 0.3, size / 16 * 0.5, size / 16 * 0.2), D, f)
+            ctx.frame_warp(p)
+            o.frame_warp(O.params_from(p))
+            ctx.sync()
+            _compare_all(ctx, o, f"D={D} frame {f}")
+        o.close()
+
+
+def test_error_paths(ctx):
+    with pytest.raises(S.SvoError):
+        ctx.resize(8, 8)
+    with pytest.raises(S.SvoError):
+        ctx.upload_octree(np.zeros(5, np.uint32), 0, 8)
+    bad = np.zeros(2097152 + 64, np.uint32)
+    bad[0] = (5 << 9) | 256 | 1  # a block ref directly under the root (blocks start at depth D-6)
+    with pytest.raises(S.SvoError):
+        ctx.upload_octree(bad, 0x01, 8)
+    with pytest.raises(S.SvoError):
+        ctx.proj(0, S.FrameParams())
+    with pytest.raises(S.SvoError):
+        ctx.set_tuning(refill=65)

@@ -222,93 +222,100 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, uint32_t H, const R
 #define STK(rek) stk[(rek) * RAY_BLOCK + tid]
 
 // One trip of the CAST_RAY do-while loop (kernel.cl:152-211).  Returns true
-// when the loop exits.  The child fetch of descending lanes is issued first
-// (fetchChildOffset, kernel.cl:32-62, with its single word read hoisted), the
-// DDA step of the other lanes runs while that read is in flight, and the
-// descend completes afterwards.
+// when the loop exits.  A lone wave in the tail of a launch issues one
+// instruction per 4 cycles, so the trip is written for a short instruction
+// path: selects instead of exec-mask branches, the pop test as one compare
+// (floor(log2 m) >= rek  <=>  m >= 2^rek, m > 0), and the child fetch of
+// fetchChildOffset (kernel.cl:32-62) as a single word read that is issued
+// before the DDA step of the other lanes and consumed after it.
+template <bool kCache>
 __device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, OctBuf ob, uint32_t* stk, const uint32_t* cache,
                                          int tid)
 {
     const int D = a.depth;
-    const int DAND = (1 << D) - 1;
-    if (++r.guard > SVO_MAX_RAY_ITERS) return true;
-    const int p0x = r.ix >> r.rek, p0y = r.iy >> r.rek, p0z = r.iz >> r.rek;
+    const int rek = r.rek;
+    const int p0x = r.ix >> rek, p0y = r.iy >> rek, p0z = r.iz >> rek;
     const int ni = ((p0x & 1) | ((p0y & 1) << 1) | ((p0z & 1) << 2)) ^ r.sxyz;
-    const uint32_t test = r.node & (1u << ni);
+    const uint32_t node = r.node;
+    const uint32_t test = node & (1u << ni);
     r.test = test;
 
-    // ---- issue the child fetch.  At rek == 1 the fetched word is never
-    // consumed (the hit exits and fetchColor reads only node_before), so no
-    // read is made there.
+    // ---- descend, part 1: address and read (kernel.cl:42-61).  At rek == 1
+    // the fetched word is never consumed (the hit exits and fetchColor reads
+    // only node_before and the stack); the read there stays inside the block.
     uint32_t w = 0, nadd = 0;
     bool synth = false;
-    if (test && r.rek >= 2) {
-        uint32_t n = r.node >> 9;
-        uint32_t idx;
-        if (r.node & 256u) {
-            if (!(r.before & 256u)) {
-                r.lroot = n << 6;
-                n = 0;
-            }
-            n += r.lroot;
-            nadd = bc8(r.node & ((test << 1) - 1));
-            synth = (r.rek == 2);
-            idx = synth ? n + (nadd >> 2) : n + nadd;
-        } else {
-            idx = n + (uint32_t)ni;
-        }
-        // normal records of the top levels sit in the LDS prefix
-        if (idx < a.cache_words)
+    if (test) {
+        const uint32_t n0 = node >> 9;
+        const bool blk = (node & 256u) != 0;
+        const bool enter = blk && !(r.before & 256u);
+        r.lroot = enter ? (n0 << 6) : r.lroot;
+        nadd = (uint32_t)__popc(node & ((test << 1) - 1u));
+        synth = blk && rek == 2;
+        const uint32_t base = (enter ? 0u : n0) + r.lroot;
+        const uint32_t idx = blk ? base + (synth ? (nadd >> 2) : nadd) : n0 + (uint32_t)ni;
+        if (kCache && idx < a.cache_words)
             w = cache[idx];
         else
             w = oct_load(ob, idx);
     }
 
+    bool done = false;
     if (!test) {
         // DDA step to the nearest cell face (kernel.cl:181-194)
-        const float mx = (float)(int)((uint32_t)(p0x + 1) << r.rek) - r.px;
-        const float my = (float)(int)((uint32_t)(p0y + 1) << r.rek) - r.py;
-        const float mz = (float)(int)((uint32_t)(p0z + 1) << r.rek) - r.pz;
-        float dist = mx * r.lx;
-        const float ddy = my * r.ly, ddz = mz * r.lz;
-        float sx = mx, sy = r.g0y * mx, sz = r.g0z * mx;
-        if (ddy < dist) { dist = ddy; sx = r.g1x * my; sy = my; sz = r.g1z * my; }
-        if (ddz < dist) { dist = ddz; sx = r.g2x * mz; sy = r.g2y * mz; sz = mz; }
-        r.px += sx; r.py += sy; r.pz += sz;
+        const float mx = (float)((p0x + 1) << rek) - r.px;
+        const float my = (float)((p0y + 1) << rek) - r.py;
+        const float mz = (float)((p0z + 1) << rek) - r.pz;
+        const float ex = mx * r.lx, ey = my * r.ly, ez = mz * r.lz;
+        const bool c1 = ey < ex;
+        float dist = c1 ? ey : ex;
+        const bool c2 = ez < dist;
+        dist = c2 ? ez : dist;
+        const float m = c2 ? mz : (c1 ? my : mx);
+        const float gx = c2 ? r.g2x : (c1 ? r.g1x : 1.0f);
+        const float gy = c2 ? r.g2y : (c1 ? 1.0f : r.g0y);
+        const float gz = c2 ? 1.0f : (c1 ? r.g1z : r.g0z);
+        r.px += gx * m;
+        r.py += gy * m;
+        r.pz += gz * m;
         const int nx = (int)r.px, ny = (int)r.py, nz = (int)r.pz;
         r.dist += dist;
-        // pop level (kernel.cl:197-208)
+        // pop level (kernel.cl:197-209)
+        const int DAND = (1 << D) - 1;
         const int x0ry = r.iy ^ ny;
-        const int x0r = (r.ix ^ nx) ^ x0ry ^ (r.iz ^ nz);
-        r.ix = nx; r.iy = ny; r.iz = nz;
-        const uint32_t m = (uint32_t)(x0r & DAND);
-        const int rek_new = m ? 31 - __clz((int)m) : -1;
-        if (r.dist > 400000.0f) return true;
-        if (rek_new >= r.rek) {
-            r.rek = rek_new + 1;
-            if (r.rek >= D) {
-                r.node = a.root;
-                r.before = a.root;
-            } else {
-                r.node = STK(r.rek);
-                // slot D belongs to no node of this ray; the reference reads a
-                // neighbour's slot there, a value consulted only for block
-                // nodes, and depth-1 nodes are normal nodes for D >= 8
-                r.before = (r.rek + 1 <= D - 1) ? STK(r.rek + 1) : a.root;
-            }
-            if (r.dist > r.lodsw) { r.lodsw *= 2.0f; ++r.lod; }
+        const uint32_t xm = (uint32_t)((r.ix ^ nx) ^ x0ry ^ (r.iz ^ nz)) & (uint32_t)DAND;
+        r.ix = nx;
+        r.iy = ny;
+        r.iz = nz;
+        const bool far = r.dist > 400000.0f;
+        if (!far && xm >= (1u << rek)) {
+            const int nr = 32 - __clz((int)xm);  // rek_new + 1
+            r.rek = nr;
+            const uint32_t sn = STK(min(nr, D - 1));
+            const uint32_t sb = STK(min(nr + 1, D - 1));
+            // rows D and D+1 belong to no node of this ray; the reference
+            // reads a neighbour's slots there, values overwritten by the root
+            // at rek == D and consulted only for block nodes at rek == D-1
+            // (depth-1 nodes are normal nodes for D >= 8)
+            r.node = nr >= D ? a.root : sn;
+            r.before = nr + 1 >= D ? a.root : sb;
+            const bool l = r.dist > r.lodsw;
+            r.lodsw = l ? r.lodsw * 2.0f : r.lodsw;
+            r.lod += l ? 1 : 0;
         }
-        return (x0ry & (2 * DAND + 2)) != 0;
+        done = far || (x0ry & (2 * DAND + 2)) != 0;
+    } else {
+        // ---- descend, part 2 (kernel.cl:58, :169-178)
+        const uint32_t sb = (w >> ((nadd & 3u) << 3)) & 255u;
+        r.node = synth ? sb + 256u + (nadd << 9) : w;
+        r.before = node;
+        done = rek <= r.lod;
+        if (!done) {
+            r.rek = rek - 1;
+            STK(rek - 1) = r.node;
+        }
     }
-
-    // ---- complete the descend (kernel.cl:162-178)
-    const uint32_t child = synth ? ((w >> ((nadd & 3) << 3)) & 255u) + 256u + (nadd << 9) : w;
-    r.before = r.node;
-    r.node = (r.rek >= 2) ? child : r.node;
-    if (r.rek <= r.lod) return true;
-    r.rek--;
-    STK(r.rek) = r.node;
-    return false;
+    return done;
 }
 
 __device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, OctBuf ob, uint32_t* stk, const uint32_t* cache,
@@ -330,6 +337,7 @@ __device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, OctBuf ob, 
     if (a.image) a.image[r.pix] = colorize_word(word);
 }
 
+template <bool kCache>
 __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
 {
     extern __shared__ uint32_t smem[];
@@ -369,12 +377,22 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
             continue;
         }
         if (active) {
-            for (int it = 0; it < a.iters; ++it) {
-                if (step_ray(r, a, ob, stk, cache, tid)) {
-                    finish_ray(r, a, ob, stk, cache, tid);
-                    active = false;
+            // at most SVO_MAX_RAY_ITERS trips per ray; a ray still running
+            // then finishes at its current node
+            const uint32_t lim = min((uint32_t)a.iters, SVO_MAX_RAY_ITERS - r.guard);
+            uint32_t it = 0;
+            bool done = false;
+            for (; it < lim; ++it) {
+                if (step_ray<kCache>(r, a, ob, stk, cache, tid)) {
+                    done = true;
+                    ++it;
                     break;
                 }
+            }
+            r.guard += it;
+            if (done || r.guard >= SVO_MAX_RAY_ITERS) {
+                finish_ray(r, a, ob, stk, cache, tid);
+                active = false;
             }
         }
     }
@@ -618,8 +636,8 @@ struct svo_ctx {
     int launch_seq = 0;
     bool plane1_holes = false;
     // tuning
-    int refill = 16, wg_per_cu = 8, iters = 16;
-    uint32_t cache_req = 1024;  // requested LDS node-cache words
+    int refill = 16, wg_per_cu = 4, iters = 64;
+    uint32_t cache_req = 0;  // requested LDS node-cache words
     // timing
     bool timing = false;
     hipEvent_t ev[ST_N + 1] = {};
@@ -844,7 +862,10 @@ static int launch_rays(svo_ctx* c, const svo_frame_params* p, int use_holes, int
     a.oct_bytes = (uint32_t)std::min<size_t>((c->n_words + kOctPad) * 4, 0xfffffffcull);
     const unsigned grid = (unsigned)(c->n_cu * c->wg_per_cu);
     const size_t lds = ((size_t)c->depth * RAY_BLOCK + std::max(a.cache_words, 1u)) * 4;
-    hipLaunchKernelGGL(k_rays, dim3(grid), dim3(RAY_BLOCK), lds, c->stream, a);
+    if (a.cache_words)
+        hipLaunchKernelGGL(k_rays<true>, dim3(grid), dim3(RAY_BLOCK), lds, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_rays<false>, dim3(grid), dim3(RAY_BLOCK), lds, c->stream, a);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }

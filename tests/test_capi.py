@@ -1,0 +1,81 @@
+"""The C ABI: libsvo.so loads on a host without a GPU and exports every
+function include/svo.h declares; the declarations carry no torch/HIP types;
+host-only helpers behave (no device calls here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import svo_amd as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "svo.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(svo_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_symbols_exported():
+    lib = C.CDLL(S.LIB_PATH)
+    names = declared()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(S.EXPORTED) == names
+
+
+def test_header_is_plain_c():
+    src = open(HEADER).read()
+    for bad in ("torch", "hip", "Tensor", "cuda", "std::"):
+        assert bad not in re.sub(r"/\*.*?\*/", "", src, flags=re.S), bad
+
+
+def test_last_error_and_host_errors():
+    with pytest.raises(S.SvoError) as e:
+        S.tile_schedule(0, 10, 0)
+    assert "tile" in str(e.value)
+    with pytest.raises(S.SvoError):
+        S.scene(9, 1, 8)
+    with pytest.raises(S.SvoError):
+        S.camera(0, 0, 0, (float("nan"), 0, 0), 8, 0)
+
+
+def test_scene_determinism_and_ranges():
+    a1, c1 = S.scene(1, 1001, 8)
+    a2, c2 = S.scene(1, 1001, 8)
+    assert np.array_equal(a1, a2) and np.array_equal(c1, c2)
+    a3, _ = S.scene(1, 1002, 8)
+    assert not np.array_equal(a1[:1000], a3[:1000])
+    assert a1.max() < 256 and c1.min() > 0
+    # heights of the heightfield scene lie in [32, 224] (32 + 192*v)
+    top = {}
+    for x, y, z in a1:
+        top[(x, z)] = max(top.get((x, z), 0), y)
+    h = np.array(list(top.values()))
+    assert h.min() >= 32 and h.max() <= 224
+    # stream order z -> x -> y
+    key = a1[:, 2].astype(np.int64) * 65536 + a1[:, 0].astype(np.int64) * 256 + a1[:, 1]
+    assert np.all(np.diff(key) > 0)
+    # colour byte: palette 1 in bits 0-1
+    assert np.all((c1 & 3) == 1)
+
+
+def test_scene_spheres():
+    xyz, rgba = S.scene(2, 7, 9, 20)
+    assert len(xyz) > 0 and xyz.max() < 512
+    assert np.all(rgba != 0)
+
+
+def test_camera_path_bounds():
+    for seed, depth, n, vox, yaw in [(2001, 8, 64, 2.0, 0.5), (2002, 11, 300, 8.0, 1.0), (5, 11, 120, 16.0, 3.0)]:
+        path = S.camera_path(seed, depth, n, vox, yaw, 100.0, 0.3)
+        assert path.shape == (n, 6)
+        step = np.linalg.norm(np.diff(path[:, 3:], axis=0), axis=1) * 8   # voxels
+        assert step.max() <= vox
+        assert np.degrees(np.abs(np.diff(path[:, 0]))).max() <= yaw
+        assert np.all(path[:, 1] == np.float32(0.3))

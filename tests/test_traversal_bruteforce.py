@@ -1,0 +1,101 @@
+"""The oracle's CAST_RAY restatement (kernel.cl:114-214, fetchChildOffset
+:32-62, fetchColor :64-112) against an independent FP64 voxel-grid DDA
+(Amanatides-Woo) on a small dense scene: for rays that hit within the LOD-1
+range, the hit point lies on the first voxel the FP64 march enters and the
+returned colour byte is that voxel's leaf byte."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import svo_amd as S
+
+D = 8
+SIZE = 1 << D
+
+
+@pytest.fixture(scope="module")
+def scene():
+    rng = np.random.default_rng(77)
+    occ = np.zeros((SIZE, SIZE, SIZE), bool)          # [x, y, z]
+    col = np.zeros((SIZE, SIZE, SIZE), np.uint32)
+    occ[:, 0:4, :] = True
+    col[:, 0:4, :] = rng.integers(1, 2**32, (SIZE, 4, SIZE), dtype=np.uint64).astype(np.uint32) | 1
+    for _ in range(40):
+        x0, z0 = rng.integers(0, SIZE - 40, 2)
+        y0 = rng.integers(4, 120)
+        w, h, d = rng.integers(3, 40, 3)
+        occ[x0:x0 + w, y0:y0 + h, z0:z0 + d] = True
+        col[x0:x0 + w, y0:y0 + h, z0:z0 + d] = int(rng.integers(1, 2**32)) | 1
+    xs, ys, zs = np.nonzero(occ)
+    order = np.lexsort((ys, xs, zs))                   # stream order z -> x -> y
+    xyz = np.stack([xs, ys, zs], 1)[order].astype(np.uint16)
+    rgba = col[xs, ys, zs][order]
+    words, root = S.build_octree(xyz, rgba, D)
+    return occ, col, words, root
+
+
+def ray_dir32(p, x, y, rx, ry):
+    f32 = np.float32
+    d1x = (f32(x - rx // 2) + f32(0.5)) * f32(p.fovx) / f32(ry)
+    d1y = (f32(y - ry // 2) + f32(0.5)) * f32(p.fovy) / f32(ry)
+    cols = [(p.col0.x, p.col0.y, p.col0.z), (p.col1.x, p.col1.y, p.col1.z), (p.col2.x, p.col2.y, p.col2.z)]
+    return [float((d1x * f32(c[0]) + d1y * f32(c[1])) + f32(1) * f32(c[2])) for c in cols]
+
+
+def march(occ, o, d, max_dist):
+    """First occupied voxel (2 ray units per voxel, x and z wrapping) along
+    o + t*d in FP64; None if none within max_dist or the ray leaves in y."""
+    n = math.sqrt(sum(v * v for v in d))
+    cell = [math.floor(o[i] / 2) for i in range(3)]
+    step = [1 if d[i] > 0 else -1 for i in range(3)]
+    tmax, tdel = [], []
+    for i in range(3):
+        if d[i] == 0:
+            tmax.append(math.inf); tdel.append(math.inf); continue
+        nb = (cell[i] + (1 if d[i] > 0 else 0)) * 2
+        tmax.append((nb - o[i]) / d[i]); tdel.append(2 / abs(d[i]))
+    t = 0.0
+    while t * n <= max_dist:
+        if not (0 <= cell[1] < SIZE):
+            return None
+        if occ[cell[0] % SIZE, cell[1], cell[2] % SIZE]:
+            return cell, t
+        a = int(np.argmin(tmax))
+        t = tmax[a]
+        cell[a] += step[a]
+        tmax[a] += tdel[a]
+    return None
+
+
+def test_hits_match_fp64_dda(scene):
+    occ, col, words, root = scene
+    rng = np.random.default_rng(5)
+    rx, ry = 2048, 1024                    # lodswitch = 4096 ray units
+    checked = agree = 0
+    for cam in range(4):
+        yaw = float(rng.uniform(-3, 3))
+        pitch = float(rng.uniform(0.25, 0.7))
+        pos = (float(rng.uniform(0, 32)), float(rng.uniform(17, 24)), float(rng.uniform(0, 32)))
+        p, wr = O.camera(yaw, pitch, 0.0, pos, D, 0)
+        o = [float(np.float32(v) * np.float32(16)) for v in wr]
+        for _ in range(120):
+            x, y = int(rng.integers(0, rx)), int(rng.integers(0, ry))
+            d = ray_dir32(p, x, y, rx, ry)
+            if min(abs(v) for v in d) < 2**-20:
+                continue
+            hit = march(occ, o, d, 3000.0)
+            if hit is None:
+                continue
+            cell, t = hit
+            word, xyz, st = O.trace_pixel(words, root, D, rx, ry, p, x, y)
+            checked += 1
+            pt = [v * 16.0 for v in xyz]
+            on_box = all(cell[i] * 2 - 1e-3 <= pt[i] + (SIZE * 2 * round((cell[i] * 2 - pt[i]) / (SIZE * 2)) if i != 1 else 0)
+                         <= cell[i] * 2 + 2 + 1e-3 for i in range(3))
+            colour_ok = ((word - 0xFF000000) & 0xFF) == (int(col[cell[0] % SIZE, cell[1], cell[2] % SIZE]) & 0xFF)
+            agree += bool(on_box and colour_ok)
+    print(f'checked {checked} agree {agree}')
+    assert checked > 300
+    assert agree >= 0.99 * checked, (agree, checked)

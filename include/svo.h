@@ -191,6 +191,42 @@ const char* svo_stage_name(int stage);
 int svo_set_tuning(svo_ctx* ctx, int refill_threshold, int wg_per_cu,
                    int iters_per_refill, int lds_cache_words);
 
+/* ---- learned hole filler (no counterpart in the reference) ------------
+ * A 4-layer 3x3 CNN (5 -> 32 -> 32 -> 32 -> 3 channels, ReLU) run with BF16
+ * MFMA kernels accumulating in FP32, trained with Adam on FP32 master
+ * weights.  Parameters are a flat float vector: per layer the weights
+ * [9 taps][cout][cin] (channels padded to 16/32/32/16 in, 32/32/32/16 out;
+ * padded entries stay zero) followed by the bias [cout]. */
+typedef struct svo_cnn svo_cnn;
+int svo_cnn_create(svo_cnn** out, int device, uint64_t seed);
+int svo_cnn_destroy(svo_cnn* cnn);
+size_t svo_cnn_param_count(void);
+int svo_cnn_get_params(svo_cnn* cnn, float* dst, size_t count);
+int svo_cnn_set_params(svo_cnn* cnn, const float* src, size_t count);
+/* x: [batch][h][w][16] bf16 bit patterns (h, w multiples of 8), zero
+ * "same" padding at crop borders; out: [batch][h][w][16] float (channels
+ * 0..2 = RGB in 0..1). */
+int svo_cnn_forward(svo_cnn* cnn, const uint16_t* x, int batch, int h, int w, float* out);
+/* loss = sum mask*(out-target)^2 / (3*sum mask) over RGB; writes the
+ * gradient of every parameter (same layout as the parameter vector). */
+int svo_cnn_gradients(svo_cnn* cnn, const uint16_t* x, const float* target, const float* mask,
+                      int batch, int h, int w, float* grads, size_t count, float* loss);
+
+/* One Adam step (learning rate lr) on n_crops random crop x crop windows
+ * (origins drawn from seed) of the frame pair the two contexts currently
+ * hold: features from `warp`'s pre-filter planes, targets from `full`'s
+ * image, loss on warp-frame hole pixels.  Both contexts must have rendered the
+ * same camera at one resolution.  loss_out (optional) makes the call block
+ * and returns the step's loss. */
+int svo_cnn_train_step(svo_cnn* cnn, svo_ctx* warp, svo_ctx* full, const svo_frame_params* p,
+                       int n_crops, int crop, uint64_t seed, float lr, float* loss_out);
+/* svo_frame_warp with the learned fill and per-tile confidence gate: after
+ * the hole and tile rays, each 16x16 tile with more than max_cnn_holes hole
+ * pixels has them raycast; tiles with fewer are filled by the CNN (image
+ * only; screen words keep fillhole2's result).  cnn may be NULL: the gate
+ * alone, fillhole2 colours everywhere. */
+int svo_frame_warp_learned(svo_ctx* ctx, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
+
 #ifdef __cplusplus
 }
 #endif

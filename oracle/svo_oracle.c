@@ -809,6 +809,51 @@ void orc_frame_warp(orc_state* s, const orc_params* p, int inplace_filter)
     orc_colorize(s);
 }
 
+/* Gated warp frame (no reference counterpart): after the hole and tile rays,
+ * every 16x16 tile (partial tiles included) with more than max_holes hole
+ * pixels has those pixels raycast, in tile order; the CNN part of the learned
+ * fill is not restated here. */
+void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes)
+{
+    const int rx = s->res_x, ry = s->res_y;
+    size_t N = (size_t)rx * ry;
+    if (p->frame < 2) orc_memset_screen(s, 0, SCREEN_HOLE, 4 * N);
+    orc_memset_screen(s, 0, SCREEN_HOLE, N);
+    orc_proj(s, 1, p);
+    orc_proj(s, 2, p);
+    orc_counthole(s);
+    orc_sumids(s);
+    orc_writeids(s);
+    if (s->ids[0] > 0) orc_raycast_holes(s, p);
+    int ax, ay, ex, ey;
+    orc_tile_schedule(rx, ry, p->frame, &ax, &ay, &ex, &ey);
+    orc_raycast_rect(s, p, ax, ay, ex, ey);
+    int tx = (rx + 15) / 16, ty = (ry + 15) / 16;
+    uint32_t* list = (uint32_t*)malloc(N * 4);
+    size_t n = 0;
+    for (int t = 0; t < tx * ty; t++) {
+        int x0 = (t % tx) * 16, y0 = (t / tx) * 16, h = 0;
+        for (int j = 0; j < 16; j++)
+            for (int i = 0; i < 16; i++)
+                if (x0 + i < rx && y0 + j < ry && s->screen[(size_t)(y0 + j) * rx + x0 + i] == SCREEN_HOLE) h++;
+        if (h <= max_holes) continue;
+        for (int j = 0; j < 16; j++)
+            for (int i = 0; i < 16; i++)
+                if (x0 + i < rx && y0 + j < ry && s->screen[(size_t)(y0 + j) * rx + x0 + i] == SCREEN_HOLE)
+                    list[n++] = (uint32_t)(x0 + i) | ((uint32_t)(y0 + j) << 16);
+    }
+    for (size_t k = 0; k < n; k++) {
+        int x = (int)(list[k] & 0xffff), y = (int)(list[k] >> 16);
+        float xyz[3];
+        uint32_t w = orc_trace_pixel(s->oct, s->n_words, s->root, s->depth, rx, ry, p, x, y, xyz, &s->st);
+        write_ray(s, (size_t)y * rx + x, w, xyz);
+    }
+    free(list);
+    orc_copy_plane(s, 2);
+    orc_fillhole2_snapshot(s, s->screen + 2 * N);
+    orc_colorize(s);
+}
+
 void orc_frame_full(orc_state* s, const orc_params* p)
 {
     orc_raycast_rect(s, p, 0, 0, s->res_x, s->res_y);

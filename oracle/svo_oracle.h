@@ -115,6 +115,7 @@ void orc_tile_schedule(int res_x, int res_y, int frame, int* add_x, int* add_y,
  * inplace_filter == 0) and full raycast (raycast_fine_2 over the screen). */
 void orc_frame_warp(orc_state* s, const orc_params* p, int inplace_filter);
 void orc_frame_full(orc_state* s, const orc_params* p);
+void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes);
 
 /* exhaustive-check helpers */
 uint32_t orc_bitcount8(uint32_t b);

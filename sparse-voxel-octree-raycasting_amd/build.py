@@ -36,12 +36,15 @@ def run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     host_o = os.path.join(BUILD, "svo_host.o")
-    dev_o = os.path.join(BUILD, "svo_device.o")
     run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "svo_host.cpp"), "-o", host_o])
-    run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "svo_device.hip"), "-o", dev_o])
+    objs = [host_o]
+    for src in ("svo_device.hip", "svo_cnn.hip"):
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
     out = os.path.join(OUT_DIR, "libsvo.so")
     tmp = out + ".tmp"
-    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", host_o, dev_o, "-o", tmp, "-pthread"])
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-pthread"])
     os.replace(tmp, out)
     return out
 

@@ -1,0 +1,759 @@
+// svo_cnn.hip -- learned hole filler: a 4-layer 3x3 CNN with hand-written
+// BF16 MFMA forward / backward kernels (FP32 accumulate) and Adam, trained on
+// (warped frame, full raycast) pairs the pipeline produces itself.  No
+// counterpart in the reference (it only has the fixed filter of
+// kernel.cl:404-470); the plain warp path is unchanged by this file.
+//
+// Network: conv3x3 5->32, ReLU, 32->32, ReLU, 32->32, ReLU, 32->3 (linear),
+// zero "same" padding at crop borders.  Channels are padded to multiples of
+// 16 (input 5 -> 16, output 3 -> 16); padded weights stay zero.
+// Input features per pixel of the pre-filter warped frame: RGB of the
+// colorized screen word (0 at holes), depth z/(z+64) (0 at holes), hole mask.
+//
+// Tensors are NHWC bf16 over a batch of crops.  One wave computes an 8x8
+// output tile: 4 M-tiles of 16 pixels x (COUT/16) N-tiles with
+// v_mfma_f32_16x16x16_bf16 (A = 16 pixels x 16 input channels from LDS,
+// B = 16 input x 16 output channels from the weights, K = 9 taps x CIN).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "svo_ctx.h"
+#include "svo_internal.h"
+
+namespace {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+__device__ __forceinline__ bf16 f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr uint32_t HOLE_WORD = 0xffffff00u;
+constexpr int TILE = 8;          // output tile edge per wave
+constexpr int HALO = TILE + 2;   // input tile edge
+
+// MODE 0: + bias, ReLU, bf16 out.  MODE 1: + bias, fp32 out (last layer).
+// MODE 2: no bias, multiplied by (mask_src > 0), bf16 out (backward data).
+// Work items (crop, 8x8 tile) are taken grid-stride; the crop count is B, or
+// min(*bcount, B) when bcount is given (batches sized on the device).
+template <int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(64) void k_conv(const bf16* __restrict__ X, const bf16* __restrict__ Wt,
+                                             const float* __restrict__ bias, const bf16* __restrict__ mask_src,
+                                             void* Yout, int H, int W, int B, const uint32_t* bcount)
+{
+    constexpr int CS = CIN + 4;  // LDS pixel stride (bank spread)
+    __shared__ __align__(16) bf16 xs[HALO * HALO * CS];
+    const int lane = threadIdx.x;
+    const int tiles_x = W / TILE, tiles_y = H / TILE;
+    const int nb = bcount ? min((int)*bcount, B) : B;
+    for (int item = blockIdx.x; item < nb * tiles_x * tiles_y; item += gridDim.x) {
+    const int b = item / (tiles_x * tiles_y);
+    const int t = item % (tiles_x * tiles_y);
+    const int y0 = (t / tiles_x) * TILE, x0 = (t % tiles_x) * TILE;
+    __syncthreads();  // previous item's LDS reads are done
+    // stage the (TILE+2)^2 x CIN input halo, zero outside the crop
+    for (int i = lane; i < HALO * HALO * (CIN / 4); i += 64) {
+        const int pix = i / (CIN / 4), c4 = (i % (CIN / 4)) * 4;
+        const int yy = y0 + pix / HALO - 1, xx = x0 + pix % HALO - 1;
+        uint2 v = make_uint2(0u, 0u);
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+            v = *(const uint2*)(X + (((size_t)b * H + yy) * W + xx) * CIN + c4);
+        *(uint2*)(xs + pix * CS + c4) = v;
+    }
+    __syncthreads();
+
+    constexpr int NT = COUT / 16;
+    f4v acc[4][NT];
+    for (int m = 0; m < 4; m++)
+        for (int n = 0; n < NT; n++) acc[m][n] = f4v{0.f, 0.f, 0.f, 0.f};
+    const int row = lane & 15, kq = (lane >> 4) * 4;
+    for (int tap = 0; tap < 9; tap++) {
+        const int dy = tap / 3, dx = tap % 3;
+        for (int kc = 0; kc < CIN / 16; kc++) {
+            s4v bfr[NT];
+            for (int n = 0; n < NT; n++)
+                bfr[n] = *(const s4v*)(Wt + ((size_t)tap * COUT + n * 16 + row) * CIN + kc * 16 + kq);
+            for (int m = 0; m < 4; m++) {
+                const int p = m * 16 + row, py = p / TILE, px = p % TILE;
+                const s4v afr = *(const s4v*)(xs + ((py + dy) * HALO + px + dx) * CS + kc * 16 + kq);
+                for (int n = 0; n < NT; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[n], acc[m][n], 0, 0, 0);
+            }
+        }
+    }
+    // D layout: lane holds pixels 4*(lane/16)+i of the M-tile, channel lane%16
+    for (int m = 0; m < 4; m++)
+        for (int n = 0; n < NT; n++) {
+            const int co = n * 16 + row;
+            for (int i = 0; i < 4; i++) {
+                const int p = m * 16 + kq + i, py = p / TILE, px = p % TILE;
+                const size_t o = (((size_t)b * H + y0 + py) * W + x0 + px) * COUT + co;
+                float v = acc[m][n][i];
+                if (MODE == 0) {
+                    v += bias[co];
+                    v = v > 0.f ? v : 0.f;
+                    ((bf16*)Yout)[o] = f2bf(v);
+                } else if (MODE == 1) {
+                    ((float*)Yout)[o] = v + bias[co];
+                } else {
+                    // ReLU derivative: the forward output (>= 0) is positive
+                    ((bf16*)Yout)[o] = mask_src[o] != 0 ? f2bf(v) : (bf16)0;
+                }
+            }
+        }
+    }
+}
+
+// dW[tap][co][ci] += sum_p dZ[p][co] X[p + tap - 1][ci]; db[co] += sum_p dZ[p][co].
+// grid = (crops, 9 taps); one wave loops over the crop's pixels, K = 16 pixels
+// per MFMA (A = dZ^T: co x pixels, B = shifted X: pixels x ci).
+template <int CIN, int COUT>
+__global__ __launch_bounds__(64) void k_conv_dw(const bf16* __restrict__ X, const bf16* __restrict__ dZ,
+                                                float* dW, int H, int W)
+{
+    const int b = blockIdx.x, tap = blockIdx.y;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int lane = threadIdx.x, row = lane & 15, kq = (lane >> 4) * 4;
+    constexpr int MT = COUT / 16, NT = CIN / 16;
+    f4v acc[MT][NT];
+    for (int m = 0; m < MT; m++)
+        for (int n = 0; n < NT; n++) acc[m][n] = f4v{0.f, 0.f, 0.f, 0.f};
+    const int npix = H * W;
+    const bf16* dz = dZ + (size_t)b * npix * COUT;
+    const bf16* xb = X + (size_t)b * npix * CIN;
+    for (int k0 = 0; k0 < npix; k0 += 16) {
+        s4v afr[MT], bfr[NT];
+        for (int m = 0; m < MT; m++) {
+            short tmp[4];
+            for (int j = 0; j < 4; j++) tmp[j] = (short)dz[(size_t)(k0 + kq + j) * COUT + m * 16 + row];
+            afr[m] = s4v{tmp[0], tmp[1], tmp[2], tmp[3]};
+        }
+        for (int n = 0; n < NT; n++) {
+            short tmp[4];
+            for (int j = 0; j < 4; j++) {
+                const int p = k0 + kq + j, y = p / W + dy, x = p % W + dx;
+                tmp[j] = (y >= 0 && y < H && x >= 0 && x < W) ? (short)xb[((size_t)y * W + x) * CIN + n * 16 + row] : 0;
+            }
+            bfr[n] = s4v{tmp[0], tmp[1], tmp[2], tmp[3]};
+        }
+        for (int m = 0; m < MT; m++)
+            for (int n = 0; n < NT; n++)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr[m], bfr[n], acc[m][n], 0, 0, 0);
+    }
+    for (int m = 0; m < MT; m++)
+        for (int n = 0; n < NT; n++)
+            for (int i = 0; i < 4; i++) {
+                const int co = m * 16 + kq + i, ci = n * 16 + row;
+                atomicAdd(&dW[((size_t)tap * COUT + co) * CIN + ci], acc[m][n][i]);
+            }
+}
+
+// db[co] += sum_p dZ[p][co]; each block sums 4096 pixels
+template <int COUT>
+__global__ __launch_bounds__(256) void k_conv_db(const bf16* __restrict__ dZ, float* db, int npix)
+{
+    const int co = threadIdx.x % COUT, lane_p = threadIdx.x / COUT;
+    const int p0 = blockIdx.x * 4096, p1 = min(p0 + 4096, npix);
+    float s = 0.f;
+    for (int p = p0 + lane_p; p < p1; p += 256 / COUT) s += bf2f(dZ[(size_t)p * COUT + co]);
+    atomicAdd(&db[co], s);
+}
+
+// masked MSE on RGB: loss += sum mask*(pred-t)^2 / (3*count); dZ = d loss/d pred
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ pred, const float* __restrict__ target,
+                                              const float* __restrict__ mask, bf16* dZ, float* loss, int npix,
+                                              const float* __restrict__ mask_count)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    const float inv_count = *mask_count > 0.f ? 1.f / (3.f * *mask_count) : 0.f;
+    float l = 0.f;
+    for (int c = 0; c < 16; c++) {
+        float g = 0.f;
+        if (c < 3) {
+            const float d = pred[(size_t)p * 16 + c] - target[(size_t)p * 3 + c];
+            l += mask[p] * d * d;
+            g = 2.f * mask[p] * d * inv_count;
+        }
+        dZ[(size_t)p * 16 + c] = f2bf(g);
+    }
+    if (l != 0.f) atomicAdd(loss, l * inv_count);
+}
+
+// Input features of crop c (origin from `origins`, or tile list -> tile*16-4):
+// RGB of the colorized pre-filter word (plane 2), camera depth z/(z+64) and
+// the hole mask; zero outside the screen and at holes.  With `image`, also
+// the target RGB (full raycast) and the loss mask (holes inside the screen),
+// counting mask pixels into *mask_count.
+__device__ __forceinline__ void rgb_of(uint32_t word, float& r, float& g, float& b)
+{
+    const int a = (int)word, c = a & 3;
+    const float i = (float)(a & (255 - 7));
+    float tr = 1.0f, tg = 1.0f, tb = 1.0f;
+    if (c == 1) { tg = 0.7f; tb = 0.3f; }
+    else if (c == 2) { tr = 1.5f; tg = 0.8f; tb = 0.1f; }
+    else if (c == 3) { tr = 0.2f; tg = 0.8f; tb = 0.2f; }
+    r = fminf((float)(int)(i * tr), 255.f) * (1.f / 255.f);
+    g = fminf((float)(int)(i * tg), 255.f) * (1.f / 255.f);
+    b = fminf((float)(int)(i * tb), 255.f) * (1.f / 255.f);
+}
+
+struct FeatArgs {
+    const uint32_t* words;   // plane 2 screen words (pre-filter)
+    const float4* coords;    // plane 2 coords
+    const uint32_t* image;   // full-raycast image (targets) or null
+    int res_x, res_y, crop;
+    const int2* origins;     // explicit crop origins, or null
+    const uint32_t* tiles;   // tile list (origin = tile*16 - 4), or null
+    int tiles_x;
+    const uint32_t* count;   // device crop count (tile list), or null
+    int max_crops;
+    float m0x, m0y, m0z, r20, r21, r22;
+    bf16* x;
+    float* target;
+    float* mask;
+    float* mask_count;
+};
+
+__global__ __launch_bounds__(256) void k_features(FeatArgs a)
+{
+    const int nb = a.count ? min((int)*a.count, a.max_crops) : a.max_crops;
+    const size_t per = (size_t)a.crop * a.crop;
+    float cnt = 0.f;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < (size_t)nb * per; i += (size_t)gridDim.x * 256) {
+        const int c = (int)(i / per), q = (int)(i % per);
+        int ox, oy;
+        if (a.origins) {
+            ox = a.origins[c].x;
+            oy = a.origins[c].y;
+        } else {
+            const int t = (int)a.tiles[c];
+            ox = (t % a.tiles_x) * 16 - 4;
+            oy = (t / a.tiles_x) * 16 - 4;
+        }
+        const int x = ox + q % a.crop, y = oy + q / a.crop;
+        float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        float tr = 0.f, tg = 0.f, tb = 0.f, m = 0.f;
+        if (x >= 0 && y >= 0 && x < a.res_x && y < a.res_y) {
+            const size_t o = (size_t)y * a.res_x + x;
+            const uint32_t w = a.words[o];
+            if (w == HOLE_WORD) {
+                f[4] = 1.f;
+                m = 1.f;
+            } else {
+                rgb_of(w, f[0], f[1], f[2]);
+                const float4 cc = a.coords[o];
+                const float qx = cc.x - a.m0x, qy = cc.y - a.m0y, qz = cc.z - a.m0z;
+                const float z = fmaxf((qx * a.r20 + qy * a.r21) + qz * a.r22, 0.f);
+                f[3] = z / (z + 64.f);
+            }
+            if (a.image) {
+                const uint32_t px = a.image[o];
+                tr = (float)((px >> 16) & 255u) * (1.f / 255.f);
+                tg = (float)((px >> 8) & 255u) * (1.f / 255.f);
+                tb = (float)(px & 255u) * (1.f / 255.f);
+            }
+        }
+        bf16* xo = a.x + i * 16;
+        for (int k = 0; k < 16; k++) xo[k] = k < 5 ? f2bf(f[k]) : (bf16)0;
+        if (a.target) {
+            a.target[i * 3 + 0] = tr;
+            a.target[i * 3 + 1] = tg;
+            a.target[i * 3 + 2] = tb;
+            a.mask[i] = m;
+            cnt += m;
+        }
+    }
+    if (a.target && cnt != 0.f) atomicAdd(a.mask_count, cnt);
+}
+
+// CNN RGB -> image at the hole pixels of the centre 16x16 of each tile crop
+__global__ __launch_bounds__(256) void k_write_fill(const float* __restrict__ out, const uint32_t* __restrict__ tiles,
+                                                    const uint32_t* __restrict__ count, int max_crops,
+                                                    const uint32_t* __restrict__ words, uint32_t* image, int res_x,
+                                                    int res_y, int tiles_x)
+{
+    const int nb = min((int)*count, max_crops);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nb * 256; i += gridDim.x * 256) {
+        const int c = i / 256, q = i % 256;
+        const int t = (int)tiles[c];
+        const int x = (t % tiles_x) * 16 + (q & 15), y = (t / tiles_x) * 16 + (q >> 4);
+        if (x >= res_x || y >= res_y) continue;
+        const size_t o = (size_t)y * res_x + x;
+        if (words[o] != HOLE_WORD) continue;
+        const float* v = out + ((size_t)c * 576 + (size_t)((q >> 4) + 4) * 24 + (q & 15) + 4) * 16;
+        const uint32_t r = (uint32_t)fminf(fmaxf(v[0] * 255.f + 0.5f, 0.f), 255.f);
+        const uint32_t g = (uint32_t)fminf(fmaxf(v[1] * 255.f + 0.5f, 0.f), 255.f);
+        const uint32_t b = (uint32_t)fminf(fmaxf(v[2] * 255.f + 0.5f, 0.f), 255.f);
+        image[o] = b | (g << 8) | (r << 16);
+    }
+}
+
+// W'[t][ci][co] = W[8-t][co][ci] (backward-data weights), bf16
+__global__ void k_flip(const float* __restrict__ W, bf16* Wf, int cin, int cout)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 9 * cin * cout) return;
+    const int t = i / (cin * cout), r = i % (cin * cout), ci = r / cout, co = r % cout;
+    Wf[i] = f2bf(W[((size_t)(8 - t) * cout + co) * cin + ci]);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ a, bf16* b, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+
+// Adam (FP32 master weights): m, v moments, bias-corrected step
+__global__ void k_adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
+                       float c1, float c2, float eps)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+}  // namespace
+
+// ====================================================================== host
+struct CnnLayer {
+    int cin, cout;
+    size_t nw;          // 9*cin*cout
+    float* w = nullptr; // fp32 master [9][cout][cin]
+    float* b = nullptr; // fp32 [cout]
+    float* gw = nullptr;
+    float* gb = nullptr;
+    float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;
+    bf16* wbf = nullptr;   // forward weights
+    bf16* wflip = nullptr; // backward-data weights [9][cin][cout]
+};
+
+struct svo_cnn {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    CnnLayer L[4];
+    int step = 0;
+    // activation workspace for the current batch size
+    size_t cap_pix = 0;
+    bf16* x0 = nullptr;    // [pix][16]
+    bf16* y[3] = {};       // [pix][32]
+    float* out = nullptr;  // [pix][16]
+    bf16* dz16 = nullptr;  // [pix][16]
+    bf16* dz32a = nullptr; // [pix][32]
+    bf16* dz32b = nullptr; // [pix][32]
+    float* target = nullptr;
+    float* mask = nullptr;
+    float* loss = nullptr;
+    float* mask_count = nullptr;
+    int2* origins = nullptr;
+    int max_origins = 0;
+};
+
+#define HIPCHECK(expr)                                                                    \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return svo_fail(SVO_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+static const int kCin[4] = {16, 32, 32, 32}, kCout[4] = {32, 32, 32, 16};
+
+static void free_ws(svo_cnn* n)
+{
+    (void)hipFree(n->x0);
+    for (auto& p : n->y) (void)hipFree(p);
+    (void)hipFree(n->out);
+    (void)hipFree(n->dz16);
+    (void)hipFree(n->dz32a);
+    (void)hipFree(n->dz32b);
+    (void)hipFree(n->target);
+    (void)hipFree(n->mask);
+    n->x0 = nullptr;
+    for (auto& p : n->y) p = nullptr;
+    n->out = nullptr;
+    n->dz16 = n->dz32a = n->dz32b = nullptr;
+    n->target = n->mask = nullptr;
+    n->cap_pix = 0;
+}
+
+static int ensure_ws(svo_cnn* n, size_t pix)
+{
+    if (pix <= n->cap_pix) return SVO_OK;
+    free_ws(n);
+    bool ok = hipMalloc(&n->x0, pix * 16 * 2) == hipSuccess && hipMalloc(&n->y[0], pix * 32 * 2) == hipSuccess &&
+              hipMalloc(&n->y[1], pix * 32 * 2) == hipSuccess && hipMalloc(&n->y[2], pix * 32 * 2) == hipSuccess &&
+              hipMalloc(&n->out, pix * 16 * 4) == hipSuccess && hipMalloc(&n->dz16, pix * 16 * 2) == hipSuccess &&
+              hipMalloc(&n->dz32a, pix * 32 * 2) == hipSuccess && hipMalloc(&n->dz32b, pix * 32 * 2) == hipSuccess &&
+              hipMalloc(&n->target, pix * 3 * 4) == hipSuccess && hipMalloc(&n->mask, pix * 4) == hipSuccess;
+    if (!ok) {
+        free_ws(n);
+        return svo_fail(SVO_ERR_NOMEM, "CNN workspace for %zu pixels", pix);
+    }
+    n->cap_pix = pix;
+    return SVO_OK;
+}
+
+static unsigned nb(size_t n) { return (unsigned)((n + 255) / 256); }
+
+static int refresh_bf16(svo_cnn* n)
+{
+    for (auto& l : n->L) {
+        hipLaunchKernelGGL(k_to_bf16, dim3(nb(l.nw)), dim3(256), 0, n->stream, l.w, l.wbf, (int)l.nw);
+        hipLaunchKernelGGL(k_flip, dim3(nb(l.nw)), dim3(256), 0, n->stream, l.w, l.wflip, l.cin, l.cout);
+    }
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+extern "C" int svo_cnn_destroy(svo_cnn* n)
+{
+    if (!n) return SVO_OK;
+    (void)hipSetDevice(n->device);
+    (void)hipStreamSynchronize(n->stream);
+    free_ws(n);
+    for (auto& l : n->L) {
+        for (float* p : {l.w, l.b, l.gw, l.gb, l.mw, l.vw, l.mb, l.vb}) (void)hipFree(p);
+        (void)hipFree(l.wbf);
+        (void)hipFree(l.wflip);
+    }
+    (void)hipFree(n->loss);
+    (void)hipFree(n->mask_count);
+    (void)hipFree(n->origins);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+    return SVO_OK;
+}
+
+extern "C" size_t svo_cnn_param_count(void)
+{
+    size_t s = 0;
+    for (int i = 0; i < 4; i++) s += 9 * (size_t)kCin[i] * kCout[i] + kCout[i];
+    return s;
+}
+
+extern "C" int svo_cnn_create(svo_cnn** out, int device, uint64_t seed)
+{
+    if (!out) return svo_fail(SVO_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    HIPCHECK(hipSetDevice(device));
+    svo_cnn* n = new svo_cnn();
+    n->device = device;
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&n->loss, 4) != hipSuccess || hipMalloc(&n->mask_count, 4) != hipSuccess) {
+        svo_cnn_destroy(n);
+        return svo_fail(SVO_ERR_HIP, "CNN set-up failed");
+    }
+    // He-normal init of the real channels (Box-Muller over a splitmix stream)
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int real_in[4] = {5, 32, 32, 32}, real_out[4] = {32, 32, 32, 3};
+    for (int li = 0; li < 4; li++) {
+        CnnLayer& l = n->L[li];
+        l.cin = kCin[li];
+        l.cout = kCout[li];
+        l.nw = 9 * (size_t)l.cin * l.cout;
+        std::vector<float> w(l.nw, 0.f), b(l.cout, 0.f);
+        const double sd = sqrt(2.0 / (9.0 * real_in[li])) * (li == 3 ? 0.5 : 1.0);
+        for (int t = 0; t < 9; t++)
+            for (int co = 0; co < real_out[li]; co++)
+                for (int ci = 0; ci < real_in[li]; ci++) {
+                    double u1 = std::max(next(), 1e-12), u2 = next();
+                    w[((size_t)t * l.cout + co) * l.cin + ci] = (float)(sd * sqrt(-2 * log(u1)) * cos(6.283185307179586 * u2));
+                }
+        bool ok = true;
+        for (float** p : {&l.w, &l.gw, &l.mw, &l.vw}) ok = ok && hipMalloc(p, l.nw * 4) == hipSuccess;
+        for (float** p : {&l.b, &l.gb, &l.mb, &l.vb}) ok = ok && hipMalloc(p, l.cout * 4) == hipSuccess;
+        ok = ok && hipMalloc(&l.wbf, l.nw * 2) == hipSuccess && hipMalloc(&l.wflip, l.nw * 2) == hipSuccess;
+        if (!ok) {
+            svo_cnn_destroy(n);
+            return svo_fail(SVO_ERR_NOMEM, "CNN weights");
+        }
+        (void)hipMemcpy(l.w, w.data(), l.nw * 4, hipMemcpyHostToDevice);
+        (void)hipMemcpy(l.b, b.data(), l.cout * 4, hipMemcpyHostToDevice);
+        for (float* p : {l.mw, l.vw}) (void)hipMemset(p, 0, l.nw * 4);
+        for (float* p : {l.mb, l.vb}) (void)hipMemset(p, 0, l.cout * 4);
+    }
+    int rc = refresh_bf16(n);
+    if (rc) {
+        svo_cnn_destroy(n);
+        return rc;
+    }
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    *out = n;
+    return SVO_OK;
+}
+
+// flat parameter vector: per layer W [9][cout][cin] then b [cout]
+extern "C" int svo_cnn_get_params(svo_cnn* n, float* dst, size_t count)
+{
+    if (!n || !dst || count != svo_cnn_param_count()) return svo_fail(SVO_ERR_ARG, "need %zu floats", svo_cnn_param_count());
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    for (auto& l : n->L) {
+        HIPCHECK(hipMemcpy(dst, l.w, l.nw * 4, hipMemcpyDeviceToHost));
+        dst += l.nw;
+        HIPCHECK(hipMemcpy(dst, l.b, l.cout * 4, hipMemcpyDeviceToHost));
+        dst += l.cout;
+    }
+    return SVO_OK;
+}
+
+extern "C" int svo_cnn_set_params(svo_cnn* n, const float* src, size_t count)
+{
+    if (!n || !src || count != svo_cnn_param_count()) return svo_fail(SVO_ERR_ARG, "need %zu floats", svo_cnn_param_count());
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    for (auto& l : n->L) {
+        HIPCHECK(hipMemcpy(l.w, src, l.nw * 4, hipMemcpyHostToDevice));
+        src += l.nw;
+        HIPCHECK(hipMemcpy(l.b, src, l.cout * 4, hipMemcpyHostToDevice));
+        src += l.cout;
+    }
+    return refresh_bf16(n);
+}
+
+static unsigned conv_grid(int B, int H, int W) { return (unsigned)std::min(B * (H / TILE) * (W / TILE), 16384); }
+
+int cnn_forward_dev(svo_cnn* n, int B, int H, int W, const uint32_t* bcount = nullptr)
+{
+    const unsigned g = conv_grid(B, H, W);
+    hipLaunchKernelGGL((k_conv<16, 32, 0>), dim3(g), dim3(64), 0, n->stream, n->x0, n->L[0].wbf, n->L[0].b,
+                       (const bf16*)nullptr, (void*)n->y[0], H, W, B, bcount);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(g), dim3(64), 0, n->stream, n->y[0], n->L[1].wbf, n->L[1].b,
+                       (const bf16*)nullptr, (void*)n->y[1], H, W, B, bcount);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(g), dim3(64), 0, n->stream, n->y[1], n->L[2].wbf, n->L[2].b,
+                       (const bf16*)nullptr, (void*)n->y[2], H, W, B, bcount);
+    hipLaunchKernelGGL((k_conv<32, 16, 1>), dim3(g), dim3(64), 0, n->stream, n->y[2], n->L[3].wbf, n->L[3].b,
+                       (const bf16*)nullptr, (void*)n->out, H, W, B, bcount);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+// backward from n->out vs n->target/n->mask; gradients summed into L[].gw/gb
+int cnn_backward_dev(svo_cnn* n, int B, int H, int W)
+{
+    const size_t pix = (size_t)B * H * W;
+    const unsigned g = conv_grid(B, H, W);
+    for (auto& l : n->L) {
+        HIPCHECK(hipMemsetAsync(l.gw, 0, l.nw * 4, n->stream));
+        HIPCHECK(hipMemsetAsync(l.gb, 0, l.cout * 4, n->stream));
+    }
+    HIPCHECK(hipMemsetAsync(n->loss, 0, 4, n->stream));
+    hipLaunchKernelGGL(k_loss, dim3(nb(pix)), dim3(256), 0, n->stream, n->out, n->target, n->mask, n->dz16, n->loss,
+                       (int)pix, n->mask_count);
+    const unsigned gdb = (unsigned)((pix + 4095) / 4096);
+    // layer 4 (32 -> 16)
+    hipLaunchKernelGGL((k_conv_dw<32, 16>), dim3(B, 9), dim3(64), 0, n->stream, n->y[2], n->dz16, n->L[3].gw, H, W);
+    hipLaunchKernelGGL((k_conv_db<16>), dim3(gdb), dim3(256), 0, n->stream, n->dz16, n->L[3].gb, (int)pix);
+    hipLaunchKernelGGL((k_conv<16, 32, 2>), dim3(g), dim3(64), 0, n->stream, n->dz16, n->L[3].wflip,
+                       (const float*)nullptr, n->y[2], (void*)n->dz32a, H, W, B, (const uint32_t*)nullptr);
+    // layer 3
+    hipLaunchKernelGGL((k_conv_dw<32, 32>), dim3(B, 9), dim3(64), 0, n->stream, n->y[1], n->dz32a, n->L[2].gw, H, W);
+    hipLaunchKernelGGL((k_conv_db<32>), dim3(gdb), dim3(256), 0, n->stream, n->dz32a, n->L[2].gb, (int)pix);
+    hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(g), dim3(64), 0, n->stream, n->dz32a, n->L[2].wflip,
+                       (const float*)nullptr, n->y[1], (void*)n->dz32b, H, W, B, (const uint32_t*)nullptr);
+    // layer 2
+    hipLaunchKernelGGL((k_conv_dw<32, 32>), dim3(B, 9), dim3(64), 0, n->stream, n->y[0], n->dz32b, n->L[1].gw, H, W);
+    hipLaunchKernelGGL((k_conv_db<32>), dim3(gdb), dim3(256), 0, n->stream, n->dz32b, n->L[1].gb, (int)pix);
+    hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(g), dim3(64), 0, n->stream, n->dz32b, n->L[1].wflip,
+                       (const float*)nullptr, n->y[0], (void*)n->dz32a, H, W, B, (const uint32_t*)nullptr);
+    // layer 1 (16 -> 32)
+    hipLaunchKernelGGL((k_conv_dw<16, 32>), dim3(B, 9), dim3(64), 0, n->stream, n->x0, n->dz32a, n->L[0].gw, H, W);
+    hipLaunchKernelGGL((k_conv_db<32>), dim3(gdb), dim3(256), 0, n->stream, n->dz32a, n->L[0].gb, (int)pix);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+int cnn_adam_dev(svo_cnn* n, float lr)
+{
+    n->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)n->step), c2 = 1.f - powf(b2, (float)n->step);
+    for (auto& l : n->L) {
+        hipLaunchKernelGGL(k_adam, dim3(nb(l.nw)), dim3(256), 0, n->stream, l.w, l.gw, l.mw, l.vw, (int)l.nw, lr, b1,
+                           b2, c1, c2, 1e-8f);
+        hipLaunchKernelGGL(k_adam, dim3(nb(l.cout)), dim3(256), 0, n->stream, l.b, l.gb, l.mb, l.vb, l.cout, lr, b1,
+                           b2, c1, c2, 1e-8f);
+    }
+    HIPCHECK(hipGetLastError());
+    return refresh_bf16(n);
+}
+
+static int check_shape(int B, int H, int W)
+{
+    if (B <= 0 || H <= 0 || W <= 0 || H % TILE || W % TILE || (size_t)B * H * W > (1u << 26) || (H * W) % 16)
+        return svo_fail(SVO_ERR_ARG, "batch %dx%dx%d: H and W must be positive multiples of 8", B, H, W);
+    return SVO_OK;
+}
+
+// Raw forward on host data: x = [B][H][W][16] bf16 bits, out = [B][H][W][16] fp32.
+extern "C" int svo_cnn_forward(svo_cnn* n, const uint16_t* x, int B, int H, int W, float* out)
+{
+    if (!n || !x || !out) return svo_fail(SVO_ERR_ARG, "null argument");
+    int rc = check_shape(B, H, W);
+    if (rc) return rc;
+    HIPCHECK(hipSetDevice(n->device));
+    const size_t pix = (size_t)B * H * W;
+    rc = ensure_ws(n, pix);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpyAsync(n->x0, x, pix * 16 * 2, hipMemcpyHostToDevice, n->stream));
+    rc = cnn_forward_dev(n, B, H, W);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpyAsync(out, n->out, pix * 16 * 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    return SVO_OK;
+}
+
+// Raw loss + gradients (no update): target [B][H][W][3], mask [B][H][W];
+// grads written as the flat parameter vector layout; returns the loss.
+extern "C" int svo_cnn_gradients(svo_cnn* n, const uint16_t* x, const float* target, const float* mask, int B, int H,
+                                 int W, float* grads, size_t count, float* loss)
+{
+    if (!n || !x || !target || !mask || !grads || !loss) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (count != svo_cnn_param_count()) return svo_fail(SVO_ERR_ARG, "need %zu floats", svo_cnn_param_count());
+    int rc = check_shape(B, H, W);
+    if (rc) return rc;
+    HIPCHECK(hipSetDevice(n->device));
+    const size_t pix = (size_t)B * H * W;
+    rc = ensure_ws(n, pix);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpyAsync(n->x0, x, pix * 16 * 2, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipMemcpyAsync(n->target, target, pix * 3 * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipMemcpyAsync(n->mask, mask, pix * 4, hipMemcpyHostToDevice, n->stream));
+    float cnt = 0.f;
+    for (size_t i = 0; i < pix; i++) cnt += mask[i];
+    HIPCHECK(hipMemcpyAsync(n->mask_count, &cnt, 4, hipMemcpyHostToDevice, n->stream));
+    if ((rc = cnn_forward_dev(n, B, H, W)) || (rc = cnn_backward_dev(n, B, H, W))) return rc;
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    for (auto& l : n->L) {
+        HIPCHECK(hipMemcpy(grads, l.gw, l.nw * 4, hipMemcpyDeviceToHost));
+        grads += l.nw;
+        HIPCHECK(hipMemcpy(grads, l.gb, l.cout * 4, hipMemcpyDeviceToHost));
+        grads += l.cout;
+    }
+    HIPCHECK(hipMemcpy(loss, n->loss, 4, hipMemcpyDeviceToHost));
+    return SVO_OK;
+}
+
+static FeatArgs feat_args(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, int crop)
+{
+    FeatArgs a;
+    memset(&a, 0, sizeof(a));
+    a.words = c->d_screen + 2 * c->N;
+    a.coords = c->d_coords + 2 * c->N;
+    a.res_x = c->res_x;
+    a.res_y = c->res_y;
+    a.crop = crop;
+    a.tiles_x = c->n_tiles_x;
+    a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
+    a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
+    a.x = n->x0;
+    return a;
+}
+
+// One Adam step on n_crops crop x crop windows of the current frame pair:
+// features from the warp context's pre-filter planes, targets from the full
+// context's image, loss on the warp frame's hole pixels.  Both contexts must
+// have rendered the same camera.  The CNN's stream waits for both contexts.
+extern "C" int svo_cnn_train_step(svo_cnn* n, svo_ctx* warp, svo_ctx* full, const svo_frame_params* p, int n_crops,
+                                  int crop, uint64_t seed, float lr, float* loss_out)
+{
+    if (!n || !warp || !full || !p) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (warp->res_x != full->res_x || warp->res_y != full->res_y || !warp->d_screen)
+        return svo_fail(SVO_ERR_STATE, "contexts must share one resolution");
+    if (n_crops <= 0 || n_crops > 256 || crop < 16 || crop % 8 || crop > warp->res_x || crop > warp->res_y)
+        return svo_fail(SVO_ERR_ARG, "bad crop batch %d x %d", n_crops, crop);
+    HIPCHECK(hipSetDevice(n->device));
+    int rc = ensure_ws(n, (size_t)n_crops * crop * crop);
+    if (rc) return rc;
+    if (n->max_origins < n_crops) {
+        (void)hipFree(n->origins);
+        n->origins = nullptr;
+        HIPCHECK(hipMalloc(&n->origins, sizeof(int2) * 256));
+        n->max_origins = 256;
+    }
+    std::vector<int2> org(n_crops);
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x1234567ull;
+    for (auto& o : org) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        o.x = (int)((st >> 33) % (uint64_t)(warp->res_x - crop + 1));
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        o.y = (int)((st >> 33) % (uint64_t)(warp->res_y - crop + 1));
+    }
+    // order after both contexts' queued frames
+    hipEvent_t e1, e2;
+    HIPCHECK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+    (void)hipEventRecord(e1, warp->stream);
+    (void)hipEventRecord(e2, full->stream);
+    (void)hipStreamWaitEvent(n->stream, e1, 0);
+    (void)hipStreamWaitEvent(n->stream, e2, 0);
+    HIPCHECK(hipMemcpyAsync(n->origins, org.data(), sizeof(int2) * n_crops, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipMemsetAsync(n->mask_count, 0, 4, n->stream));
+    FeatArgs a = feat_args(n, warp, p, crop);
+    a.image = full->d_image;
+    a.origins = n->origins;
+    a.max_crops = n_crops;
+    a.target = n->target;
+    a.mask = n->mask;
+    a.mask_count = n->mask_count;
+    hipLaunchKernelGGL(k_features, dim3(1024), dim3(256), 0, n->stream, a);
+    HIPCHECK(hipGetLastError());
+    if ((rc = cnn_forward_dev(n, n_crops, crop, crop)) || (rc = cnn_backward_dev(n, n_crops, crop, crop)) ||
+        (rc = cnn_adam_dev(n, lr)))
+        return rc;
+    // the next frames of either context wait for this step's reads
+    (void)hipEventRecord(e1, n->stream);
+    (void)hipStreamWaitEvent(warp->stream, e1, 0);
+    (void)hipStreamWaitEvent(full->stream, e1, 0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (loss_out) {
+        HIPCHECK(hipStreamSynchronize(n->stream));
+        HIPCHECK(hipMemcpy(loss_out, n->loss, 4, hipMemcpyDeviceToHost));
+    }
+    return SVO_OK;
+}
+
+// Inference on the gate's tile list (24x24 crops around each 16x16 tile),
+// queued on the context's stream.
+int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
+                   const uint32_t* tile_count, int max_tiles)
+{
+    int rc = ensure_ws(n, (size_t)max_tiles * 576);
+    if (rc) return rc;
+    hipStream_t keep = n->stream;
+    n->stream = c->stream;  // run on the frame's stream
+    FeatArgs a = feat_args(n, c, p, 24);
+    a.tiles = tile_list;
+    a.count = tile_count;
+    a.max_crops = max_tiles;
+    hipLaunchKernelGGL(k_features, dim3(2048), dim3(256), 0, n->stream, a);
+    rc = cnn_forward_dev(n, max_tiles, 24, 24, tile_count);
+    if (!rc) {
+        hipLaunchKernelGGL(k_write_fill, dim3(2048), dim3(256), 0, n->stream, n->out, tile_list, tile_count,
+                           max_tiles, c->d_screen + 2 * c->N, c->d_image, c->res_x, c->res_y, c->n_tiles_x);
+        if (hipGetLastError() != hipSuccess) rc = svo_fail(SVO_ERR_HIP, "CNN fill launch failed");
+    }
+    n->stream = keep;
+    return rc;
+}

@@ -1,0 +1,54 @@
+// svo_ctx.h -- the per-device context shared by the ray/warp runtime
+// (svo_device.hip) and the learned hole filler (svo_cnn.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "svo_internal.h"
+
+enum Stage { ST_CLEAR, ST_SCATTER, ST_RESOLVE, ST_COUNT, ST_SCAN, ST_WRITE, ST_RAYS, ST_COPY, ST_FILTER, ST_N };
+static const char* const kStageNames[ST_N] = {"clear", "proj_scatter", "proj_resolve", "counthole", "sumids",
+                                        "writeids",  "rays",        "copy",         "fill_colorize"};
+
+struct svo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    // octree
+    uint32_t* d_oct = nullptr;  // validated copy, normal records in breadth-first order
+    size_t n_words = 0;
+    uint32_t normal_words = 0;  // size of the renumbered normal region
+    uint32_t root = 0;
+    int depth = 0;
+    // frame buffers
+    int res_x = 0, res_y = 0;
+    size_t N = 0, S = 0;
+    uint32_t* d_screen = nullptr;
+    float4* d_coords = nullptr;
+    uint32_t* d_ids = nullptr;
+    uint32_t* d_image = nullptr;
+    unsigned long long* d_keys = nullptr;
+    uint32_t* d_snap = nullptr;
+    uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [8..9] ray count (u64)
+    // learned-fill gate: [0] CNN tile count, [1] gate ray count
+    uint32_t* d_gate = nullptr;
+    uint32_t* d_gate_tiles = nullptr;  // tile indices sent to the CNN
+    uint32_t* d_gate_rays = nullptr;   // x | y << 16 of holes sent back to raycasting
+    int n_tiles_x = 0, n_tiles_y = 0;  // 16x16 gate tiles (partial ones included)
+    int launch_seq = 0;
+    bool plane1_holes = false;
+    // tuning
+    int refill = 16, wg_per_cu = 4, iters = 64;
+    uint32_t cache_req = 0;  // requested LDS node-cache words
+    // timing
+    bool timing = false;
+    hipEvent_t ev[ST_N + 1] = {};
+    float stage_ms[ST_N] = {};
+    bool stage_used[ST_N] = {};
+};
+
+
+// learned-fill hooks implemented in svo_cnn.hip
+int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
+                   const uint32_t* tile_count, int max_tiles);

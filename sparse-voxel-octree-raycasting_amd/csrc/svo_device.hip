@@ -29,6 +29,7 @@
 #include <cstring>
 #include <vector>
 
+#include "svo_ctx.h"
 #include "svo_internal.h"
 
 #define HOLE 0xffffff00u
@@ -114,9 +115,8 @@ struct RayArgs {
     uint32_t* screen;  // plane 0
     float* coords;     // plane 0, float4 per pixel
     uint32_t* image;   // optional: colorized output (full-frame mode)
-    const uint32_t* ids;
-    uint32_t S;
-    int use_holes;
+    const uint32_t* list;        // pixel list (x | y << 16), or null
+    const uint32_t* list_count;  // its length, read on the device
     int add_x, add_y, ext_x, ext_y, tiles_x;
     uint32_t n_rect;
     int res_x, res_y;
@@ -166,7 +166,7 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, uint32_t H, const R
 {
     int x, y;
     if (k < H) {
-        uint32_t v = a.ids[2u * a.S + k];
+        uint32_t v = a.list[k];
         x = (int)(v & 0xffffu);
         y = (int)(v >> 16);
     } else {
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     const OctBuf ob = oct_buffer(a.oct, a.oct_bytes);
     const int lane = __lane_id();
     const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
-    const uint32_t H = a.use_holes ? min(a.ids[0], N) : 0u;  // the id list holds at most N entries
+    const uint32_t H = a.list ? min(*a.list_count, N) : 0u;  // a pixel list holds at most N entries
     const uint32_t total = H + a.n_rect;
     if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
 
@@ -593,6 +593,36 @@ __global__ __launch_bounds__(256) void k_fill_color(const uint32_t* __restrict__
     if (image) image[i] = colorize_word(v);
 }
 
+// ---------------------------------------------------------------- learned-fill gate
+__global__ void k_gate_reset(uint32_t* gate) { gate[0] = 0u; gate[1] = 0u; }
+
+// One 16x16 tile per block: count its holes (plane 0 after the hole and tile
+// rays); 0 < holes <= max_cnn -> tile goes to the CNN, more -> its hole pixels
+// go back to raycasting.
+__global__ __launch_bounds__(256) void k_gate(const uint32_t* __restrict__ screen, int res_x, int res_y,
+                                              int tiles_x, int max_cnn, uint32_t* gate, uint32_t* tiles,
+                                              uint32_t* rays, uint32_t max_tiles)
+{
+    const int t = blockIdx.x;
+    const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    const bool hole = x < res_x && y < res_y && screen[(size_t)y * res_x + x] == HOLE;
+    const int n = __syncthreads_count(hole);
+    if (n == 0) return;
+    if (n <= max_cnn) {
+        if (threadIdx.x == 0) {
+            const uint32_t k = atomicAdd(&gate[0], 1u);
+            if (k < max_tiles) tiles[k] = (uint32_t)t;
+        }
+        return;
+    }
+    const unsigned long long m = __ballot(hole);
+    const int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(&gate[1], (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (hole) rays[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)x | ((uint32_t)y << 16);
+}
+
 }  // namespace
 
 // ====================================================================== runtime
@@ -609,42 +639,6 @@ __global__ __launch_bounds__(256) void k_fill_color(const uint32_t* __restrict__
             return svo_fail(SVO_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
     } while (0)
 
-enum Stage { ST_CLEAR, ST_SCATTER, ST_RESOLVE, ST_COUNT, ST_SCAN, ST_WRITE, ST_RAYS, ST_COPY, ST_FILTER, ST_N };
-static const char* kStageNames[ST_N] = {"clear", "proj_scatter", "proj_resolve", "counthole", "sumids",
-                                        "writeids",  "rays",        "copy",         "fill_colorize"};
-
-struct svo_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int n_cu = 256;
-    // octree
-    uint32_t* d_oct = nullptr;  // validated copy, normal records in breadth-first order
-    size_t n_words = 0;
-    uint32_t normal_words = 0;  // size of the renumbered normal region
-    uint32_t root = 0;
-    int depth = 0;
-    // frame buffers
-    int res_x = 0, res_y = 0;
-    size_t N = 0, S = 0;
-    uint32_t* d_screen = nullptr;
-    float4* d_coords = nullptr;
-    uint32_t* d_ids = nullptr;
-    uint32_t* d_image = nullptr;
-    unsigned long long* d_keys = nullptr;
-    uint32_t* d_snap = nullptr;
-    uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [8..9] ray count (u64)
-    int launch_seq = 0;
-    bool plane1_holes = false;
-    // tuning
-    int refill = 16, wg_per_cu = 4, iters = 64;
-    uint32_t cache_req = 0;  // requested LDS node-cache words
-    // timing
-    bool timing = false;
-    hipEvent_t ev[ST_N + 1] = {};
-    float stage_ms[ST_N] = {};
-    bool stage_used[ST_N] = {};
-};
-
 static int free_frame(svo_ctx* c)
 {
     (void)hipFree(c->d_screen);
@@ -659,6 +653,10 @@ static int free_frame(svo_ctx* c)
     c->d_image = nullptr;
     c->d_keys = nullptr;
     c->d_snap = nullptr;
+    (void)hipFree(c->d_gate);
+    (void)hipFree(c->d_gate_tiles);
+    (void)hipFree(c->d_gate_rays);
+    c->d_gate = c->d_gate_tiles = c->d_gate_rays = nullptr;
     c->res_x = c->res_y = 0;
     c->N = c->S = 0;
     return SVO_OK;
@@ -742,7 +740,10 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
     const size_t ids_words = 2 * S + N + 64;
     if (hipMalloc(&c->d_screen, 4 * N * 4) != hipSuccess || hipMalloc(&c->d_coords, 4 * N * 16) != hipSuccess ||
         hipMalloc(&c->d_ids, ids_words * 4) != hipSuccess || hipMalloc(&c->d_image, N * 4) != hipSuccess ||
-        hipMalloc(&c->d_keys, N * 8) != hipSuccess || hipMalloc(&c->d_snap, N * 4) != hipSuccess) {
+        hipMalloc(&c->d_keys, N * 8) != hipSuccess || hipMalloc(&c->d_snap, N * 4) != hipSuccess ||
+        hipMalloc(&c->d_gate, 64) != hipSuccess ||
+        hipMalloc(&c->d_gate_tiles, ((size_t)(res_x + 15) / 16) * ((res_y + 15) / 16) * 4) != hipSuccess ||
+        hipMalloc(&c->d_gate_rays, N * 4) != hipSuccess) {
         free_frame(c);
         return svo_fail(SVO_ERR_NOMEM, "frame buffers for %dx%d", res_x, res_y);
     }
@@ -752,6 +753,9 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
     HIPCHECK(hipMemset(c->d_image, 0, N * 4));
     HIPCHECK(hipMemset(c->d_keys, 0xff, N * 8));
     HIPCHECK(hipMemset(c->d_snap, 0, N * 4));
+    HIPCHECK(hipMemset(c->d_gate, 0, 64));
+    c->n_tiles_x = (res_x + 15) / 16;
+    c->n_tiles_y = (res_y + 15) / 16;
     HIPCHECK(hipDeviceSynchronize());
     c->res_x = res_x;
     c->res_y = res_y;
@@ -824,8 +828,8 @@ static int launch_scatter_resolve(svo_ctx* c, const svo_frame_params* p, int pla
     return SVO_OK;
 }
 
-static int launch_rays(svo_ctx* c, const svo_frame_params* p, int use_holes, int add_x, int add_y, int ext_x,
-                       int ext_y, uint32_t* image)
+static int launch_rays(svo_ctx* c, const svo_frame_params* p, const uint32_t* list, const uint32_t* list_count,
+                       int add_x, int add_y, int ext_x, int ext_y, uint32_t* image)
 {
     RayArgs a;
     a.oct = c->d_oct;
@@ -834,9 +838,8 @@ static int launch_rays(svo_ctx* c, const svo_frame_params* p, int use_holes, int
     a.screen = c->d_screen;
     a.coords = (float*)c->d_coords;
     a.image = image;
-    a.ids = c->d_ids;
-    a.S = (uint32_t)c->S;
-    a.use_holes = use_holes;
+    a.list = list;
+    a.list_count = list_count;
     a.add_x = add_x;
     a.add_y = add_y;
     a.ext_x = std::max(0, ext_x);
@@ -931,11 +934,56 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
     mark(c, ST_RAYS);
-    RC(launch_rays(c, p, 1, ax, ay, ex, ey, nullptr));
+    RC(launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr));
     mark(c, ST_COPY);
     RC(launch_copy(c, 2));
     mark(c, ST_FILTER);
     RC(launch_fill(c, c->d_screen + 2 * c->N, c->d_image, 1));
+    if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
+    return SVO_OK;
+}
+
+// Warp frame with the learned fill: after the hole and tile rays, a per-tile
+// gate sends tiles with more than max_cnn_holes hole pixels back to
+// raycasting and the rest to the CNN, whose RGB replaces the filtered colour
+// of their hole pixels in the image.  Screen words keep the reference
+// filter's result.  cnn == NULL keeps the filter's colours (gate only).
+extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
+{
+    RC(need_octree(c));
+    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
+    if (max_cnn_holes < 0 || max_cnn_holes > 256) return svo_fail(SVO_ERR_ARG, "max_cnn_holes outside 0..256");
+    HIPCHECK(hipSetDevice(c->device));
+    for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
+    mark(c, ST_CLEAR);
+    if (p->frame < 2) {
+        RC(launch_memset(c, c->d_screen, HOLE, 4 * c->N));
+        c->plane1_holes = true;
+    }
+    const int lo = c->plane1_holes ? 2 : 1;
+    RC(launch_scatter_resolve(c, p, lo, 3 - lo, 1));
+    mark(c, ST_COUNT);
+    RC(launch_count(c));
+    mark(c, ST_SCAN);
+    RC(launch_sum(c));
+    mark(c, ST_WRITE);
+    RC(launch_write(c));
+    int ax, ay, ex, ey;
+    svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
+    mark(c, ST_RAYS);
+    RC(launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr));
+    const int tiles = c->n_tiles_x * c->n_tiles_y;
+    const int max_tiles = std::min(tiles, 4096);
+    hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, c->d_gate);
+    hipLaunchKernelGGL(k_gate, dim3((unsigned)tiles), dim3(256), 0, c->stream, c->d_screen, c->res_x, c->res_y,
+                       c->n_tiles_x, max_cnn_holes, c->d_gate, c->d_gate_tiles, c->d_gate_rays, (uint32_t)max_tiles);
+    HIPCHECK(hipGetLastError());
+    RC(launch_rays(c, p, c->d_gate_rays, c->d_gate + 1, 0, 0, 0, 0, nullptr));
+    mark(c, ST_COPY);
+    RC(launch_copy(c, 2));
+    mark(c, ST_FILTER);
+    RC(launch_fill(c, c->d_screen + 2 * c->N, c->d_image, 1));
+    if (cnn) RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, max_tiles));
     if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
     return SVO_OK;
 }
@@ -947,7 +995,7 @@ extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
     HIPCHECK(hipSetDevice(c->device));
     for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
     mark(c, ST_RAYS);
-    RC(launch_rays(c, p, 0, 0, 0, c->res_x, c->res_y, c->d_image));
+    RC(launch_rays(c, p, nullptr, nullptr, 0, 0, c->res_x, c->res_y, c->d_image));
     if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
     return SVO_OK;
 }
@@ -996,7 +1044,7 @@ extern "C" int svo_raycast_holes(svo_ctx* c, const svo_frame_params* p)
     RC(need_octree(c));
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     HIPCHECK(hipSetDevice(c->device));
-    return launch_rays(c, p, 1, 0, 0, 0, 0, nullptr);
+    return launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, 0, 0, 0, 0, nullptr);
 }
 extern "C" int svo_raycast_rect(svo_ctx* c, const svo_frame_params* p, int add_x, int add_y, int ext_x, int ext_y)
 {
@@ -1005,7 +1053,7 @@ extern "C" int svo_raycast_rect(svo_ctx* c, const svo_frame_params* p, int add_x
     if (add_x < 0 || add_y < 0 || ext_x < 0 || ext_y < 0 || ext_x > 65536 || ext_y > 65536)
         return svo_fail(SVO_ERR_ARG, "bad rectangle");
     HIPCHECK(hipSetDevice(c->device));
-    return launch_rays(c, p, 0, add_x, add_y, ext_x, ext_y, nullptr);
+    return launch_rays(c, p, nullptr, nullptr, add_x, add_y, ext_x, ext_y, nullptr);
 }
 extern "C" int svo_copy_cache(svo_ctx* c, int dst_plane)
 {

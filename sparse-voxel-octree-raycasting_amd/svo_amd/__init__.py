@@ -102,6 +102,17 @@ _f = {
                         C.POINTER(C.c_size_t), C.POINTER(C.c_uint32), C.POINTER(C.c_int)),
     "scene_generate": _sig("svo_scene_generate", C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
                            C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
+    "cnn_create": _sig("svo_cnn_create", C.c_int, C.POINTER(_ctxp), C.c_int, C.c_uint64),
+    "cnn_destroy": _sig("svo_cnn_destroy", C.c_int, _ctxp),
+    "cnn_param_count": _sig("svo_cnn_param_count", C.c_size_t),
+    "cnn_get_params": _sig("svo_cnn_get_params", C.c_int, _ctxp, C.POINTER(C.c_float), C.c_size_t),
+    "cnn_set_params": _sig("svo_cnn_set_params", C.c_int, _ctxp, C.POINTER(C.c_float), C.c_size_t),
+    "cnn_forward": _sig("svo_cnn_forward", C.c_int, _ctxp, _u16p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)),
+    "cnn_gradients": _sig("svo_cnn_gradients", C.c_int, _ctxp, _u16p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                          C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float)),
+    "cnn_train_step": _sig("svo_cnn_train_step", C.c_int, _ctxp, _ctxp, _ctxp, C.POINTER(FrameParams), C.c_int,
+                           C.c_int, C.c_uint64, C.c_float, C.POINTER(C.c_float)),
+    "frame_warp_learned": _sig("svo_frame_warp_learned", C.c_int, _ctxp, _ctxp, C.POINTER(FrameParams), C.c_int),
     "camera_path": _sig("svo_camera_path", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_float,
                         C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)),
 }
@@ -115,6 +126,8 @@ EXPORTED = [
     "svo_tile_schedule", "svo_build_octree", "svo_free", "svo_octree_save", "svo_octree_load",
     "svo_scene_generate", "svo_camera_path", "svo_enable_timing", "svo_stage_times",
     "svo_stage_name", "svo_set_tuning", "svo_ray_count",
+    "svo_cnn_create", "svo_cnn_destroy", "svo_cnn_param_count", "svo_cnn_get_params", "svo_cnn_set_params",
+    "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
 ]
 
 
@@ -250,6 +263,11 @@ class Context:
     def frame_full(self, p):
         _check(_f["frame_full"](self._h, C.byref(p)))
 
+    def frame_warp_learned(self, cnn, p, max_cnn_holes=64):
+        """Warp frame with the CNN fill and per-tile gate."""
+        _check(_f["frame_warp_learned"](self._h, cnn._h if cnn is not None else None, C.byref(p),
+                                        int(max_cnn_holes)))
+
     # per-stage (reference kernel names)
     def memset(self, byte_ofs, val, nbytes):
         _check(_f["memset"](self._h, int(byte_ofs), int(val) & 0xFFFFFFFF, int(nbytes)))
@@ -335,6 +353,64 @@ class Context:
 
     def image(self):
         return self.read(BUF_IMAGE, 0, 4 * self.N).reshape(self.res_y, self.res_x)
+
+
+_fp = C.POINTER(C.c_float)
+
+
+class Cnn:
+    """Learned hole filler: BF16 MFMA CNN with its own training kernels."""
+
+    def __init__(self, device=0, seed=1):
+        self._h = _ctxp()
+        _check(_f["cnn_create"](C.byref(self._h), int(device), int(seed)))
+        self.n_params = int(_f["cnn_param_count"]())
+
+    def close(self):
+        if self._h:
+            _f["cnn_destroy"](self._h)
+            self._h = _ctxp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        _check(_f["cnn_get_params"](self._h, out.ctypes.data_as(_fp), self.n_params))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(_f["cnn_set_params"](self._h, p.ctypes.data_as(_fp), len(p)))
+
+    def forward(self, x_bf16):
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16)
+        B, H, W, Cc = x.shape
+        out = np.zeros((B, H, W, 16), np.float32)
+        _check(_f["cnn_forward"](self._h, x.ctypes.data_as(_u16p), B, H, W, out.ctypes.data_as(_fp)))
+        return out
+
+    def train_step(self, warp, full, p, n_crops=16, crop=64, seed=0, lr=1e-3, want_loss=False):
+        """One Adam step on crops of the frame pair held by warp/full."""
+        loss = C.c_float()
+        _check(_f["cnn_train_step"](self._h, warp._h, full._h, C.byref(p), int(n_crops), int(crop),
+                                    int(seed), float(lr), C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def gradients(self, x_bf16, target, mask):
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.float32)
+        B, H, W, _ = x.shape
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        _check(_f["cnn_gradients"](self._h, x.ctypes.data_as(_u16p), t.ctypes.data_as(_fp),
+                                   m.ctypes.data_as(_fp), B, H, W, g.ctypes.data_as(_fp), self.n_params,
+                                   C.byref(loss)))
+        return g, loss.value
 
 
 def image_rgb(img):

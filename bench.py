@@ -53,6 +53,13 @@ def run_gpu(args, name):
         c.upload_octree(words, root, depth)
         c.resize(rx, ry)
     res = {}
+    if args.tune:
+        from svo_amd import autotune as A
+        tp = [W.frame_params(name, rows, f) for f in range(min(total, 40))]
+        for c, mode in ((warp, "warp"), (full, "full")):
+            best, ms, _ = A.tune(c, tp, warmup=10, mode=mode)
+            res[f"tuned_{mode}"] = dict(best, ms=round(ms, 4))
+            log(f"[bench] autotune {mode}: {best} -> {ms:.4f} ms/frame")
 
     # ---- pass A: throughput, frames back to back (no per-frame host sync)
     for f in range(args.warmup):
@@ -67,23 +74,28 @@ def run_gpu(args, name):
     rays_warp = warp.ray_count(reset=True)
     log(f"[bench] warp: {args.steps} frames in {t_warp * 1e3:.1f} ms")
 
-    for f in range(min(args.warmup, 3)):
-        full.frame_full(params[f])
-    full.sync()
-    full.ray_count(reset=True)
-    t0 = time.perf_counter()
-    for f in range(args.warmup, total):
-        full.frame_full(params[f])
-    full.sync()
-    t_full = time.perf_counter() - t0
-    rays_full = full.ray_count(reset=True)
-    log(f"[bench] full: {args.steps} frames in {t_full * 1e3:.1f} ms")
+    t_full, rays_full = float("nan"), 0
+    if args.legs == "all":
+        for f in range(min(args.warmup, 3)):
+            full.frame_full(params[f])
+        full.sync()
+        full.ray_count(reset=True)
+        t0 = time.perf_counter()
+        for f in range(args.warmup, total):
+            full.frame_full(params[f])
+        full.sync()
+        t_full = time.perf_counter() - t0
+        rays_full = full.ray_count(reset=True)
+        log(f"[bench] full: {args.steps} frames in {t_full * 1e3:.1f} ms")
 
     # ---- pass B: per-frame GPU time and per-stage times (HIP events on the
     # library's stream), replaying the same frames on fresh state
     warp2 = S.Context(0)
     warp2.upload_octree(words, root, depth)
     warp2.resize(rx, ry)
+    if "tuned_warp" in res:
+        t = res["tuned_warp"]
+        warp2.set_tuning(t["refill"], t["wg_per_cu"], t["iters"], t["lds_cache_words"])
     for f in range(args.warmup):
         warp2.frame_warp(params[f])
     warp2.sync()
@@ -98,6 +110,8 @@ def run_gpu(args, name):
         frame_ms.append(sum(st.values()))
         for k, v in st.items():
             stage_acc.setdefault(k, []).append(v)
+        if args.legs != "all":
+            continue
         full.frame_full(params[f])
         full.sync()
         fs = full.stage_times()
@@ -112,8 +126,35 @@ def run_gpu(args, name):
         warp_ms=frame_ms, full_ms=full_ms, stage_mean=stage_mean,
         rays_per_frame=rays_per_frame, rays_full_per_frame=rays_full / args.steps,
         mrays_warp=rays_warp / t_warp / 1e6, mrays_full=rays_full / t_full / 1e6,
-        ray_ms_full=float(np.mean(ray_ms_full)), psnr=psnrs, depth=depth, words=len(words),
+        ray_ms_full=float(np.mean(ray_ms_full)) if ray_ms_full else float("nan"), psnr=psnrs, depth=depth, words=len(words),
         rx=rx, ry=ry)
+    # ---- learned fill (gate + BF16 CNN) with the committed weights
+    if args.learned and args.legs == "all":
+        from svo_amd import learned as L
+        if os.path.exists(L.WEIGHTS):
+            cnn = S.Cnn(0)
+            L.load(cnn, L.WEIGHTS)
+            lw = S.Context(0)
+            lw.upload_octree(words, root, depth)
+            lw.resize(rx, ry)
+            if "tuned_warp" in res:
+                t = res["tuned_warp"]
+                lw.set_tuning(t["refill"], t["wg_per_cu"], t["iters"], t["lds_cache_words"])
+            for f in range(args.warmup):
+                lw.frame_warp_learned(cnn, params[f], args.max_cnn_holes)
+            lw.sync()
+            t0 = time.perf_counter()
+            for f in range(args.warmup, total):
+                lw.frame_warp_learned(cnn, params[f], args.max_cnn_holes)
+            lw.sync()
+            res["learned_fps"] = args.steps / (time.perf_counter() - t0)
+            lw.close()
+            allp, holes = L.evaluate(cnn, name, seed_off=1000, n_frames=40, max_cnn_holes=args.max_cnn_holes)
+            res["psnr_modes"] = {k: round(v, 2) for k, v in allp.items()}
+            res["psnr_hole_pixels"] = {k: round(v, 2) for k, v in holes.items()}
+            cnn.close()
+        else:
+            log(f"[bench] no trained weights at {L.WEIGHTS}: learned leg skipped")
     for c in (warp, warp2, full):
         c.close()
     return res
@@ -162,7 +203,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="flythrough_2048")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-frames", type=int, default=4)
+    ap.add_argument("--cpu-frames", type=int, default=8)
+    ap.add_argument("--no-tune", dest="tune", action="store_false", help="skip the ray-kernel autotuner")
+    ap.add_argument("--no-learned", dest="learned", action="store_false", help="skip the learned-fill leg")
+    ap.add_argument("--max-cnn-holes", type=int, default=64)
+    ap.add_argument("--legs", default="all", choices=["all", "warp"],
+                    help="warp: only the warp-frame legs (for profiling the warp kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -209,6 +255,11 @@ def main():
         "rays_per_warp_frame": round(g["rays_per_frame"]),
         "psnr_warp_vs_full_db": round(float(np.mean([p for p in g["psnr"] if p != float("inf")] or [99.0])), 2),
         "stage_ms": {k: round(v, 4) for k, v in g["stage_mean"].items()},
+        "autotuned": {k: g[k] for k in ("tuned_warp", "tuned_full") if k in g},
+        "learned_fill": None if "learned_fps" not in g else {
+            "fps": round(g["learned_fps"], 2), "speedup_vs_full": round(g["learned_fps"] / g["full_fps"], 3),
+            "psnr_vs_full_db": g["psnr_modes"], "psnr_hole_pixels_db": g["psnr_hole_pixels"],
+            "max_cnn_holes_per_tile": args.max_cnn_holes},
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["warp_fps"], 3), "unit": "frames/s", "cores": threads, "kind": "port",

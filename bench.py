@@ -160,10 +160,91 @@ def run_gpu(args, name):
     return res
 
 
-def run_cpu_baseline(name, n_warp_frames, threads):
+def run_batch(args):
+    """Batched views: 24 seeded cameras of uneven resolution over the depth-11
+    scene.  The same frames run once view by view (svo_frame_warp per
+    context, back to back) and once as batched frames (one launch per stage
+    over all views, one shared ray launch)."""
+    import svo_amd as S
+    from svo_amd import workloads as W
+
+    views = W.batch_views()
+    words, root, depth, _ = W.octree(views[0]["scene"])
+    n_frames = views[0]["frames"]
+    frames = []
+    paths = [W.view_path(v) for v in views]
+    for f in range(n_frames):
+        frames.append([W.frame_params(v["scene"], paths[i], f) for i, v in enumerate(views)])
+    ctxs = []
+    for v in views:
+        c = S.Context(0)
+        c.upload_octree(words, root, depth)
+        c.resize(*v["res"])
+        ctxs.append(c)
+    pixels = sum(v["res"][0] * v["res"][1] for v in views)
+    warm = max(2, min(args.batch_warmup, n_frames - 1))
+    out = dict(views=len(views), frames_per_view=n_frames, warmup_frames=warm,
+               pixels_per_batch_frame=pixels,
+               res_counts={f"{rx}x{ry}": sum(1 for v in views if v["res"] == (rx, ry)) for rx, ry in W.VIEW_RES})
+
+    def seq_run():
+        for ps in frames[:warm]:
+            for c, p in zip(ctxs, ps):
+                c.frame_warp(p)
+        for c in ctxs:
+            c.sync()
+            c.ray_count(reset=True)
+        t0 = time.perf_counter()
+        for ps in frames[warm:]:
+            for c, p in zip(ctxs, ps):
+                c.frame_warp(p)
+        for c in ctxs:
+            c.sync()
+        t = time.perf_counter() - t0
+        return t, sum(c.ray_count(reset=True) for c in ctxs)
+
+    batch = S.Batch(ctxs)
+
+    def batch_run():
+        for ps in frames[:warm]:
+            batch.frame_warp(ps)
+        batch.sync()
+        ctxs[0].ray_count(reset=True)
+        t0 = time.perf_counter()
+        for ps in frames[warm:]:
+            batch.frame_warp(ps)
+        batch.sync()
+        t = time.perf_counter() - t0
+        return t, ctxs[0].ray_count(reset=True)
+
+    if args.tune:
+        from svo_amd import autotune as A
+        best, ms, _ = A.tune_batch(batch, frames[:min(n_frames, 16)], warmup=6)
+        out["tuned"] = dict(best, ms=round(ms, 4))
+        log(f"[bench] batch autotune: {best} -> {ms:.4f} ms/batch frame")
+        for c in ctxs[1:]:  # the view-by-view run uses the same knobs
+            c.set_tuning(best["refill"], best["wg_per_cu"], best["iters"], best["lds_cache_words"])
+    t_seq, rays_seq = seq_run()
+    t_bat, rays_bat = batch_run()
+    k = n_frames - warm
+    out.update(
+        batch_frames_per_s=round(k / t_bat, 2), batch_view_frames_per_s=round(k * len(views) / t_bat, 1),
+        sequential_view_frames_per_s=round(k * len(views) / t_seq, 1),
+        speedup_batch_vs_sequential=round(t_seq / t_bat, 3),
+        batch_ms_per_frame=round(t_bat / k * 1e3, 4),
+        mrays_s_batch=round(rays_bat / t_bat / 1e6, 1), mrays_s_sequential=round(rays_seq / t_seq / 1e6, 1),
+        mpixels_s_batch=round(pixels * k / t_bat / 1e6, 1))
+    log(f"[bench] batch: {out}")
+    batch.close()
+    for c in ctxs:
+        c.close()
+    return out
+
+
+def run_cpu_baseline(name, n_warp_frames, n_full_frames, threads):
     """Oracle (the CPU restatement, kind "port") on a bounded sample of the
     same workload: frames 0-1 (full raycasts through the hole path) untimed,
-    then n_warp_frames warp frames timed; plus one full frame."""
+    then n_warp_frames warp frames timed; plus n_full_frames full frames."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from svo_amd import workloads as W
@@ -185,12 +266,13 @@ def run_cpu_baseline(name, n_warp_frames, threads):
     fetch = st1["fetch_words"] - st0["fetch_words"]
     fr2 = O.Frame(rx, ry, words, root, depth, nthreads=threads)
     t0 = time.perf_counter()
-    fr2.frame_full(O.params_from(W.frame_params(name, rows, 2)))
+    for f in range(2, 2 + n_full_frames):
+        fr2.frame_full(O.params_from(W.frame_params(name, rows, f)))
     t_full = time.perf_counter() - t0
     fst = fr2.stats
     fr.close()
     fr2.close()
-    return dict(warp_fps=n_warp_frames / t_warp, full_fps=1.0 / t_full,
+    return dict(warp_fps=n_warp_frames / t_warp, full_fps=n_full_frames / t_full, full_frames=n_full_frames,
                 fetch_per_ray_warp=fetch / max(1, rays),
                 fetch_per_ray_full=fst["fetch_words"] / max(1, fst["rays"]),
                 t=t_warp + t_full, frames=n_warp_frames)
@@ -203,10 +285,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="flythrough_2048")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-frames", type=int, default=8)
+    ap.add_argument("--cpu-frames", type=int, default=280, help="warp frames in the CPU sample")
+    ap.add_argument("--cpu-full-frames", type=int, default=48, help="full frames in the CPU sample")
     ap.add_argument("--no-tune", dest="tune", action="store_false", help="skip the ray-kernel autotuner")
     ap.add_argument("--no-learned", dest="learned", action="store_false", help="skip the learned-fill leg")
     ap.add_argument("--max-cnn-holes", type=int, default=64)
+    ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the batched-views leg")
+    ap.add_argument("--batch-warmup", type=int, default=8)
     ap.add_argument("--legs", default="all", choices=["all", "warp"],
                     help="warp: only the warp-frame legs (for profiling the warp kernels)")
     args = ap.parse_args()
@@ -218,12 +303,13 @@ def main():
         raise SystemExit("--warmup must be >= 2 (frames 0-1 are full clears)")
 
     g = run_gpu(args, args.workload)
+    batch = run_batch(args) if (args.batch and args.legs == "all") else None
     # dominant kernel: ray traversal in the warp frames
     ray_ms = g["stage_mean"].get("rays", float("nan"))
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
-        cpu = run_cpu_baseline(args.workload, args.cpu_frames, threads)
+        cpu = run_cpu_baseline(args.workload, args.cpu_frames, args.cpu_full_frames, threads)
         log(f"[bench] cpu baseline: {cpu}")
     fbar = cpu["fetch_per_ray_warp"] if cpu else None
     roof = None
@@ -260,12 +346,13 @@ def main():
             "fps": round(g["learned_fps"], 2), "speedup_vs_full": round(g["learned_fps"] / g["full_fps"], 3),
             "psnr_vs_full_db": g["psnr_modes"], "psnr_hole_pixels_db": g["psnr_hole_pixels"],
             "max_cnn_holes_per_tile": args.max_cnn_holes},
+        "batched_views": batch,
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["warp_fps"], 3), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{cpu['frames']} warp frames (after 2 clear frames) + 1 full frame of the same "
-                      f"workload, oracle restatement, {threads} threads; full raycast "
-                      f"{cpu['full_fps']:.3f} frames/s"},
+            "sample": f"{cpu['frames']} warp frames (after 2 clear frames) + {cpu['full_frames']} full "
+                      f"frames of the same workload, oracle restatement, {threads} threads, "
+                      f"{cpu['t']:.1f} s; full raycast {cpu['full_fps']:.3f} frames/s"},
     }
     print(json.dumps(out), flush=True)
 

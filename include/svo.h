@@ -227,6 +227,23 @@ int svo_cnn_train_step(svo_cnn* cnn, svo_ctx* warp, svo_ctx* full, const svo_fra
  * alone, fillhole2 colours everywhere. */
 int svo_frame_warp_learned(svo_ctx* ctx, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
 
+/* ---- batched views (no counterpart in the reference) -------------------
+ * A batch runs the warp frame of many contexts that hold the same octree on
+ * one device: every warp stage is one launch over all views and the hole and
+ * tile rays of all views share one persistent ray launch, so the long-ray
+ * tails of different views overlap.  Each view's outputs equal those of
+ * svo_frame_warp on that context.  The views keep their own resolution and
+ * frame counter; the contexts must not be resized or given a new octree while
+ * the batch exists.  The ray kernel uses the first view's tuning and counts
+ * rays on the first view. */
+typedef struct svo_batch svo_batch;
+int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_views);
+int svo_batch_destroy(svo_batch* batch);
+/* params[i] drives view i; ordered after each view's queued work, and each
+ * view's later work is ordered after the batch. */
+int svo_batch_frame_warp(svo_batch* batch, const svo_frame_params* params);
+int svo_batch_sync(svo_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,21 +108,39 @@ __device__ uint32_t fetch_color(OctBuf oct, const uint32_t* cache, uint32_t cach
     return oct_word(oct, cache, cache_words, n + nadd);
 }
 
+// Per-view frame description shared by every kernel.  Single-view calls pass
+// it by value; a batch passes a device table and each block (grid z) or ray
+// (ray kernel) picks its view.
+struct View {
+    uint32_t* screen;          // 4 planes x N words
+    float4* coords;            // 4 planes x N float4
+    uint32_t* ids;             // [S counts | S offsets | hole list]
+    uint32_t* image;           // colorized output, or null
+    unsigned long long* keys;  // warp keys (N)
+    const uint32_t* list;       // ray pixel list (x | y << 16), or null
+    const uint32_t* list_count; // its length, read on the device
+    int res_x, res_y;
+    uint32_t N, S;
+    float m0x, m0y, m0z;
+    float r00, r01, r02, r10, r11, r12, r20, r21, r22;  // proj rows
+    float cx0, cx1, cx2, cy0, cy1, cy2, cz0, cz1, cz2;  // ray columns
+    float fovx, fovy;
+    int plane_lo, n_planes;                // proj source planes
+    int add_x, add_y, ext_x, ext_y, tiles_x;  // refresh rectangle
+    uint32_t n_rect;
+    uint32_t* ray_image;                   // colorize ray words straight into this, or null
+};
+
+#define VIEW_OF(idx) const View& V = views ? views[(idx)] : v
+
 struct RayArgs {
+    View v;                       // single view
+    const View* views;            // batch table (device), or null
+    const uint32_t* view_first;   // batch: first ray of each view, [n_views + 1]
+    int n_views;
     const uint32_t* oct;
     uint32_t root;
     int depth;
-    uint32_t* screen;  // plane 0
-    float* coords;     // plane 0, float4 per pixel
-    uint32_t* image;   // optional: colorized output (full-frame mode)
-    const uint32_t* list;        // pixel list (x | y << 16), or null
-    const uint32_t* list_count;  // its length, read on the device
-    int add_x, add_y, ext_x, ext_y, tiles_x;
-    uint32_t n_rect;
-    int res_x, res_y;
-    float m0x, m0y, m0z;
-    float cx0, cx1, cx2, cy0, cy1, cy2, cz0, cz1, cz2;
-    float fovx, fovy;
     uint32_t* queue;
     int slot;
     int refill;
@@ -143,6 +161,7 @@ struct Ray {
     uint32_t node, before, lroot, test;
     uint32_t pix;
     uint32_t guard;
+    int view;
 };
 
 __device__ __forceinline__ uint32_t colorize_word(uint32_t word)
@@ -161,34 +180,54 @@ __device__ __forceinline__ uint32_t colorize_word(uint32_t word)
     return (uint32_t)(b + g * 256 + r * 65536);
 }
 
-// Ray set-up: kernel.cl:883-908 and CAST_RAY prologue :115-150.
-__device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, uint32_t H, const RayArgs& a)
+__device__ __forceinline__ uint32_t view_rays(const View& V)
 {
+    // a pixel list holds at most N entries
+    return (V.list ? min(*V.list_count, V.N) : 0u) + V.n_rect;
+}
+
+// Ray set-up: kernel.cl:883-908 and CAST_RAY prologue :115-150.
+__device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
+{
+    int vi = 0;
+    if (a.views) {  // last view whose first ray is <= k
+        int lo = 0, hi = a.n_views - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.view_first[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        vi = lo;
+        k -= a.view_first[vi];
+    }
+    const View& V = a.views ? a.views[vi] : a.v;
+    const uint32_t H = V.list ? min(*V.list_count, V.N) : 0u;
     int x, y;
     if (k < H) {
-        uint32_t v = a.list[k];
-        x = (int)(v & 0xffffu);
-        y = (int)(v >> 16);
+        uint32_t w = V.list[k];
+        x = (int)(w & 0xffffu);
+        y = (int)(w >> 16);
     } else {
         k -= H;
         uint32_t t = k >> 6, w = k & 63u;
-        int tx = (int)(t % (uint32_t)a.tiles_x), ty = (int)(t / (uint32_t)a.tiles_x);
+        int tx = (int)(t % (uint32_t)V.tiles_x), ty = (int)(t / (uint32_t)V.tiles_x);
         int lx = tx * 8 + (int)(w & 7u), ly = ty * 8 + (int)(w >> 3);
-        if (lx >= a.ext_x || ly >= a.ext_y) return false;
-        x = a.add_x + lx;
-        y = a.add_y + ly;
+        if (lx >= V.ext_x || ly >= V.ext_y) return false;
+        x = V.add_x + lx;
+        y = V.add_y + ly;
     }
-    if (x >= a.res_x || y >= a.res_y) return false;
-    r.pix = (uint32_t)(y * a.res_x + x);
+    if (x >= V.res_x || y >= V.res_y) return false;
+    r.pix = (uint32_t)(y * V.res_x + x);
+    r.view = vi;
 
     const int D = a.depth;
     const float SCALE_MAX = (float)(1 << (D + 1));
-    float d1x = ((float)(x - a.res_x / 2) + 0.5f) * a.fovx / (float)a.res_y;
-    float d1y = ((float)(y - a.res_y / 2) + 0.5f) * a.fovy / (float)a.res_y;
-    float dx = dot3(d1x, d1y, 1.0f, a.cx0, a.cx1, a.cx2);
-    float dy = dot3(d1x, d1y, 1.0f, a.cy0, a.cy1, a.cy2);
-    float dz = dot3(d1x, d1y, 1.0f, a.cz0, a.cz1, a.cz2);
-    float px = a.m0x * 16.0f, py = a.m0y * 16.0f, pz = a.m0z * 16.0f;
+    float d1x = ((float)(x - V.res_x / 2) + 0.5f) * V.fovx / (float)V.res_y;
+    float d1y = ((float)(y - V.res_y / 2) + 0.5f) * V.fovy / (float)V.res_y;
+    float dx = dot3(d1x, d1y, 1.0f, V.cx0, V.cx1, V.cx2);
+    float dy = dot3(d1x, d1y, 1.0f, V.cy0, V.cy1, V.cy2);
+    float dz = dot3(d1x, d1y, 1.0f, V.cz0, V.cz1, V.cz2);
+    float px = V.m0x * 16.0f, py = V.m0y * 16.0f, pz = V.m0z * 16.0f;
     const float eps = 0x1p-20f;
     if (fabsf(dx) < eps) dx = cl_sign(dx) * eps;
     if (fabsf(dy) < eps) dy = cl_sign(dy) * eps;
@@ -209,7 +248,7 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, uint32_t H, const R
     r.rek = D;
     r.lod = 1;
     r.dist = 0.0f;
-    r.lodsw = (float)(a.res_x * 1 * 2);
+    r.lodsw = (float)(V.res_x * 1 * 2);
     r.node = a.root;
     r.before = a.root;
     r.lroot = 0;
@@ -329,12 +368,13 @@ __device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, OctBuf ob, 
     uint32_t b2 = (r.rek == 1) ? STK(2) : 0u;
     uint32_t col = fetch_color(ob, cache, a.cache_words, r.node, r.before, b2, r.lroot, r.rek, r.test);
     uint32_t word = 0xff000000u + col;
-    a.screen[r.pix] = word;
-    float* c = a.coords + 4u * r.pix;
+    const View& V = a.views ? a.views[r.view] : a.v;
+    V.screen[r.pix] = word;
+    float* c = (float*)V.coords + 4u * r.pix;
     c[0] = px / 16.0f;
     c[1] = py / 16.0f;
     c[2] = pz / 16.0f;
-    if (a.image) a.image[r.pix] = colorize_word(word);
+    if (V.ray_image) V.ray_image[r.pix] = colorize_word(word);
 }
 
 template <bool kCache>
@@ -348,9 +388,7 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     __syncthreads();
     const OctBuf ob = oct_buffer(a.oct, a.oct_bytes);
     const int lane = __lane_id();
-    const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
-    const uint32_t H = a.list ? min(*a.list_count, N) : 0u;  // a pixel list holds at most N entries
-    const uint32_t total = H + a.n_rect;
+    const uint32_t total = a.views ? a.view_first[a.n_views] : view_rays(a.v);
     if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
 
     Ray r;
@@ -368,7 +406,7 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
             if (!active) {
                 const int rank = __popcll(idle & ((1ull << lane) - 1ull));
                 const uint32_t k = base + (uint32_t)rank;
-                if (k < total) active = init_ray(r, k, H, a);
+                if (k < total) active = init_ray(r, k, a);
             }
             traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)(64 - n_idle);
         }
@@ -407,66 +445,57 @@ __global__ __launch_bounds__(256) void k_memset(uint32_t* __restrict__ dst, uint
     for (; i < n; i += stride) dst[i] = val;
 }
 
-struct ProjArgs {
-    uint32_t* screen;
-    const float4* coords;
-    unsigned long long* keys;
-    int res_x, res_y;
-    int plane_lo;
-    float m0x, m0y, m0z;
-    float r00, r01, r02, r10, r11, r12, r20, r21, r22;
-};
-
-// kernel.cl:498-572 per source pixel; blockIdx.y selects the source plane.
-__global__ __launch_bounds__(256) void k_proj_scatter(ProjArgs a)
+// kernel.cl:498-572 per source pixel; grid y = source plane, z = view.
+__global__ __launch_bounds__(256) void k_proj_scatter(View v, const View* views)
 {
-    const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
+    VIEW_OF(blockIdx.z);
+    const uint32_t N = V.N;
     const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
-    if (pix >= N) return;
-    const uint32_t src = (uint32_t)(a.plane_lo + (int)blockIdx.y) * N + pix;
-    const uint32_t col = a.screen[src];
+    if (pix >= N || (int)blockIdx.y >= V.n_planes) return;
+    const uint32_t src = (uint32_t)(V.plane_lo + (int)blockIdx.y) * N + pix;
+    const uint32_t col = V.screen[src];
     if (col == HOLE) return;
-    const float4 c = a.coords[src];
-    const float qx = c.x - a.m0x, qy = c.y - a.m0y, qz = c.z - a.m0z;
-    const float hx = dot3(qx, qy, qz, a.r00, a.r01, a.r02);
-    const float hy = dot3(qx, qy, qz, a.r10, a.r11, a.r12);
-    const float hz = dot3(qx, qy, qz, a.r20, a.r21, a.r22);
-    if (hz < 0.05f) { a.screen[src] = HOLE; return; }
-    const float sfx = (hx * (float)a.res_y + 0.0f) / hz + (float)a.res_x / 2 - 0.0f;
-    const float sfy = (hy * (float)a.res_y + 0.0f) / hz + (float)a.res_y / 2 - 0.0f;
+    const float4 c = V.coords[src];
+    const float qx = c.x - V.m0x, qy = c.y - V.m0y, qz = c.z - V.m0z;
+    const float hx = dot3(qx, qy, qz, V.r00, V.r01, V.r02);
+    const float hy = dot3(qx, qy, qz, V.r10, V.r11, V.r12);
+    const float hz = dot3(qx, qy, qz, V.r20, V.r21, V.r22);
+    if (hz < 0.05f) { V.screen[src] = HOLE; return; }
+    const float sfx = (hx * (float)V.res_y + 0.0f) / hz + (float)V.res_x / 2 - 0.0f;
+    const float sfy = (hy * (float)V.res_y + 0.0f) / hz + (float)V.res_y / 2 - 0.0f;
     const int sxi = (int)sfx, syi = (int)sfy;
-    if (sxi >= a.res_x - 1 || sxi < 0 || syi >= a.res_y - 1 || syi < 0) { a.screen[src] = HOLE; return; }
-    const uint32_t t = (uint32_t)syi * (uint32_t)a.res_x + (uint32_t)sxi;
+    if (sxi >= V.res_x - 1 || sxi < 0 || syi >= V.res_y - 1 || syi < 0) { V.screen[src] = HOLE; return; }
+    const uint32_t t = (uint32_t)syi * (uint32_t)V.res_x + (uint32_t)sxi;
     const uint32_t depth = (uint32_t)(int)(hz * 1000.0f) & 0xffffffu;
     const unsigned long long key = ((unsigned long long)depth << 40) |
                                    ((unsigned long long)src << 8) | (unsigned long long)(col & 255u);
-    if (a.keys[t] > key) atomicMin(&a.keys[t], key);
+    if (V.keys[t] > key) atomicMin(&V.keys[t], key);
 }
 
 // Resolve: winner words/coords into plane 0 (kernel.cl:571-585 semantics
 // against the plane-0 word w0), key reset; clear=1 also writes the hole word
 // to every pixel without a winner (the per-frame memset of plane 0).
-__global__ __launch_bounds__(256) void k_proj_resolve(ProjArgs a, int clear)
+__global__ __launch_bounds__(256) void k_proj_resolve(View v, const View* views, int clear)
 {
-    const uint32_t N = (uint32_t)a.res_x * (uint32_t)a.res_y;
+    VIEW_OF(blockIdx.z);
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= N) return;
-    const unsigned long long key = a.keys[i];
+    if (i >= V.N) return;
+    const unsigned long long key = V.keys[i];
     if (key != ~0ull) {
-        a.keys[i] = ~0ull;
-        const uint32_t w0 = clear ? HOLE : a.screen[i];
+        V.keys[i] = ~0ull;
+        const uint32_t w0 = clear ? HOLE : V.screen[i];
         const uint32_t depth = (uint32_t)(key >> 40);
         if (depth < (w0 >> 8)) {
             const uint32_t src = (uint32_t)(key >> 8);
-            const float4 c = a.coords[src];
-            const float qx = c.x - a.m0x, qy = c.y - a.m0y, qz = c.z - a.m0z;
-            const float hz = dot3(qx, qy, qz, a.r20, a.r21, a.r22);
-            a.screen[i] = (depth << 8) + (uint32_t)(key & 255u);
-            ((float4*)a.coords)[i] = make_float4(c.x, c.y, c.z, hz);
+            const float4 c = V.coords[src];
+            const float qx = c.x - V.m0x, qy = c.y - V.m0y, qz = c.z - V.m0z;
+            const float hz = dot3(qx, qy, qz, V.r20, V.r21, V.r22);
+            V.screen[i] = (depth << 8) + (uint32_t)(key & 255u);
+            V.coords[i] = make_float4(c.x, c.y, c.z, hz);
             return;
         }
     }
-    if (clear) a.screen[i] = HOLE;
+    if (clear) V.screen[i] = HOLE;
 }
 
 // ---------------------------------------------------------------- compaction
@@ -478,44 +507,51 @@ __device__ __forceinline__ bool quad_is_hole(const uint32_t* __restrict__ s, int
     return h >= 4;
 }
 
-__global__ __launch_bounds__(64) void k_counthole(const uint32_t* __restrict__ screen, uint32_t* ids, int res_x, int res_y)
+__global__ __launch_bounds__(64) void k_counthole(View v, const View* views)
 {
-    const int bxn = res_x / 16;
-    const int b = blockIdx.x;
-    const int x = (b % bxn) * 16 + 2 * (threadIdx.x & 7), y = (b / bxn) * 16 + 2 * (threadIdx.x >> 3);
-    const unsigned long long m = __ballot(quad_is_hole(screen, res_x, x, y));
-    if (threadIdx.x == 0) ids[b] = 4u * (uint32_t)__popcll(m);
+    VIEW_OF(blockIdx.z);
+    const int bxn = V.res_x / 16;
+    const uint32_t b = blockIdx.x;
+    if (b >= V.S) return;
+    const int x = (int)(b % bxn) * 16 + 2 * (threadIdx.x & 7), y = (int)(b / bxn) * 16 + 2 * (threadIdx.x >> 3);
+    const unsigned long long m = __ballot(quad_is_hole(V.screen, V.res_x, x, y));
+    if (threadIdx.x == 0) V.ids[b] = 4u * (uint32_t)__popcll(m);
 }
 
-__global__ __launch_bounds__(64) void k_writeids(const uint32_t* __restrict__ screen, uint32_t* ids, int res_x, int res_y)
+__global__ __launch_bounds__(64) void k_writeids(View v, const View* views)
 {
-    const int bxn = res_x / 16;
-    const uint32_t S = (uint32_t)bxn * (uint32_t)(res_y / 16);
-    const int b = blockIdx.x;
+    VIEW_OF(blockIdx.z);
+    const int bxn = V.res_x / 16;
+    const uint32_t S = V.S;
+    const uint32_t b = blockIdx.x;
+    if (b >= S) return;
     const int lane = threadIdx.x;
-    const int x = (b % bxn) * 16 + 2 * (lane & 7), y = (b / bxn) * 16 + 2 * (lane >> 3);
-    const bool hole = quad_is_hole(screen, res_x, x, y);
+    const int x = (int)(b % bxn) * 16 + 2 * (lane & 7), y = (int)(b / bxn) * 16 + 2 * (lane >> 3);
+    const bool hole = quad_is_hole(V.screen, V.res_x, x, y);
     const unsigned long long m = __ballot(hole);
     if (!hole) return;
     const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    const uint32_t dst = 2u * S + ids[S + b] + 4u * rank;
+    const uint32_t dst = 2u * S + V.ids[S + b] + 4u * rank;
     const uint32_t val = (uint32_t)x + ((uint32_t)y << 16);
-    ids[dst + 0] = val;
-    ids[dst + 1] = val + 1u;
-    ids[dst + 2] = val + 1u + (1u << 16);
-    ids[dst + 3] = val + (1u << 16);
+    V.ids[dst + 0] = val;
+    V.ids[dst + 1] = val + 1u;
+    V.ids[dst + 2] = val + 1u + (1u << 16);
+    V.ids[dst + 3] = val + (1u << 16);
 }
 
-// exclusive scan of ids[0..S) into ids[S..2S), total -> ids[0]
-__global__ __launch_bounds__(1024) void k_sumids(uint32_t* ids, uint32_t S)
+// exclusive scan of ids[0..S) into ids[S..2S), total -> ids[0]; one block per view
+__global__ __launch_bounds__(1024) void k_sumids(View v, const View* views)
 {
+    VIEW_OF(blockIdx.z);
+    uint32_t* ids = V.ids;
+    const uint32_t S = V.S;
     __shared__ uint32_t wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < S; base += 1024) {
         const uint32_t i = base + (uint32_t)tid;
-        const uint32_t v = (i < S) ? ids[i] : 0u;
-        uint32_t incl = v;
+        const uint32_t x = (i < S) ? ids[i] : 0u;
+        uint32_t incl = x;
         for (int d = 1; d < 64; d <<= 1) {
             uint32_t t = __shfl_up(incl, d);
             if (lane >= d) incl += t;
@@ -532,21 +568,33 @@ __global__ __launch_bounds__(1024) void k_sumids(uint32_t* ids, uint32_t S)
         }
         __syncthreads();
         const uint32_t wpre = (w > 0) ? wsum[w - 1] : 0u;
-        if (i < S) ids[S + i] = carry + wpre + incl - v;
+        if (i < S) ids[S + i] = carry + wpre + incl - x;
         carry += wsum[15];
         __syncthreads();
     }
     if (tid == 0) ids[0] = carry;
 }
 
-// ---------------------------------------------------------------- copy / filter / colorize
-__global__ __launch_bounds__(256) void k_copy_plane(uint32_t* screen, float4* coords, uint32_t N, uint32_t dst_plane)
+// first ray of each batched view (after their compaction), one thread
+__global__ void k_view_first(const View* views, int n, uint32_t* first)
 {
+    uint32_t acc = 0;
+    for (int i = 0; i < n; i++) {
+        first[i] = acc;
+        acc += view_rays(views[i]);
+    }
+    first[n] = acc;
+}
+
+// ---------------------------------------------------------------- copy / filter / colorize
+__global__ __launch_bounds__(256) void k_copy_plane(View v, const View* views, uint32_t dst_plane)
+{
+    VIEW_OF(blockIdx.z);
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= N) return;
-    const size_t d = (size_t)dst_plane * N + i;
-    screen[d] = screen[i];
-    coords[d] = coords[i];
+    if (i >= V.N) return;
+    const size_t d = (size_t)dst_plane * V.N + i;
+    V.screen[d] = V.screen[i];
+    V.coords[d] = V.coords[i];
 }
 
 __device__ __forceinline__ uint32_t rs(const uint32_t* __restrict__ s, long o, long limit)
@@ -576,21 +624,23 @@ __device__ uint32_t fill_pixel(const uint32_t* __restrict__ src, long ofs, int r
     return col;
 }
 
-// snapshot -> dst (plane 0) for holes inside the filter window; optional
-// colorize of the filtered word.
-__global__ __launch_bounds__(256) void k_fill_color(const uint32_t* __restrict__ snap, uint32_t* dst,
-                                                    uint32_t* image, int res_x, int res_y, int do_fill)
+// snapshot (plane 2, or `snap` when given) -> plane 0 for holes inside the
+// filter window; colorize of the filtered word into V.image when do_color.
+__global__ __launch_bounds__(256) void k_fill_color(View v, const View* views, const uint32_t* __restrict__ snap_in,
+                                                    int do_fill, int do_color)
 {
-    const uint32_t N = (uint32_t)res_x * (uint32_t)res_y;
+    VIEW_OF(blockIdx.z);
+    const uint32_t N = V.N;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
-    const int x = (int)(i % (uint32_t)res_x), y = (int)(i / (uint32_t)res_x);
-    uint32_t v = snap[i];
-    if (do_fill && v == HOLE && x >= 2 && y >= 2 && x < res_x - 1 && y < res_y - 1) {
-        v = fill_pixel(snap, (long)i, res_x, (long)N);
-        dst[i] = v;
+    const uint32_t* snap = snap_in ? snap_in : V.screen + 2 * (size_t)N;
+    const int x = (int)(i % (uint32_t)V.res_x), y = (int)(i / (uint32_t)V.res_x);
+    uint32_t w = snap[i];
+    if (do_fill && w == HOLE && x >= 2 && y >= 2 && x < V.res_x - 1 && y < V.res_y - 1) {
+        w = fill_pixel(snap, (long)i, V.res_x, (long)N);
+        V.screen[i] = w;
     }
-    if (image) image[i] = colorize_word(v);
+    if (do_color) V.image[i] = colorize_word(w);
 }
 
 // ---------------------------------------------------------------- learned-fill gate
@@ -787,20 +837,46 @@ static inline void mark(svo_ctx* c, int st)
     }
 }
 
-static ProjArgs proj_args(svo_ctx* c, const svo_frame_params* p)
+static View make_view(svo_ctx* c, const svo_frame_params* p)
 {
-    ProjArgs a;
-    a.screen = c->d_screen;
-    a.coords = c->d_coords;
-    a.keys = c->d_keys;
-    a.res_x = c->res_x;
-    a.res_y = c->res_y;
-    a.plane_lo = 1;
-    a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
-    a.r00 = p->row0[0]; a.r01 = p->row0[1]; a.r02 = p->row0[2];
-    a.r10 = p->row1[0]; a.r11 = p->row1[1]; a.r12 = p->row1[2];
-    a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
-    return a;
+    View v;
+    memset(&v, 0, sizeof(v));
+    v.screen = c->d_screen;
+    v.coords = c->d_coords;
+    v.ids = c->d_ids;
+    v.image = c->d_image;
+    v.keys = c->d_keys;
+    v.res_x = c->res_x;
+    v.res_y = c->res_y;
+    v.N = (uint32_t)c->N;
+    v.S = (uint32_t)c->S;
+    v.tiles_x = 1;
+    if (p) {
+        v.m0x = p->m0[0]; v.m0y = p->m0[1]; v.m0z = p->m0[2];
+        v.r00 = p->row0[0]; v.r01 = p->row0[1]; v.r02 = p->row0[2];
+        v.r10 = p->row1[0]; v.r11 = p->row1[1]; v.r12 = p->row1[2];
+        v.r20 = p->row2[0]; v.r21 = p->row2[1]; v.r22 = p->row2[2];
+        v.cx0 = p->col0[0]; v.cx1 = p->col0[1]; v.cx2 = p->col0[2];
+        v.cy0 = p->col1[0]; v.cy1 = p->col1[1]; v.cy2 = p->col1[2];
+        v.cz0 = p->col2[0]; v.cz1 = p->col2[1]; v.cz2 = p->col2[2];
+        v.fovx = p->fovx;
+        v.fovy = p->fovy;
+    }
+    return v;
+}
+
+static void set_rays(View& v, const uint32_t* list, const uint32_t* list_count, int add_x, int add_y, int ext_x,
+                     int ext_y, uint32_t* ray_image)
+{
+    v.list = list;
+    v.list_count = list_count;
+    v.add_x = add_x;
+    v.add_y = add_y;
+    v.ext_x = std::max(0, ext_x);
+    v.ext_y = std::max(0, ext_y);
+    v.tiles_x = std::max(1, (v.ext_x + 7) / 8);
+    v.n_rect = (uint32_t)(((v.ext_x + 7) / 8) * ((v.ext_y + 7) / 8) * 64);
+    v.ray_image = ray_image;
 }
 
 static int launch_memset(svo_ctx* c, uint32_t* dst, uint32_t val, size_t n)
@@ -812,51 +888,80 @@ static int launch_memset(svo_ctx* c, uint32_t* dst, uint32_t val, size_t n)
     return SVO_OK;
 }
 
-static int launch_scatter_resolve(svo_ctx* c, const svo_frame_params* p, int plane_lo, int n_planes, int clear)
+// The warp-stage launchers take the view by value (views == null) or a
+// device table of n views (grid z); maxN / maxS size the grid for the largest.
+struct Launch {
+    hipStream_t stream;
+    const View* views;
+    int n;
+    uint32_t maxN, maxS;
+};
+static Launch single(svo_ctx* c) { return Launch{c->stream, nullptr, 1, (uint32_t)c->N, (uint32_t)c->S}; }
+
+static int launch_scatter_resolve(const Launch& L, const View& v, int max_planes, int clear)
 {
-    ProjArgs a = proj_args(c, p);
-    a.plane_lo = plane_lo;
-    const unsigned gx = (unsigned)((c->N + 255) / 256);
-    mark(c, ST_SCATTER);
-    if (n_planes > 0) {
-        hipLaunchKernelGGL(k_proj_scatter, dim3(gx, (unsigned)n_planes), dim3(256), 0, c->stream, a);
+    const unsigned gx = (L.maxN + 255) / 256;
+    if (max_planes > 0) {
+        hipLaunchKernelGGL(k_proj_scatter, dim3(gx, (unsigned)max_planes, (unsigned)L.n), dim3(256), 0, L.stream, v,
+                           L.views);
         HIPCHECK(hipGetLastError());
     }
-    mark(c, ST_RESOLVE);
-    hipLaunchKernelGGL(k_proj_resolve, dim3(gx), dim3(256), 0, c->stream, a, clear);
+    hipLaunchKernelGGL(k_proj_resolve, dim3(gx, 1, (unsigned)L.n), dim3(256), 0, L.stream, v, L.views, clear);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }
 
-static int launch_rays(svo_ctx* c, const svo_frame_params* p, const uint32_t* list, const uint32_t* list_count,
-                       int add_x, int add_y, int ext_x, int ext_y, uint32_t* image)
+static int launch_count(const Launch& L, const View& v)
+{
+    if (L.maxS == 0) return SVO_OK;
+    hipLaunchKernelGGL(k_counthole, dim3(L.maxS, 1, (unsigned)L.n), dim3(64), 0, L.stream, v, L.views);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_sum(const Launch& L, const View& v)
+{
+    hipLaunchKernelGGL(k_sumids, dim3(1, 1, (unsigned)L.n), dim3(1024), 0, L.stream, v, L.views);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_write(const Launch& L, const View& v)
+{
+    if (L.maxS == 0) return SVO_OK;
+    hipLaunchKernelGGL(k_writeids, dim3(L.maxS, 1, (unsigned)L.n), dim3(64), 0, L.stream, v, L.views);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_copy(const Launch& L, const View& v, int dst_plane)
+{
+    hipLaunchKernelGGL(k_copy_plane, dim3((L.maxN + 255) / 256, 1, (unsigned)L.n), dim3(256), 0, L.stream, v,
+                       L.views, (uint32_t)dst_plane);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_fill(const Launch& L, const View& v, const uint32_t* snap, int do_fill, int do_color)
+{
+    hipLaunchKernelGGL(k_fill_color, dim3((L.maxN + 255) / 256, 1, (unsigned)L.n), dim3(256), 0, L.stream, v,
+                       L.views, snap, do_fill, do_color);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+
+// persistent ray launch over one view (views == null) or a batch table
+static int launch_rays_v(svo_ctx* c, hipStream_t stream, const View& v, const View* views,
+                         const uint32_t* view_first, int n_views, uint32_t* queue, int& seq)
 {
     RayArgs a;
+    memset(&a, 0, sizeof(a));
+    a.v = v;
+    a.views = views;
+    a.view_first = view_first;
+    a.n_views = n_views;
     a.oct = c->d_oct;
     a.root = c->root;
     a.depth = c->depth;
-    a.screen = c->d_screen;
-    a.coords = (float*)c->d_coords;
-    a.image = image;
-    a.list = list;
-    a.list_count = list_count;
-    a.add_x = add_x;
-    a.add_y = add_y;
-    a.ext_x = std::max(0, ext_x);
-    a.ext_y = std::max(0, ext_y);
-    a.tiles_x = std::max(1, (a.ext_x + 7) / 8);
-    a.n_rect = (uint32_t)(((a.ext_x + 7) / 8) * ((a.ext_y + 7) / 8) * 64);
-    a.res_x = c->res_x;
-    a.res_y = c->res_y;
-    a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
-    a.cx0 = p->col0[0]; a.cx1 = p->col0[1]; a.cx2 = p->col0[2];
-    a.cy0 = p->col1[0]; a.cy1 = p->col1[1]; a.cy2 = p->col1[2];
-    a.cz0 = p->col2[0]; a.cz1 = p->col2[1]; a.cz2 = p->col2[2];
-    a.fovx = p->fovx;
-    a.fovy = p->fovy;
-    a.queue = c->d_queue;
-    a.slot = c->launch_seq & 1;
-    c->launch_seq++;
+    a.queue = queue;
+    a.slot = seq & 1;
+    seq++;
     a.refill = c->refill;
     a.iters = c->iters;
     a.ray_count = (unsigned long long*)(c->d_queue + 8);
@@ -866,55 +971,25 @@ static int launch_rays(svo_ctx* c, const svo_frame_params* p, const uint32_t* li
     const unsigned grid = (unsigned)(c->n_cu * c->wg_per_cu);
     const size_t lds = ((size_t)c->depth * RAY_BLOCK + std::max(a.cache_words, 1u)) * 4;
     if (a.cache_words)
-        hipLaunchKernelGGL(k_rays<true>, dim3(grid), dim3(RAY_BLOCK), lds, c->stream, a);
+        hipLaunchKernelGGL(k_rays<true>, dim3(grid), dim3(RAY_BLOCK), lds, stream, a);
     else
-        hipLaunchKernelGGL(k_rays<false>, dim3(grid), dim3(RAY_BLOCK), lds, c->stream, a);
+        hipLaunchKernelGGL(k_rays<false>, dim3(grid), dim3(RAY_BLOCK), lds, stream, a);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }
 
-static int launch_count(svo_ctx* c)
+static int launch_rays(svo_ctx* c, const svo_frame_params* p, const uint32_t* list, const uint32_t* list_count,
+                       int add_x, int add_y, int ext_x, int ext_y, uint32_t* ray_image)
 {
-    if (c->S == 0) return SVO_OK;
-    hipLaunchKernelGGL(k_counthole, dim3((unsigned)c->S), dim3(64), 0, c->stream, c->d_screen, c->d_ids, c->res_x,
-                       c->res_y);
-    HIPCHECK(hipGetLastError());
-    return SVO_OK;
-}
-static int launch_sum(svo_ctx* c)
-{
-    hipLaunchKernelGGL(k_sumids, dim3(1), dim3(1024), 0, c->stream, c->d_ids, (uint32_t)c->S);
-    HIPCHECK(hipGetLastError());
-    return SVO_OK;
-}
-static int launch_write(svo_ctx* c)
-{
-    if (c->S == 0) return SVO_OK;
-    hipLaunchKernelGGL(k_writeids, dim3((unsigned)c->S), dim3(64), 0, c->stream, c->d_screen, c->d_ids, c->res_x,
-                       c->res_y);
-    HIPCHECK(hipGetLastError());
-    return SVO_OK;
-}
-static int launch_copy(svo_ctx* c, int dst_plane)
-{
-    hipLaunchKernelGGL(k_copy_plane, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, c->d_screen,
-                       c->d_coords, (uint32_t)c->N, (uint32_t)dst_plane);
-    HIPCHECK(hipGetLastError());
-    return SVO_OK;
-}
-static int launch_fill(svo_ctx* c, const uint32_t* snap, uint32_t* image, int do_fill)
-{
-    hipLaunchKernelGGL(k_fill_color, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, snap,
-                       c->d_screen, image, c->res_x, c->res_y, do_fill);
-    HIPCHECK(hipGetLastError());
-    return SVO_OK;
+    View v = make_view(c, p);
+    set_rays(v, list, list_count, add_x, add_y, ext_x, ext_y, ray_image);
+    return launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq);
 }
 
-extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
+// the warp stages of raycast_draw up to the hole and tile rays
+static int warp_front(svo_ctx* c, const svo_frame_params* p)
 {
-    RC(need_octree(c));
-    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
-    HIPCHECK(hipSetDevice(c->device));
+    const Launch L = single(c);
     for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
     mark(c, ST_CLEAR);
     if (p->frame < 2) {  // raycast.h:150-154
@@ -923,22 +998,35 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     }
     // planes 1 and 2 (raycast.h:176-198); plane 1 is skipped while the
     // library knows it holds only hole words (then it contributes nothing)
-    const int lo = c->plane1_holes ? 2 : 1;
-    RC(launch_scatter_resolve(c, p, lo, 3 - lo, 1));
+    View v = make_view(c, p);
+    v.plane_lo = c->plane1_holes ? 2 : 1;
+    v.n_planes = 3 - v.plane_lo;
+    mark(c, ST_SCATTER);
+    RC(launch_scatter_resolve(L, v, v.n_planes, 1));
     mark(c, ST_COUNT);
-    RC(launch_count(c));
+    RC(launch_count(L, v));
     mark(c, ST_SCAN);
-    RC(launch_sum(c));
+    RC(launch_sum(L, v));
     mark(c, ST_WRITE);
-    RC(launch_write(c));
+    RC(launch_write(L, v));
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
     mark(c, ST_RAYS);
-    RC(launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr));
+    return launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
+}
+
+extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
+{
+    RC(need_octree(c));
+    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
+    HIPCHECK(hipSetDevice(c->device));
+    RC(warp_front(c, p));
+    const Launch L = single(c);
+    const View v = make_view(c, p);
     mark(c, ST_COPY);
-    RC(launch_copy(c, 2));
+    RC(launch_copy(L, v, 2));
     mark(c, ST_FILTER);
-    RC(launch_fill(c, c->d_screen + 2 * c->N, c->d_image, 1));
+    RC(launch_fill(L, v, nullptr, 1, 1));
     if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
     return SVO_OK;
 }
@@ -954,24 +1042,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     if (max_cnn_holes < 0 || max_cnn_holes > 256) return svo_fail(SVO_ERR_ARG, "max_cnn_holes outside 0..256");
     HIPCHECK(hipSetDevice(c->device));
-    for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
-    mark(c, ST_CLEAR);
-    if (p->frame < 2) {
-        RC(launch_memset(c, c->d_screen, HOLE, 4 * c->N));
-        c->plane1_holes = true;
-    }
-    const int lo = c->plane1_holes ? 2 : 1;
-    RC(launch_scatter_resolve(c, p, lo, 3 - lo, 1));
-    mark(c, ST_COUNT);
-    RC(launch_count(c));
-    mark(c, ST_SCAN);
-    RC(launch_sum(c));
-    mark(c, ST_WRITE);
-    RC(launch_write(c));
-    int ax, ay, ex, ey;
-    svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
-    mark(c, ST_RAYS);
-    RC(launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr));
+    RC(warp_front(c, p));
     const int tiles = c->n_tiles_x * c->n_tiles_y;
     const int max_tiles = std::min(tiles, 4096);
     hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, c->d_gate);
@@ -979,10 +1050,12 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
                        c->n_tiles_x, max_cnn_holes, c->d_gate, c->d_gate_tiles, c->d_gate_rays, (uint32_t)max_tiles);
     HIPCHECK(hipGetLastError());
     RC(launch_rays(c, p, c->d_gate_rays, c->d_gate + 1, 0, 0, 0, 0, nullptr));
+    const Launch L = single(c);
+    const View v = make_view(c, p);
     mark(c, ST_COPY);
-    RC(launch_copy(c, 2));
+    RC(launch_copy(L, v, 2));
     mark(c, ST_FILTER);
-    RC(launch_fill(c, c->d_screen + 2 * c->N, c->d_image, 1));
+    RC(launch_fill(L, v, nullptr, 1, 1));
     if (cnn) RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, max_tiles));
     if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
     return SVO_OK;
@@ -1018,26 +1091,29 @@ extern "C" int svo_proj(svo_ctx* c, int plane, const svo_frame_params* p)
     RC(need_frame(c));
     if (!p || plane < 1 || plane > 3) return svo_fail(SVO_ERR_ARG, "plane %d outside 1..3", plane);
     HIPCHECK(hipSetDevice(c->device));
-    return launch_scatter_resolve(c, p, plane, 1, 0);
+    View v = make_view(c, p);
+    v.plane_lo = plane;
+    v.n_planes = 1;
+    return launch_scatter_resolve(single(c), v, 1, 0);
 }
 
 extern "C" int svo_counthole(svo_ctx* c)
 {
     RC(need_frame(c));
     HIPCHECK(hipSetDevice(c->device));
-    return launch_count(c);
+    return launch_count(single(c), make_view(c, nullptr));
 }
 extern "C" int svo_sumids(svo_ctx* c)
 {
     RC(need_frame(c));
     HIPCHECK(hipSetDevice(c->device));
-    return launch_sum(c);
+    return launch_sum(single(c), make_view(c, nullptr));
 }
 extern "C" int svo_writeids(svo_ctx* c)
 {
     RC(need_frame(c));
     HIPCHECK(hipSetDevice(c->device));
-    return launch_write(c);
+    return launch_write(single(c), make_view(c, nullptr));
 }
 extern "C" int svo_raycast_holes(svo_ctx* c, const svo_frame_params* p)
 {
@@ -1061,20 +1137,158 @@ extern "C" int svo_copy_cache(svo_ctx* c, int dst_plane)
     if (dst_plane < 1 || dst_plane > 3) return svo_fail(SVO_ERR_ARG, "plane %d outside 1..3", dst_plane);
     HIPCHECK(hipSetDevice(c->device));
     if (dst_plane == 1) c->plane1_holes = false;
-    return launch_copy(c, dst_plane);
+    return launch_copy(single(c), make_view(c, nullptr), dst_plane);
 }
 extern "C" int svo_fillhole(svo_ctx* c)
 {
     RC(need_frame(c));
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipMemcpyAsync(c->d_snap, c->d_screen, c->N * 4, hipMemcpyDeviceToDevice, c->stream));
-    return launch_fill(c, c->d_snap, nullptr, 1);
+    return launch_fill(single(c), make_view(c, nullptr), c->d_snap, 1, 0);
 }
 extern "C" int svo_colorize(svo_ctx* c)
 {
     RC(need_frame(c));
     HIPCHECK(hipSetDevice(c->device));
-    return launch_fill(c, c->d_screen, c->d_image, 0);
+    return launch_fill(single(c), make_view(c, nullptr), c->d_screen, 0, 1);
+}
+
+// ---------------------------------------------------------------- batched views
+// Many views (contexts) of one scene run each warp stage as one launch over a
+// device view table (grid z = view) and all their hole + tile rays in ONE
+// persistent ray launch, so the long-ray tails of the views overlap.
+struct svo_batch {
+    std::vector<svo_ctx*> views;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    View* d_views = nullptr;     // 2 slots x n
+    View* h_views = nullptr;     // pinned staging, 2 slots x n
+    hipEvent_t slot_done[2] = {};
+    std::vector<hipEvent_t> pre;  // per view: its stream's queued work
+    uint32_t* d_first = nullptr;
+    uint32_t* d_queue = nullptr;
+    int seq = 0;
+    int frame_no = 0;
+    uint32_t maxN = 0, maxS = 0;
+};
+
+extern "C" int svo_batch_destroy(svo_batch* b)
+{
+    if (!b) return SVO_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)hipFree(b->d_views);
+    (void)hipHostFree(b->h_views);
+    (void)hipFree(b->d_first);
+    (void)hipFree(b->d_queue);
+    for (auto& e : b->slot_done)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : b->pre)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return SVO_OK;
+}
+
+extern "C" int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_views)
+{
+    if (!out || !views || n_views < 1 || n_views > 256) return svo_fail(SVO_ERR_ARG, "1..256 views required");
+    *out = nullptr;
+    svo_ctx* v0 = views[0];
+    for (int i = 0; i < n_views; i++) {
+        svo_ctx* c = views[i];
+        if (!c || !c->d_screen || !c->d_oct) return svo_fail(SVO_ERR_STATE, "view %d has no octree or frame", i);
+        if (c->device != v0->device || c->n_words != v0->n_words || c->root != v0->root || c->depth != v0->depth)
+            return svo_fail(SVO_ERR_ARG, "view %d does not hold the same octree on the same device", i);
+        for (int j = 0; j < i; j++)
+            if (views[j] == c) return svo_fail(SVO_ERR_ARG, "view %d repeats view %d", i, j);
+    }
+    HIPCHECK(hipSetDevice(v0->device));
+    svo_batch* b = new svo_batch();
+    b->device = v0->device;
+    b->views.assign(views, views + n_views);
+    for (auto* c : b->views) {
+        b->maxN = std::max(b->maxN, (uint32_t)c->N);
+        b->maxS = std::max(b->maxS, (uint32_t)c->S);
+    }
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&b->d_views, sizeof(View) * 2 * n_views) == hipSuccess &&
+              hipHostMalloc(&b->h_views, sizeof(View) * 2 * n_views) == hipSuccess &&
+              hipMalloc(&b->d_first, 4 * (n_views + 1)) == hipSuccess && hipMalloc(&b->d_queue, 64) == hipSuccess &&
+              hipMemset(b->d_queue, 0, 64) == hipSuccess &&
+              hipEventCreateWithFlags(&b->slot_done[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->slot_done[1], hipEventDisableTiming) == hipSuccess &&
+              hipEventRecord(b->slot_done[0], b->stream) == hipSuccess &&
+              hipEventRecord(b->slot_done[1], b->stream) == hipSuccess;
+    b->pre.assign(n_views, nullptr);
+    for (int i = 0; ok && i < n_views; i++)
+        ok = hipEventCreateWithFlags(&b->pre[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        svo_batch_destroy(b);
+        return svo_fail(SVO_ERR_HIP, "batch set-up failed");
+    }
+    *out = b;
+    return SVO_OK;
+}
+
+// svo_frame_warp for every view of the batch (params[i] for view i), with all
+// stages batched over the views.  Results per view are identical to
+// svo_frame_warp on that view alone.
+extern "C" int svo_batch_frame_warp(svo_batch* b, const svo_frame_params* params)
+{
+    if (!b || !params) return svo_fail(SVO_ERR_ARG, "null argument");
+    const int n = (int)b->views.size();
+    svo_ctx* c0 = b->views[0];
+    HIPCHECK(hipSetDevice(b->device));
+    // order after the views' own queued work
+    for (int i = 0; i < n; i++) {
+        HIPCHECK(hipEventRecord(b->pre[i], b->views[i]->stream));
+        HIPCHECK(hipStreamWaitEvent(b->stream, b->pre[i], 0));
+    }
+    const int slot = b->frame_no & 1;
+    HIPCHECK(hipEventSynchronize(b->slot_done[slot]));  // staging slot free again
+    View* hv = b->h_views + slot * n;
+    View* dv = b->d_views + slot * n;
+    for (int i = 0; i < n; i++) {
+        svo_ctx* c = b->views[i];
+        const svo_frame_params* p = &params[i];
+        if (p->frame < 2) {
+            hipLaunchKernelGGL(k_memset, dim3((unsigned)std::min<size_t>((4 * c->N + 255) / 256, 4096)), dim3(256),
+                               0, b->stream, c->d_screen, HOLE, 4 * c->N);
+            c->plane1_holes = true;
+        }
+        View v = make_view(c, p);
+        v.plane_lo = c->plane1_holes ? 2 : 1;
+        v.n_planes = 3 - v.plane_lo;
+        int ax, ay, ex, ey;
+        svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
+        set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
+        hv[i] = v;
+    }
+    HIPCHECK(hipMemcpyAsync(dv, hv, sizeof(View) * n, hipMemcpyHostToDevice, b->stream));
+    const Launch L{b->stream, dv, n, b->maxN, b->maxS};
+    const View dummy = hv[0];
+    RC(launch_scatter_resolve(L, dummy, 2, 1));
+    RC(launch_count(L, dummy));
+    RC(launch_sum(L, dummy));
+    RC(launch_write(L, dummy));
+    hipLaunchKernelGGL(k_view_first, dim3(1), dim3(1), 0, b->stream, (const View*)dv, n, b->d_first);
+    HIPCHECK(hipGetLastError());
+    RC(launch_rays_v(c0, b->stream, dummy, dv, b->d_first, n, b->d_queue, b->seq));
+    RC(launch_copy(L, dummy, 2));
+    RC(launch_fill(L, dummy, nullptr, 1, 1));
+    HIPCHECK(hipEventRecord(b->slot_done[slot], b->stream));
+    for (auto* c : b->views) HIPCHECK(hipStreamWaitEvent(c->stream, b->slot_done[slot], 0));
+    b->frame_no++;
+    return SVO_OK;
+}
+
+extern "C" int svo_batch_sync(svo_batch* b)
+{
+    if (!b) return svo_fail(SVO_ERR_ARG, "null batch");
+    HIPCHECK(hipSetDevice(b->device));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    return SVO_OK;
 }
 
 static int buf_ptr(svo_ctx* c, int buf, size_t byte_ofs, size_t bytes, char** p)

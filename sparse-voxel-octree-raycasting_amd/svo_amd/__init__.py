@@ -113,6 +113,10 @@ _f = {
     "cnn_train_step": _sig("svo_cnn_train_step", C.c_int, _ctxp, _ctxp, _ctxp, C.POINTER(FrameParams), C.c_int,
                            C.c_int, C.c_uint64, C.c_float, C.POINTER(C.c_float)),
     "frame_warp_learned": _sig("svo_frame_warp_learned", C.c_int, _ctxp, _ctxp, C.POINTER(FrameParams), C.c_int),
+    "batch_create": _sig("svo_batch_create", C.c_int, C.POINTER(_ctxp), C.POINTER(_ctxp), C.c_int),
+    "batch_destroy": _sig("svo_batch_destroy", C.c_int, _ctxp),
+    "batch_frame_warp": _sig("svo_batch_frame_warp", C.c_int, _ctxp, C.POINTER(FrameParams)),
+    "batch_sync": _sig("svo_batch_sync", C.c_int, _ctxp),
     "camera_path": _sig("svo_camera_path", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_float,
                         C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)),
 }
@@ -128,6 +132,7 @@ EXPORTED = [
     "svo_stage_name", "svo_set_tuning", "svo_ray_count",
     "svo_cnn_create", "svo_cnn_destroy", "svo_cnn_param_count", "svo_cnn_get_params", "svo_cnn_set_params",
     "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
+    "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
 ]
 
 
@@ -425,3 +430,41 @@ def psnr(a, b):
     b = np.asarray(b, dtype=np.float64)
     mse = np.mean((a - b) ** 2)
     return float("inf") if mse == 0 else 10.0 * np.log10(255.0 ** 2 / mse)
+
+
+# ------------------------------------------------------------------ batched views
+class Batch:
+    """Warp frames of several contexts (same octree, one device) run as one
+    launch per stage and one shared persistent ray launch."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (_ctxp * len(self.contexts))(*[c._h for c in self.contexts])
+        self._h = _ctxp()
+        _check(_f["batch_create"](C.byref(self._h), arr, len(self.contexts)))
+
+    def frame_warp(self, params):
+        if len(params) != len(self.contexts):
+            raise ValueError("one FrameParams per view")
+        arr = (FrameParams * len(params))(*params)
+        _check(_f["batch_frame_warp"](self._h, arr))
+
+    def sync(self):
+        _check(_f["batch_sync"](self._h))
+
+    def close(self):
+        if self._h:
+            _f["batch_destroy"](self._h)
+            self._h = _ctxp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

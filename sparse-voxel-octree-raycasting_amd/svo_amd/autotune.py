@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import itertools
 import statistics
+import time
 
 DEFAULT_SPACE = {
     "refill": [8, 16, 32, 48],
@@ -37,26 +38,9 @@ def _frame_time(ctx, params, warmup, mode):
     return statistics.median(ts)
 
 
-def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):
-    """Pick the fastest knob setting for this context's scene/resolution.
-
-    params: FrameParams list (params[0] must have frame 0).  With
-    coordinate=True the knobs are tuned one at a time (coordinate descent,
-    two sweeps); otherwise the full grid is measured.  Applies and returns
-    (best_setting, best_ms, trials)."""
-    space = dict(DEFAULT_SPACE if space is None else space)
+def _search(space, coordinate, measure):
     keys = list(space)
     best = {k: space[k][min(1, len(space[k]) - 1)] for k in keys}
-    trials = []
-
-    def measure(cfg):
-        ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
-        ms = _frame_time(ctx, params, warmup, mode)
-        trials.append((dict(cfg), ms))
-        if log:
-            log(f"[autotune] {cfg} -> {ms:.4f} ms")
-        return ms
-
     if coordinate:
         best_ms = measure(best)
         for _ in range(2):
@@ -75,5 +59,63 @@ def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=No
             ms = measure(cfg)
             if ms < best_ms:
                 best, best_ms = cfg, ms
-    ctx.set_tuning(best["refill"], best["wg_per_cu"], best["iters"], best["lds_cache_words"])
+    return best, best_ms
+
+
+def _apply(ctx, cfg):
+    ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
+
+
+def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):
+    """Pick the fastest knob setting for this context's scene/resolution.
+
+    params: FrameParams list (params[0] must have frame 0).  With
+    coordinate=True the knobs are tuned one at a time (coordinate descent,
+    two sweeps); otherwise the full grid is measured.  Applies and returns
+    (best_setting, best_ms, trials)."""
+    space = dict(DEFAULT_SPACE if space is None else space)
+    trials = []
+
+    def measure(cfg):
+        _apply(ctx, cfg)
+        ms = _frame_time(ctx, params, warmup, mode)
+        trials.append((dict(cfg), ms))
+        if log:
+            log(f"[autotune] {cfg} -> {ms:.4f} ms")
+        return ms
+
+    best, best_ms = _search(space, coordinate, measure)
+    _apply(ctx, best)
+    return best, best_ms, trials
+
+
+def tune_batch(batch, frames, warmup=4, space=None, coordinate=True, log=None):
+    """Tune the shared ray launch of a Batch (its first view's knobs).
+
+    frames: list of per-frame FrameParams lists, one entry per view, starting
+    at frame 0.  The score is the median wall time of one batched frame
+    (launch + device completion); returns (best_setting, best_ms, trials)."""
+    space = dict(DEFAULT_SPACE if space is None else space)
+    ctx0 = batch.contexts[0]
+    trials = []
+
+    def measure(cfg):
+        _apply(ctx0, cfg)
+        for ps in frames[:warmup]:
+            batch.frame_warp(ps)
+        batch.sync()
+        ts = []
+        for ps in frames[warmup:]:
+            t0 = time.perf_counter()
+            batch.frame_warp(ps)
+            batch.sync()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        ms = statistics.median(ts)
+        trials.append((dict(cfg), ms))
+        if log:
+            log(f"[autotune batch] {cfg} -> {ms:.4f} ms")
+        return ms
+
+    best, best_ms = _search(space, coordinate, measure)
+    _apply(ctx0, best)
     return best, best_ms, trials

@@ -6,10 +6,16 @@ arguments.  Scene seed = 1000 + n, path seed = 2000 + n for workload n.
   flythrough_2048: depth 11 (2048^3) 4-octave fractal terrain shell + 2000
                    sphere shells (radius 4..64 voxels), 1920x1024, 300 frames,
                    1..8 voxels and <= 1 deg yaw per frame
+  views_24:        24 independent cameras over the flythrough_2048 scene,
+                   per-view resolution drawn from {320x240, 640x480,
+                   1280x720, 1920x1024}, per-view motion up to 0.5..16
+                   voxels and 0..3 deg yaw per frame, 120 frames per view
 """
 from __future__ import annotations
 
 import functools
+
+import numpy as np
 
 from . import build_octree, camera, camera_path, scene
 
@@ -42,3 +48,24 @@ def frame_params(name, rows, f, frame_index=None):
     yaw, pitch, roll, x, y, z = [float(v) for v in rows[f]]
     p, _ = camera(yaw, pitch, roll, (x, y, z), w["depth"], f if frame_index is None else frame_index)
     return p
+
+
+VIEW_RES = [(320, 240), (640, 480), (1280, 720), (1920, 1024)]
+
+
+def batch_views(n_views=24, seed=5005, frames=120, scene="flythrough_2048"):
+    """Per-view dicts (res, vox, yaw, path_seed) of the batched-views
+    workload, drawn from seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_views):
+        res = VIEW_RES[int(rng.integers(len(VIEW_RES)))]
+        out.append(dict(scene=scene, res=res, vox=float(rng.uniform(0.5, 16.0)),
+                        yaw=float(rng.uniform(0.0, 3.0)), path_seed=seed + 1 + i, frames=frames))
+    return out
+
+
+def view_path(view):
+    w = WORKLOADS[view["scene"]]
+    return camera_path(view["path_seed"], w["depth"], view["frames"], view["vox"], view["yaw"],
+                       w["height"], w["pitch"])

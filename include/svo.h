@@ -172,10 +172,24 @@ int svo_camera_path(uint64_t seed, int depth, int n_frames,
                     float height_voxels, float pitch, float* out);
 
 /* ---- measurement -------------------------------------------------------- */
-/* Per-stage GPU timing with HIP events (off by default).  After a frame,
- * svo_stage_times fills up to n floats (ms) in stage order and returns the
- * number of stages written; names via svo_stage_name. */
-int svo_enable_timing(svo_ctx* ctx, int enable);
+/* GPU timing with HIP events on the context's stream (off by default).
+ *   mode 0  off
+ *   mode 1  per stage: frames run as direct launches with an event before
+ *           each stage; after svo_sync, svo_stage_times fills up to n floats
+ *           (ms) in stage order and returns the number written (names via
+ *           svo_stage_name)
+ *   mode 2  per frame: an event pair around each frame (graph launches
+ *           included)
+ * In modes 1 and 2, svo_frame_time gives the last frame's GPU time (ms)
+ * once svo_sync has returned. */
+int svo_enable_timing(svo_ctx* ctx, int mode);
+int svo_frame_time(svo_ctx* ctx, float* ms);
+/* svo_frame_warp runs frames >= 2 as one captured HIP graph per context
+ * (launch sequence fixed, the frame's camera and tile in a device-side view
+ * record written before each launch).  On by default; 0 makes every frame
+ * use direct launches.  The graph is re-captured after svo_resize,
+ * svo_upload_octree and svo_set_tuning.  Results are identical either way. */
+int svo_set_graph(svo_ctx* ctx, int enable);
 /* Rays traced by the ray kernels since the last reset (blocking). */
 int svo_ray_count(svo_ctx* ctx, uint64_t* rays, int reset);
 int svo_stage_times(svo_ctx* ctx, float* ms, int n);

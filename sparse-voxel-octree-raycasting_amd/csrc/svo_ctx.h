@@ -30,7 +30,7 @@ struct svo_ctx {
     uint32_t* d_image = nullptr;
     unsigned long long* d_keys = nullptr;
     uint32_t* d_snap = nullptr;
-    uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [8..9] ray count (u64)
+    uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [2..3] the graph's, [8..9] ray count (u64)
     // learned-fill gate: [0] CNN tile count, [1] gate ray count
     uint32_t* d_gate = nullptr;
     uint32_t* d_gate_tiles = nullptr;  // tile indices sent to the CNN
@@ -41,11 +41,22 @@ struct svo_ctx {
     // tuning
     int refill = 16, wg_per_cu = 4, iters = 64;
     uint32_t cache_req = 0;  // requested LDS node-cache words
-    // timing
-    bool timing = false;
+    // per-frame graph of the warp frame (frames >= 2): its kernels read the
+    // frame's view from d_view, refreshed from a pinned 2-slot staging ring
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;
+    void* d_view = nullptr;  // one view descriptor (device)
+    void* h_view = nullptr;  // two view descriptors (pinned host)
+    hipEvent_t view_done[2] = {};
+    uint32_t* d_first = nullptr;  // [0] = 0, [1] = rays of the frame
+    int graph_frames = 0;
+    // timing: 0 off, 1 per stage (direct launches), 2 per frame
+    int timing = 0;
     hipEvent_t ev[ST_N + 1] = {};
     float stage_ms[ST_N] = {};
     bool stage_used[ST_N] = {};
+    bool frame_timed = false;
+    float frame_ms = 0.0f;
 };
 
 

@@ -576,8 +576,9 @@ __global__ __launch_bounds__(1024) void k_sumids(View v, const View* views)
 }
 
 // first ray of each batched view (after their compaction), one thread
-__global__ void k_view_first(const View* views, int n, uint32_t* first)
+__global__ void k_view_first(const View* views, int n, uint32_t* first, uint32_t* zero)
 {
+    if (zero) *zero = 0u;  // the ray-queue counter of the launch that follows
     uint32_t acc = 0;
     for (int i = 0; i < n; i++) {
         first[i] = acc;
@@ -712,6 +713,13 @@ static int free_frame(svo_ctx* c)
     return SVO_OK;
 }
 
+// the captured frame graph holds buffer pointers and launch knobs
+static void drop_graph(svo_ctx* c)
+{
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+}
+
 extern "C" int svo_create(svo_ctx** out, int device)
 {
     if (!out) return svo_fail(SVO_ERR_ARG, "null context pointer");
@@ -742,6 +750,12 @@ extern "C" int svo_destroy(svo_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_frame(c);
+    drop_graph(c);
+    (void)hipFree(c->d_view);
+    (void)hipHostFree(c->h_view);
+    (void)hipFree(c->d_first);
+    for (auto& e : c->view_done)
+        if (e) (void)hipEventDestroy(e);
     (void)hipFree(c->d_oct);
     (void)hipFree(c->d_queue);
     for (int i = 0; i <= ST_N; i++)
@@ -766,6 +780,7 @@ extern "C" int svo_upload_octree(svo_ctx* c, const uint32_t* words, size_t n_wor
     RC(svo_device_layout(words, n_words, root_ref, depth, dev, &new_root, &normal_words));
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
     (void)hipFree(c->d_oct);
     c->d_oct = nullptr;
     HIPCHECK(hipMalloc(&c->d_oct, (n_words + kOctPad) * 4));
@@ -785,6 +800,7 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
         return svo_fail(SVO_ERR_ARG, "resolution %dx%d outside 16..8192", res_x, res_y);
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
     free_frame(c);
     const size_t N = (size_t)res_x * res_y, S = (size_t)(res_x / 16) * (res_y / 16);
     const size_t ids_words = 2 * S + N + 64;
@@ -831,10 +847,22 @@ static int need_octree(svo_ctx* c)
 
 static inline void mark(svo_ctx* c, int st)
 {
-    if (c->timing) {
+    if (c->timing == 1) {
         (void)hipEventRecord(c->ev[st], c->stream);
         c->stage_used[st] = true;
     }
+}
+
+static void frame_begin(svo_ctx* c)
+{
+    for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
+    c->frame_timed = c->timing != 0;
+    if (c->timing == 2) (void)hipEventRecord(c->ev[0], c->stream);
+}
+static int frame_end(svo_ctx* c)
+{
+    if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
+    return SVO_OK;
 }
 
 static View make_view(svo_ctx* c, const svo_frame_params* p)
@@ -990,7 +1018,6 @@ static int launch_rays(svo_ctx* c, const svo_frame_params* p, const uint32_t* li
 static int warp_front(svo_ctx* c, const svo_frame_params* p)
 {
     const Launch L = single(c);
-    for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
     mark(c, ST_CLEAR);
     if (p->frame < 2) {  // raycast.h:150-154
         RC(launch_memset(c, c->d_screen, HOLE, 4 * c->N));
@@ -1015,11 +1042,77 @@ static int warp_front(svo_ctx* c, const svo_frame_params* p)
     return launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
 }
 
+// Capture the warp frame's launches (frames >= 2: no clear) into one graph.
+// Every kernel reads the frame's view from c->d_view; the ray launch uses its
+// own queue counters (d_queue[2..3]), reset by k_view_first inside the graph.
+static int build_graph(svo_ctx* c)
+{
+    if (!c->d_view) {
+        bool ok = hipMalloc(&c->d_view, sizeof(View)) == hipSuccess &&
+                  hipHostMalloc(&c->h_view, 2 * sizeof(View)) == hipSuccess &&
+                  hipMalloc(&c->d_first, 8) == hipSuccess;
+        for (int i = 0; ok && i < 2; i++)
+            ok = hipEventCreateWithFlags(&c->view_done[i], hipEventDisableTiming) == hipSuccess &&
+                 hipEventRecord(c->view_done[i], c->stream) == hipSuccess;
+        if (!ok) return svo_fail(SVO_ERR_HIP, "frame graph buffers");
+    }
+    const View* dv = (const View*)c->d_view;
+    const View ph = make_view(c, nullptr);  // by-value placeholder, unused when a table is given
+    const Launch L{c->stream, dv, 1, (uint32_t)c->N, (uint32_t)c->S};
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int seq = 0;
+    int rc = launch_scatter_resolve(L, ph, 2, 1);
+    if (!rc) rc = launch_count(L, ph);
+    if (!rc) rc = launch_sum(L, ph);
+    if (!rc) rc = launch_write(L, ph);
+    if (!rc) {
+        hipLaunchKernelGGL(k_view_first, dim3(1), dim3(1), 0, c->stream, dv, 1, c->d_first, c->d_queue + 2);
+        if (hipGetLastError() != hipSuccess) rc = svo_fail(SVO_ERR_HIP, "graph capture: k_view_first");
+    }
+    if (!rc) rc = launch_rays_v(c, c->stream, ph, dv, c->d_first, 1, c->d_queue + 2, seq);
+    if (!rc) rc = launch_copy(L, ph, 2);
+    if (!rc) rc = launch_fill(L, ph, nullptr, 1, 1);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (!rc && e != hipSuccess) rc = svo_fail(SVO_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    if (!rc && hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+        c->graph = nullptr;
+        rc = svo_fail(SVO_ERR_HIP, "graph instantiation failed");
+    }
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+static int frame_warp_graph(svo_ctx* c, const svo_frame_params* p)
+{
+    if (!c->graph) RC(build_graph(c));
+    View v = make_view(c, p);
+    v.plane_lo = c->plane1_holes ? 2 : 1;
+    v.n_planes = 3 - v.plane_lo;
+    int ax, ay, ex, ey;
+    svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
+    set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
+    const int slot = c->graph_frames & 1;
+    View* hv = (View*)c->h_view + slot;
+    HIPCHECK(hipEventSynchronize(c->view_done[slot]));  // its previous copy has been consumed
+    *hv = v;
+    HIPCHECK(hipMemcpyAsync(c->d_view, hv, sizeof(View), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipEventRecord(c->view_done[slot], c->stream));
+    HIPCHECK(hipGraphLaunch(c->graph, c->stream));
+    c->graph_frames++;
+    return SVO_OK;
+}
+
 extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
 {
     RC(need_octree(c));
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     HIPCHECK(hipSetDevice(c->device));
+    frame_begin(c);
+    if (c->use_graph && c->timing != 1 && p->frame >= 2) {
+        RC(frame_warp_graph(c, p));
+        return frame_end(c);
+    }
     RC(warp_front(c, p));
     const Launch L = single(c);
     const View v = make_view(c, p);
@@ -1027,8 +1120,7 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     RC(launch_copy(L, v, 2));
     mark(c, ST_FILTER);
     RC(launch_fill(L, v, nullptr, 1, 1));
-    if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
-    return SVO_OK;
+    return frame_end(c);
 }
 
 // Warp frame with the learned fill: after the hole and tile rays, a per-tile
@@ -1042,6 +1134,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     if (max_cnn_holes < 0 || max_cnn_holes > 256) return svo_fail(SVO_ERR_ARG, "max_cnn_holes outside 0..256");
     HIPCHECK(hipSetDevice(c->device));
+    frame_begin(c);
     RC(warp_front(c, p));
     const int tiles = c->n_tiles_x * c->n_tiles_y;
     const int max_tiles = std::min(tiles, 4096);
@@ -1057,8 +1150,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     mark(c, ST_FILTER);
     RC(launch_fill(L, v, nullptr, 1, 1));
     if (cnn) RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, max_tiles));
-    if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
-    return SVO_OK;
+    return frame_end(c);
 }
 
 extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
@@ -1066,11 +1158,10 @@ extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
     RC(need_octree(c));
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     HIPCHECK(hipSetDevice(c->device));
-    for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
+    frame_begin(c);
     mark(c, ST_RAYS);
     RC(launch_rays(c, p, nullptr, nullptr, 0, 0, c->res_x, c->res_y, c->d_image));
-    if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
-    return SVO_OK;
+    return frame_end(c);
 }
 
 extern "C" int svo_memset(svo_ctx* c, size_t byte_ofs, uint32_t val, size_t bytes)
@@ -1272,7 +1363,8 @@ extern "C" int svo_batch_frame_warp(svo_batch* b, const svo_frame_params* params
     RC(launch_count(L, dummy));
     RC(launch_sum(L, dummy));
     RC(launch_write(L, dummy));
-    hipLaunchKernelGGL(k_view_first, dim3(1), dim3(1), 0, b->stream, (const View*)dv, n, b->d_first);
+    hipLaunchKernelGGL(k_view_first, dim3(1), dim3(1), 0, b->stream, (const View*)dv, n, b->d_first,
+                       (uint32_t*)nullptr);
     HIPCHECK(hipGetLastError());
     RC(launch_rays_v(c0, b->stream, dummy, dv, b->d_first, n, b->d_queue, b->seq));
     RC(launch_copy(L, dummy, 2));
@@ -1338,8 +1430,9 @@ extern "C" int svo_sync(svo_ctx* c)
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    if (c->timing) {
-        int last = ST_N;
+    if (c->frame_timed) {
+        c->frame_timed = false;
+        int last = ST_N, first = -1;
         for (int i = ST_N - 1; i >= 0; i--) {
             c->stage_ms[i] = 0.0f;
             if (!c->stage_used[i]) continue;
@@ -1347,7 +1440,11 @@ extern "C" int svo_sync(svo_ctx* c)
             (void)hipEventElapsedTime(&ms, c->ev[i], c->ev[last]);
             c->stage_ms[i] = ms;
             last = i;
+            first = i;
         }
+        c->frame_ms = 0.0f;
+        if (first >= 0 || c->timing == 2)
+            (void)hipEventElapsedTime(&c->frame_ms, c->ev[first >= 0 ? first : 0], c->ev[ST_N]);
     }
     return SVO_OK;
 }
@@ -1368,10 +1465,28 @@ extern "C" int svo_ray_count(svo_ctx* c, uint64_t* rays, int reset)
     return SVO_OK;
 }
 
-extern "C" int svo_enable_timing(svo_ctx* c, int enable)
+extern "C" int svo_enable_timing(svo_ctx* c, int mode)
 {
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
-    c->timing = enable != 0;
+    if (mode < 0 || mode > 2) return svo_fail(SVO_ERR_ARG, "timing mode %d outside 0..2", mode);
+    c->timing = mode;
+    return SVO_OK;
+}
+
+extern "C" int svo_frame_time(svo_ctx* c, float* ms)
+{
+    if (!c || !ms) return svo_fail(SVO_ERR_ARG, "null argument");
+    *ms = c->frame_ms;
+    return SVO_OK;
+}
+
+extern "C" int svo_set_graph(svo_ctx* c, int enable)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->use_graph = enable != 0;
+    if (!c->use_graph) drop_graph(c);
     return SVO_OK;
 }
 
@@ -1395,6 +1510,9 @@ extern "C" int svo_set_tuning(svo_ctx* c, int refill_threshold, int wg_per_cu, i
     if (refill_threshold == 0 || refill_threshold > 64 || wg_per_cu == 0 || wg_per_cu > 32 ||
         iters_per_refill == 0 || iters_per_refill > 4096 || lds_cache_words > 12288)
         return svo_fail(SVO_ERR_ARG, "tuning value out of range");
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
     if (refill_threshold > 0) c->refill = refill_threshold;
     if (wg_per_cu > 0) c->wg_per_cu = wg_per_cu;
     if (iters_per_refill > 0) c->iters = iters_per_refill;

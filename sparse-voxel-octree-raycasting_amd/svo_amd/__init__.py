@@ -88,6 +88,8 @@ _f = {
     "enable_timing": _sig("svo_enable_timing", C.c_int, _ctxp, C.c_int),
     "stage_times": _sig("svo_stage_times", C.c_int, _ctxp, C.POINTER(C.c_float), C.c_int),
     "stage_name": _sig("svo_stage_name", C.c_char_p, C.c_int),
+    "frame_time": _sig("svo_frame_time", C.c_int, _ctxp, C.POINTER(C.c_float)),
+    "set_graph": _sig("svo_set_graph", C.c_int, _ctxp, C.c_int),
     "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int, C.c_int),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
@@ -133,6 +135,7 @@ EXPORTED = [
     "svo_cnn_create", "svo_cnn_destroy", "svo_cnn_param_count", "svo_cnn_get_params", "svo_cnn_set_params",
     "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
+    "svo_frame_time", "svo_set_graph",
 ]
 
 
@@ -322,8 +325,19 @@ class Context:
         _check(_f["ray_count"](self._h, C.byref(v), 1 if reset else 0))
         return v.value
 
-    def enable_timing(self, on=True):
-        _check(_f["enable_timing"](self._h, 1 if on else 0))
+    def enable_timing(self, on=True, mode=None):
+        """on: per-stage timing (mode 1); mode=2: one event pair per frame."""
+        _check(_f["enable_timing"](self._h, int(mode) if mode is not None else (1 if on else 0)))
+
+    def frame_time(self):
+        """GPU ms of the last frame (timing mode 1 or 2, after sync())."""
+        v = C.c_float()
+        _check(_f["frame_time"](self._h, C.byref(v)))
+        return v.value
+
+    def set_graph(self, on=True):
+        """Run warp frames >= 2 as one captured HIP graph (default on)."""
+        _check(_f["set_graph"](self._h, 1 if on else 0))
 
     def stage_times(self):
         buf = (C.c_float * 16)()

@@ -6,7 +6,7 @@ LDS node-cache depth (octree words of the breadth-first top levels kept in
 LDS).  Every candidate replays the same frame window from the same starting
 state (the window starts with the two clearing frames), so candidates see
 identical work; the score is the median GPU time per frame over the timed
-frames, measured with the library's HIP-event stage timer.  Results do not
+frames, measured with the library's per-frame HIP-event pair.  Results do not
 depend on the knobs (the tests check bit-exactness under several settings).
 """
 from __future__ import annotations
@@ -28,12 +28,12 @@ def _frame_time(ctx, params, warmup, mode):
     for p in params[:warmup]:
         (ctx.frame_warp if mode == "warp" else ctx.frame_full)(p)
     ctx.sync()
-    ctx.enable_timing(True)
+    ctx.enable_timing(mode=2)  # event pair per frame, graph launches included
     ts = []
     for p in params[warmup:]:
         (ctx.frame_warp if mode == "warp" else ctx.frame_full)(p)
         ctx.sync()
-        ts.append(sum(ctx.stage_times().values()))
+        ts.append(ctx.frame_time())
     ctx.enable_timing(False)
     return statistics.median(ts)
 

@@ -59,3 +59,53 @@ def test_warp_sequence_config1_bitexact(ctx, scene1):
         assert_equal_u32(ctx.ids(), o.ids, f"frame {f} id buffer")
         assert_equal_u32(ctx.image(), o.image, f"frame {f} image")
     o.close()
+
+
+def test_warp_graph_vs_direct_launches(scene1):
+    """Frames >= 2 replay one captured graph; the same frames as direct
+    launches must give identical buffers, also across a re-capture (tuning
+    change), a resize and a timing mode switch."""
+    words, root = scene1
+    rx, ry = CONFIG1["res"]
+    g = S.Context(0)
+    d = S.Context(0)
+    try:
+        for c in (g, d):
+            c.upload_octree(words, root, 8)
+            c.resize(rx, ry)
+        d.set_graph(False)
+        pr = path(CONFIG1, 20)
+        for f in range(20):
+            if f == 7:
+                g.set_tuning(refill=8, iters=16)  # forces a new capture
+                d.set_tuning(refill=8, iters=16)
+            if f == 11:
+                g.enable_timing(mode=2)
+            if f == 14:
+                g.enable_timing(False)
+            p = params(CONFIG1, pr, f)
+            print(f"graph frame {f}", flush=True)
+            g.frame_warp(p)
+            d.frame_warp(p)
+            g.sync()
+            d.sync()
+            if f == 12:
+                assert g.frame_time() > 0.0
+            assert_equal_u32(g.screen(), d.screen(), f"frame {f} screen")
+            assert_equal_f32_bits(g.coords(), d.coords(), f"frame {f} coords")
+            assert_equal_u32(g.ids(), d.ids(), f"frame {f} ids")
+            assert_equal_u32(g.image(), d.image(), f"frame {f} image")
+        # a new resolution re-captures; restart the sequence on both
+        for c in (g, d):
+            c.resize(200, 136)
+        for f in range(6):
+            p = params(CONFIG1, pr, f)
+            g.frame_warp(p)
+            d.frame_warp(p)
+            g.sync()
+            d.sync()
+            assert_equal_u32(g.screen(), d.screen(), f"resized frame {f} screen")
+            assert_equal_u32(g.image(), d.image(), f"resized frame {f} image")
+    finally:
+        g.close()
+        d.close()

@@ -156,6 +156,20 @@ int svo_octree_save(const char* path, const uint32_t* words, size_t n_words,
 int svo_octree_load(const char* path, uint32_t** words, size_t* n_words,
                     uint32_t* root_ref, int* depth);
 
+/* RLE4 volume files (RLE4::load, src/octree/Rle4.cpp:8-165): map 0 of the
+ * file as a voxel stream for svo_build_octree, in the reference's set_voxel
+ * order (slice, x, rows upward), voxel y = sy-1-row, colours by the rule of
+ * Rle4.cpp:127-160 (palette != 0: palette bits + intensity; 0: inverted
+ * luminance).  Voxels are offset by add_*, and every one must lie inside the
+ * depth-`depth` cube.  Truncated or inconsistent files fail with SVO_ERR_IO
+ * (the reference reads past them).  *xyz / *rgba are malloc'ed (svo_free). */
+int svo_rle4_load(const char* path, int palette, int add_x, int add_y, int add_z, int depth,
+                  uint16_t** xyz, uint32_t** rgba, size_t* n);
+/* Write a one-map RLE4 file from a dense volume: vol[(z*sx + x)*sy + row]
+ * is the 16-bit colour word of file row `row` of column (x, z), or < 0 for
+ * empty. */
+int svo_rle4_save(const char* path, int sx, int sy, int sz, const int32_t* vol);
+
 /* Seeded synthetic scenes (voxel streams for svo_build_octree).
  *   kind 1: value-noise heightfield shell (2 octaves)
  *   kind 2: 4-octave fractal terrain shell + n_spheres sphere shells

@@ -669,6 +669,26 @@ struct VoxOut {
     }
 };
 
+int emit_voxels(VoxOut& vo, uint16_t** xyz, uint32_t** rgba, size_t* n)
+{
+    const size_t nv = vo.rgba.size();
+    uint16_t* a = (uint16_t*)malloc(nv * 6 + 6);
+    uint32_t* c = (uint32_t*)malloc(nv * 4 + 4);
+    if (!a || !c) {
+        free(a);
+        free(c);
+        return svo_fail(SVO_ERR_NOMEM, "voxel stream of %zu voxels", nv);
+    }
+    if (nv) {
+        memcpy(a, vo.xyz.data(), nv * 6);
+        memcpy(c, vo.rgba.data(), nv * 4);
+    }
+    *xyz = a;
+    *rgba = c;
+    *n = nv;
+    return SVO_OK;
+}
+
 // heightfield shell: columns in z -> x -> y order (like Rle4.cpp:95-123)
 void heightfield(VoxOut& vo, const std::vector<int>& H, int size, uint64_t seed, int hmin, int hmax)
 {
@@ -785,20 +805,162 @@ extern "C" int svo_scene_generate(int kind, uint64_t seed, int depth, int n_sphe
     } else {
         return svo_fail(SVO_ERR_ARG, "unknown scene kind %d", kind);
     }
-    size_t nv = vo.rgba.size();
-    uint16_t* a = (uint16_t*)malloc(nv * 6 + 6);
-    uint32_t* c = (uint32_t*)malloc(nv * 4 + 4);
-    if (!a || !c) {
-        free(a);
-        free(c);
-        return svo_fail(SVO_ERR_NOMEM, "scene of %zu voxels", nv);
+    return emit_voxels(vo, xyz, rgba, n);
+}
+
+// ---------------------------------------------------------------- RLE4 volumes
+// File layout read by RLE4::load (src/octree/Rle4.cpp:8-165): int32 map
+// count, then per map int32 sx, sy, sz, slab count and that many uint16
+// slabs.  The slabs hold one record per column, x fastest then z
+// (Rle4.cpp:60-90): [n_runs][n_colours][n_runs run words][n_colours colour
+// words].  A run word is skip = bits 0-9, length = bits 10-15: the run starts
+// `skip` rows after the previous run's end (Rle4.cpp:121-124).  File row r
+// is voxel y = sy-1-r.  Only map 0 becomes voxels (Rle4.cpp:111).
+namespace {
+struct Rle4Map {
+    int sx = 0, sy = 0, sz = 0;
+    std::vector<uint16_t> slabs;
+};
+
+int rle4_read(const char* path, std::vector<Rle4Map>& maps)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) return svo_fail(SVO_ERR_IO, "cannot open %s", path);
+    int32_t nmaps = 0;
+    int rc = SVO_OK;
+    if (fread(&nmaps, 4, 1, f) != 1 || nmaps < 1 || nmaps > 16) {
+        rc = svo_fail(SVO_ERR_IO, "%s: bad map count", path);
     }
-    memcpy(a, vo.xyz.data(), nv * 6);
-    memcpy(c, vo.rgba.data(), nv * 4);
-    *xyz = a;
-    *rgba = c;
-    *n = nv;
-    return SVO_OK;
+    for (int m = 0; rc == SVO_OK && m < nmaps; m++) {
+        int32_t hdr[4];
+        if (fread(hdr, 4, 4, f) != 4 || hdr[0] < 1 || hdr[1] < 1 || hdr[2] < 1 || hdr[3] < 2 || hdr[0] > 65536 ||
+            hdr[1] > 65536 || hdr[2] > 65536) {
+            rc = svo_fail(SVO_ERR_IO, "%s: bad header of map %d", path, m);
+            break;
+        }
+        Rle4Map mp;
+        mp.sx = hdr[0];
+        mp.sy = hdr[1];
+        mp.sz = hdr[2];
+        mp.slabs.resize((size_t)hdr[3]);
+        if (fread(mp.slabs.data(), 2, mp.slabs.size(), f) != mp.slabs.size()) {
+            rc = svo_fail(SVO_ERR_IO, "%s: map %d truncated", path, m);
+            break;
+        }
+        maps.push_back(std::move(mp));
+    }
+    fclose(f);
+    return rc;
+}
+
+// tcol -> uchar4 of Rle4.cpp:127-160, packed x | y<<8 | z<<16 (w = 0)
+uint32_t rle4_color(uint16_t tcol, int palette)
+{
+    const int lum = tcol & 255;
+    int inten = lum * lum / 255;
+    inten = std::min(255, std::max(2, inten));
+    uint32_t x = (uint32_t)(((tcol >> 8) & 3) + ((inten >> 3) << 3));
+    if (!palette) x = 1u + (uint32_t)((255 - lum) & 0xfc);
+    const uint32_t y = (uint32_t)((tcol >> 5) << 3) & 255u;
+    const uint32_t z = (uint32_t)((tcol >> 10) << 3) & 255u;
+    return x | (y << 8) | (z << 16);
+}
+}  // namespace
+
+extern "C" int svo_rle4_load(const char* path, int palette, int add_x, int add_y, int add_z, int depth,
+                             uint16_t** xyz, uint32_t** rgba, size_t* n)
+{
+    if (!path || !xyz || !rgba || !n) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (depth < 1 || depth > 16) return svo_fail(SVO_ERR_ARG, "depth %d outside 1..16", depth);
+    std::vector<Rle4Map> maps;
+    const int rrc = rle4_read(path, maps);
+    if (rrc) return rrc;
+    const long size = 1L << depth;
+    // column offsets of every map (the walk of Rle4.cpp:66-90)
+    std::vector<size_t> col0;
+    for (size_t m = 0; m < maps.size(); m++) {
+        const Rle4Map& mp = maps[m];
+        size_t ofs = 0;
+        const size_t cols = (size_t)mp.sx * mp.sz;
+        if (m == 0) col0.resize(cols);
+        for (size_t c = 0; c < cols; c++) {
+            if (ofs + 2 > mp.slabs.size()) return svo_fail(SVO_ERR_IO, "%s: map %zu column %zu past the slabs", path, m, c);
+            if (m == 0) col0[c] = ofs;
+            ofs += (size_t)mp.slabs[ofs] + mp.slabs[ofs + 1] + 2;
+            if (ofs > mp.slabs.size()) return svo_fail(SVO_ERR_IO, "%s: map %zu column %zu past the slabs", path, m, c);
+        }
+    }
+    const Rle4Map& mp = maps[0];
+    VoxOut vo;
+    for (int z = 0; z < mp.sz; z++)  // slice outer, x inner, rows upward (Rle4.cpp:113-121)
+        for (int x = 0; x < mp.sx; x++) {
+            const size_t ofs = col0[(size_t)z * mp.sx + x];
+            const uint32_t runs = mp.slabs[ofs], cols = mp.slabs[ofs + 1];
+            const uint16_t* run = mp.slabs.data() + ofs + 2;
+            const uint16_t* col = run + runs;
+            uint32_t used = 0;
+            uint32_t r0 = 0;
+            for (uint32_t k = 0; k < runs; k++) {
+                r0 += run[k] & 1023u;
+                const uint32_t r1 = r0 + (run[k] >> 10);
+                for (; r0 < r1; r0++, used++) {
+                    if (used >= cols)
+                        return svo_fail(SVO_ERR_IO, "%s: column (%d,%d) has more voxels than colours", path, x, z);
+                    if (r0 >= (uint32_t)mp.sy) continue;  // Rle4.cpp:126
+                    const long vx = (long)x + add_x, vy = (long)mp.sy - 1 - (long)r0 + add_y, vz = (long)z + add_z;
+                    if (vx < 0 || vy < 0 || vz < 0 || vx >= size || vy >= size || vz >= size)
+                        return svo_fail(SVO_ERR_ARG, "voxel (%ld,%ld,%ld) outside the depth-%d cube", vx, vy, vz,
+                                        depth);
+                    vo.add((int)vx, (int)vy, (int)vz, rle4_color(col[used], palette));
+                }
+            }
+        }
+    return emit_voxels(vo, xyz, rgba, n);
+}
+
+extern "C" int svo_rle4_save(const char* path, int sx, int sy, int sz, const int32_t* vol)
+{
+    if (!path || !vol) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (sx < 1 || sy < 1 || sz < 1 || sx > 65536 || sy > 65536 || sz > 65536)
+        return svo_fail(SVO_ERR_ARG, "bad volume size");
+    std::vector<uint16_t> slabs;
+    std::vector<uint16_t> runs, cols;
+    for (int z = 0; z < sz; z++)
+        for (int x = 0; x < sx; x++) {
+            runs.clear();
+            cols.clear();
+            const int32_t* c = vol + ((size_t)z * sx + x) * sy;
+            uint32_t gap = 0;
+            for (int r = 0; r < sy;) {
+                if (c[r] < 0) {
+                    gap++;
+                    r++;
+                    continue;
+                }
+                int len = 0;
+                while (r + len < sy && c[r + len] >= 0 && len < 63) len++;
+                while (gap > 1023) {  // empty run words carry long gaps
+                    runs.push_back(1023);
+                    gap -= 1023;
+                }
+                runs.push_back((uint16_t)(gap | ((uint32_t)len << 10)));
+                for (int k = 0; k < len; k++) cols.push_back((uint16_t)c[r + k]);
+                r += len;
+                gap = 0;
+            }
+            if (runs.size() > 65535 || cols.size() > 65535) return svo_fail(SVO_ERR_ARG, "column too long");
+            slabs.push_back((uint16_t)runs.size());
+            slabs.push_back((uint16_t)cols.size());
+            slabs.insert(slabs.end(), runs.begin(), runs.end());
+            slabs.insert(slabs.end(), cols.begin(), cols.end());
+        }
+    if (slabs.size() > 0x7fffffff) return svo_fail(SVO_ERR_ARG, "volume too large");
+    FILE* f = fopen(path, "wb");
+    if (!f) return svo_fail(SVO_ERR_IO, "cannot create %s", path);
+    const int32_t hdr[5] = {1, sx, sy, sz, (int32_t)slabs.size()};
+    bool ok = fwrite(hdr, 4, 5, f) == 5 && fwrite(slabs.data(), 2, slabs.size(), f) == slabs.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SVO_OK : svo_fail(SVO_ERR_IO, "write to %s failed", path);
 }
 
 // ---------------------------------------------------------------- paths

@@ -119,6 +119,9 @@ _f = {
     "batch_destroy": _sig("svo_batch_destroy", C.c_int, _ctxp),
     "batch_frame_warp": _sig("svo_batch_frame_warp", C.c_int, _ctxp, C.POINTER(FrameParams)),
     "batch_sync": _sig("svo_batch_sync", C.c_int, _ctxp),
+    "rle4_load": _sig("svo_rle4_load", C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                      C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
+    "rle4_save": _sig("svo_rle4_save", C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)),
     "camera_path": _sig("svo_camera_path", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_float,
                         C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)),
 }
@@ -135,7 +138,7 @@ EXPORTED = [
     "svo_cnn_create", "svo_cnn_destroy", "svo_cnn_param_count", "svo_cnn_get_params", "svo_cnn_set_params",
     "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
-    "svo_frame_time", "svo_set_graph",
+    "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save",
 ]
 
 
@@ -205,6 +208,35 @@ def scene(kind, seed, depth, n_spheres=0):
     _f["free"](C.cast(xyz, _vp))
     _f["free"](C.cast(rgba, _vp))
     return a, b
+
+
+def _take_stream(xyz, rgba, n):
+    nv = n.value
+    a = np.ctypeslib.as_array(xyz, (nv * 3,)).copy().reshape(-1, 3) if nv else np.zeros((0, 3), np.uint16)
+    b = np.ctypeslib.as_array(rgba, (nv,)).copy() if nv else np.zeros(0, np.uint32)
+    _f["free"](C.cast(xyz, _vp))
+    _f["free"](C.cast(rgba, _vp))
+    return a, b
+
+
+def rle4_load(path, depth, palette=1, add=(0, 0, 0)):
+    """Map 0 of an RLE4 volume file as a voxel stream (xyz uint16 [n,3],
+    rgba uint32 [n]) in the reference loader's order and colour rule."""
+    xyz = _u16p()
+    rgba = _u32p()
+    n = C.c_size_t()
+    _check(_f["rle4_load"](str(path).encode(), int(palette), int(add[0]), int(add[1]), int(add[2]), int(depth),
+                           C.byref(xyz), C.byref(rgba), C.byref(n)))
+    return _take_stream(xyz, rgba, n)
+
+
+def rle4_save(path, vol):
+    """Write a one-map RLE4 file; vol is int32 [sz, sx, sy] (file rows along
+    the last axis), < 0 = empty, else the 16-bit colour word."""
+    vol = np.ascontiguousarray(vol, dtype=np.int32)
+    sz, sx, sy = vol.shape
+    _check(_f["rle4_save"](str(path).encode(), int(sx), int(sy), int(sz),
+                           vol.ctypes.data_as(C.POINTER(C.c_int32))))
 
 
 def camera_path(seed, depth, n_frames, max_voxels_per_frame, max_yaw_deg_per_frame,

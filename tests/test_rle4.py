@@ -1,0 +1,141 @@
+"""RLE4 volume loader (RLE4::load, src/octree/Rle4.cpp:8-165): a hand-built
+file with hand-derived voxels and colours, writer round trips, long gaps and
+runs, malformed files."""
+import struct
+
+import numpy as np
+import pytest
+
+import svo_amd as S
+
+
+def _file(maps):
+    """maps: list of (sx, sy, sz, slabs)."""
+    out = struct.pack("<i", len(maps))
+    for sx, sy, sz, slabs in maps:
+        out += struct.pack("<4i", sx, sy, sz, len(slabs)) + struct.pack(f"<{len(slabs)}H", *slabs)
+    return out
+
+
+# sx=2, sy=4, sz=1.  Column x=0: one run (skip 1, length 2) with colours
+# 0x0155, 0x02ff.  Column x=1: runs (skip 0, length 1) and (skip 2, length 1)
+# with colours 0x0080, 0x7c00.
+HAND = (2, 4, 1, [1, 2, 1 | (2 << 10), 0x0155, 0x02FF,
+                  2, 2, 0 | (1 << 10), 2 | (1 << 10), 0x0080, 0x7C00])
+# Voxels in loader order (slice, x, rows upward); voxel y = sy-1-row.
+#  row 1 of x=0: tcol 0x0155: lum 85, 85*85//255 = 28, palette (0x155>>8)&3 = 1
+#     -> x byte 1 + (28>>3<<3) = 25; y byte (0x155>>5)<<3 = 80; z byte 0
+#  row 2 of x=0: tcol 0x02ff: lum 255 -> 255, palette 2 -> 2 + 248 = 250;
+#     y byte (0x2ff>>5 = 23)<<3 = 184; z 0
+#  row 0 of x=1: tcol 0x0080: lum 128, 16384//255 = 64, palette 0 -> 64;
+#     y byte 4<<3 = 32; z 0
+#  row 3 of x=1: tcol 0x7c00: lum 0 -> clamped to 2, palette 0 -> 0 + 0 = 0;
+#     y byte (0x3e0<<3) & 255 = 0; z byte 31<<3 = 248
+HAND_XYZ = [(0, 2, 0), (0, 1, 0), (1, 3, 0), (1, 0, 0)]
+HAND_RGBA_PAL = [25 | 80 << 8, 250 | 184 << 8, 64 | 32 << 8, 0 | 248 << 16]
+# palette 0: x byte = 1 + ((255 - lum) & 0xfc)
+HAND_X_NOPAL = [1 + (170 & 0xFC), 1 + (0 & 0xFC), 1 + (127 & 0xFC), 1 + (255 & 0xFC)]
+
+
+def test_hand_built_file(tmp_path):
+    p = tmp_path / "hand.rle4"
+    p.write_bytes(_file([HAND]))
+    xyz, rgba = S.rle4_load(p, depth=3)
+    assert [tuple(int(v) for v in r) for r in xyz] == HAND_XYZ
+    assert [int(v) for v in rgba] == HAND_RGBA_PAL
+    xyz0, rgba0 = S.rle4_load(p, depth=3, palette=0)
+    assert np.array_equal(xyz0, xyz)
+    assert [int(v) & 255 for v in rgba0] == HAND_X_NOPAL
+    assert [int(v) >> 8 for v in rgba0] == [int(v) >> 8 for v in rgba]
+    # offsets
+    xyz2, _ = S.rle4_load(p, depth=4, add=(3, 5, 7))
+    assert np.array_equal(xyz2.astype(int), xyz.astype(int) + np.array([3, 5, 7]))
+
+
+def test_only_map_zero_becomes_voxels(tmp_path):
+    second = (1, 2, 1, [1, 1, 1 << 10, 0x00FF])
+    p = tmp_path / "two.rle4"
+    p.write_bytes(_file([HAND, second]))
+    xyz, rgba = S.rle4_load(p, depth=3)
+    assert len(xyz) == 4 and [int(v) for v in rgba] == HAND_RGBA_PAL
+
+
+def test_rows_at_or_past_sy_are_skipped_but_consume_colours(tmp_path):
+    # sy = 2, one run of 3 rows starting at row 1: rows 1 (kept), 2, 3 (past sy)
+    m = (1, 2, 1, [2, 4, 1 | (3 << 10), 0 | (1 << 10), 0x0010, 0x0020, 0x0030, 0x0040])
+    p = tmp_path / "clip.rle4"
+    p.write_bytes(_file([m]))
+    xyz, rgba = S.rle4_load(p, depth=2)
+    # second run starts at row 4 (past sy): nothing; only row 1 of the first
+    assert [tuple(int(v) for v in r) for r in xyz] == [(0, 0, 0)]
+    lum = 0x10
+    assert int(rgba[0]) & 255 == ((max(2, lum * lum // 255) >> 3) << 3)
+
+
+def _expected(vol, palette):
+    sz, sx, sy = vol.shape
+    xyz, tc = [], []
+    for z in range(sz):
+        for x in range(sx):
+            for r in range(sy):
+                if vol[z, x, r] >= 0:
+                    xyz.append((x, sy - 1 - r, z))
+                    tc.append(int(vol[z, x, r]))
+    return np.array(xyz, dtype=np.int64).reshape(-1, 3), tc
+
+
+@pytest.mark.parametrize("sy", [7, 300, 3000])
+def test_writer_round_trip(tmp_path, sy):
+    rng = np.random.default_rng(sy)
+    sx, sz = 5, 4
+    vol = np.where(rng.random((sz, sx, sy)) < 0.3, rng.integers(0, 65536, (sz, sx, sy)), -1).astype(np.int32)
+    if sy == 3000:  # long gap (> 1023 rows) and a long run (> 63 rows)
+        vol[1, 2, :] = -1
+        vol[1, 2, 2500:2700] = 0x0123
+    p = tmp_path / "rt.rle4"
+    S.rle4_save(p, vol)
+    xyz, rgba = S.rle4_load(p, depth=12)
+    exp_xyz, tc = _expected(vol, 1)
+    assert np.array_equal(xyz.astype(np.int64), exp_xyz)
+    # colours: compare against the hand-pinned rule through the loader itself on
+    # single-voxel files of the same words
+    uniq = sorted(set(tc))[:50]
+    for t in uniq:
+        one = np.full((1, 1, 1), t, np.int32)
+        S.rle4_save(tmp_path / "one.rle4", one)
+        _, c1 = S.rle4_load(tmp_path / "one.rle4", depth=1)
+        got = {int(c) for c, tt in zip(rgba, tc) if tt == t}
+        assert got == {int(c1[0])}
+
+
+def test_malformed_files(tmp_path):
+    bad = tmp_path / "bad.rle4"
+    raw = _file([HAND])
+    bad.write_bytes(raw[:-2])  # truncated slabs
+    with pytest.raises(S.SvoError):
+        S.rle4_load(bad, depth=3)
+    # more voxels than colours
+    bad.write_bytes(_file([(1, 4, 1, [1, 1, 0 | (3 << 10), 0x0011])]))
+    with pytest.raises(S.SvoError):
+        S.rle4_load(bad, depth=3)
+    # column table runs past the slabs
+    bad.write_bytes(_file([(2, 4, 1, [1, 2, 1 | (2 << 10), 0x0155, 0x02FF])]))
+    with pytest.raises(S.SvoError):
+        S.rle4_load(bad, depth=3)
+    # voxel outside the cube
+    good = tmp_path / "good.rle4"
+    good.write_bytes(_file([HAND]))
+    with pytest.raises(S.SvoError):
+        S.rle4_load(good, depth=2, add=(3, 0, 0))
+    with pytest.raises(S.SvoError):
+        S.rle4_load(tmp_path / "missing.rle4", depth=3)
+
+
+def test_loaded_volume_builds_an_octree(tmp_path):
+    rng = np.random.default_rng(5)
+    vol = np.where(rng.random((16, 16, 16)) < 0.2, rng.integers(0, 65536, (16, 16, 16)), -1).astype(np.int32)
+    p = tmp_path / "v.rle4"
+    S.rle4_save(p, vol)
+    xyz, rgba = S.rle4_load(p, depth=8, add=(100, 100, 100))
+    words, root = S.build_octree(xyz, rgba, 8)
+    assert len(words) > 0 and root != 0

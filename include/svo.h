@@ -170,6 +170,10 @@ int svo_rle4_load(const char* path, int palette, int add_x, int add_y, int add_z
  * empty. */
 int svo_rle4_save(const char* path, int sx, int sy, int sz, const int32_t* vol);
 
+/* Write a colorized image (SVO_BUF_IMAGE words, b + g<<8 + r<<16, the PBO
+ * layout of raycast.h:425-455) as a binary PPM, rows top to bottom. */
+int svo_image_save_ppm(const char* path, const uint32_t* image, int w, int h);
+
 /* Seeded synthetic scenes (voxel streams for svo_build_octree).
  *   kind 1: value-noise heightfield shell (2 octaves)
  *   kind 2: 4-octave fractal terrain shell + n_spheres sphere shells

@@ -1003,3 +1003,23 @@ extern "C" int svo_camera_path(uint64_t seed, int depth, int n_frames, float max
     }
     return SVO_OK;
 }
+
+// ---------------------------------------------------------------- image output
+// The colorized words are the PBO layout b + g<<8 + r<<16 that raycast.h
+// uploads with glTexSubImage2D(GL_BGRA) (raycast.h:425-455); a binary PPM
+// stores r, g, b bytes per pixel, rows top to bottom.
+extern "C" int svo_image_save_ppm(const char* path, const uint32_t* image, int w, int h)
+{
+    if (!path || !image || w < 1 || h < 1) return svo_fail(SVO_ERR_ARG, "bad image arguments");
+    std::vector<unsigned char> rgb((size_t)w * h * 3);
+    for (size_t i = 0; i < (size_t)w * h; i++) {
+        rgb[3 * i + 0] = (unsigned char)(image[i] >> 16);
+        rgb[3 * i + 1] = (unsigned char)(image[i] >> 8);
+        rgb[3 * i + 2] = (unsigned char)image[i];
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) return svo_fail(SVO_ERR_IO, "cannot create %s", path);
+    bool ok = fprintf(f, "P6\n%d %d\n255\n", w, h) > 0 && fwrite(rgb.data(), 1, rgb.size(), f) == rgb.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SVO_OK : svo_fail(SVO_ERR_IO, "write to %s failed", path);
+}

@@ -122,6 +122,7 @@ _f = {
     "rle4_load": _sig("svo_rle4_load", C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                       C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
     "rle4_save": _sig("svo_rle4_save", C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)),
+    "image_save_ppm": _sig("svo_image_save_ppm", C.c_int, C.c_char_p, _u32p, C.c_int, C.c_int),
     "camera_path": _sig("svo_camera_path", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_float,
                         C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)),
 }
@@ -138,7 +139,7 @@ EXPORTED = [
     "svo_cnn_create", "svo_cnn_destroy", "svo_cnn_param_count", "svo_cnn_get_params", "svo_cnn_set_params",
     "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
-    "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save",
+    "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
 ]
 
 
@@ -228,6 +229,13 @@ def rle4_load(path, depth, palette=1, add=(0, 0, 0)):
     _check(_f["rle4_load"](str(path).encode(), int(palette), int(add[0]), int(add[1]), int(add[2]), int(depth),
                            C.byref(xyz), C.byref(rgba), C.byref(n)))
     return _take_stream(xyz, rgba, n)
+
+
+def save_ppm(path, image):
+    """Write a colorized image (uint32 [h, w], b + g<<8 + r<<16) as binary PPM."""
+    img = np.ascontiguousarray(image, dtype=np.uint32)
+    h, w = img.shape
+    _check(_f["image_save_ppm"](str(path).encode(), img.ctypes.data_as(_u32p), int(w), int(h)))
 
 
 def rle4_save(path, vol):

@@ -139,3 +139,12 @@ def test_loaded_volume_builds_an_octree(tmp_path):
     xyz, rgba = S.rle4_load(p, depth=8, add=(100, 100, 100))
     words, root = S.build_octree(xyz, rgba, 8)
     assert len(words) > 0 and root != 0
+
+
+def test_ppm_writer(tmp_path):
+    img = np.array([[0x00112233, 0x00FF0080], [0x00000000, 0x00ABCDEF]], np.uint32)
+    p = tmp_path / "a.ppm"
+    S.save_ppm(p, img)
+    raw = p.read_bytes()
+    assert raw.startswith(b"P6\n2 2\n255\n")
+    assert raw[len(b"P6\n2 2\n255\n"):] == bytes([0x11, 0x22, 0x33, 0xFF, 0x00, 0x80, 0, 0, 0, 0xAB, 0xCD, 0xEF])

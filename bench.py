@@ -37,8 +37,21 @@ def run_gpu(args, name):
 
     w = W.WORKLOADS[name]
     t = time.time()
+    xyz, rgba = W.voxels(name)
+    t_scene = time.time() - t
+    t = time.perf_counter()
     words, root, depth, nvox = W.octree(name)
-    log(f"[bench] scene {name}: {nvox} voxels -> {len(words)} words in {time.time() - t:.1f}s")
+    t_host = time.perf_counter() - t
+    log(f"[bench] scene {name}: {nvox} voxels ({t_scene:.1f}s) -> {len(words)} words in {t_host:.2f}s (host builder)")
+    # the same build on the GPU (identical words; checked here as well as in the tests)
+    S.build_octree_gpu(xyz[:1000], rgba[:1000], w["depth"])  # warm-up: library code load
+    t = time.perf_counter()
+    gwords, groot = S.build_octree_gpu(xyz, rgba, w["depth"])
+    t_gpu = time.perf_counter() - t
+    build = dict(voxels=int(nvox), words=int(len(words)), host_s=round(t_host, 3), gpu_s=round(t_gpu, 3),
+                 identical=bool(groot == root and np.array_equal(gwords, words)))
+    del gwords
+    log(f"[bench] GPU builder: {build}")
     rx, ry = w["res"]
     n_frames = w["frames"]
     total = args.warmup + args.steps
@@ -52,7 +65,7 @@ def run_gpu(args, name):
     for c in (warp, full):
         c.upload_octree(words, root, depth)
         c.resize(rx, ry)
-    res = {}
+    res = {"octree_build": build}
     if args.tune:
         from svo_amd import autotune as A
         tp = [W.frame_params(name, rows, f) for f in range(min(total, 40))]
@@ -363,6 +376,7 @@ def main():
             "fps": round(g["learned_fps"], 2), "speedup_vs_full": round(g["learned_fps"] / g["full_fps"], 3),
             "psnr_vs_full_db": g["psnr_modes"], "psnr_hole_pixels_db": g["psnr_hole_pixels"],
             "max_cnn_holes_per_tile": args.max_cnn_holes},
+        "octree_build": g["octree_build"],
         "batched_views": batch,
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {

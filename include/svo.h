@@ -148,6 +148,11 @@ int svo_build_octree(const uint16_t* xyz, const uint32_t* rgba, size_t n,
                      int depth, uint32_t** words, size_t* n_words,
                      uint32_t* root_ref);
 void svo_free(void* p);
+/* svo_build_octree on a GPU (device index): same words and root ref, built
+ * with a Morton sort and per-level scans instead of the host recursion.  No
+ * counterpart in the reference (its builder is the host code above). */
+int svo_build_octree_gpu(int device, const uint16_t* xyz, const uint32_t* rgba, size_t n,
+                         int depth, uint32_t** words, size_t* n_words, uint32_t* root_ref);
 
 /* Save / load a built octree (word array + root ref + depth) as a small
  * binary file, so large scenes are built once. */

@@ -36,12 +36,20 @@ def run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     host_o = os.path.join(BUILD, "svo_host.o")
-    run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "svo_host.cpp"), "-o", host_o])
+    cmds = [["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "svo_host.cpp"), "-o", host_o]]
     objs = [host_o]
-    for src in ("svo_device.hip", "svo_cnn.hip"):
+    for src in ("svo_device.hip", "svo_cnn.hip", "svo_build.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
+    # the translation units are independent: compile them concurrently
+    procs = []
+    for cmd in cmds:
+        print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    failed = [cmd for cmd, pr in zip(cmds, procs) if pr.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     out = os.path.join(OUT_DIR, "libsvo.so")
     tmp = out + ".tmp"
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-pthread"])

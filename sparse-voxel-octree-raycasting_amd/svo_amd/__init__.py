@@ -99,6 +99,8 @@ _f = {
     "build_octree": _sig("svo_build_octree", C.c_int, _u16p, _u32p, C.c_size_t, C.c_int,
                          C.POINTER(_u32p), C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)),
     "free": _sig("svo_free", None, _vp),
+    "build_octree_gpu": _sig("svo_build_octree_gpu", C.c_int, C.c_int, _u16p, _u32p, C.c_size_t, C.c_int,
+                             C.POINTER(_u32p), C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)),
     "octree_save": _sig("svo_octree_save", C.c_int, C.c_char_p, _u32p, C.c_size_t, C.c_uint32, C.c_int),
     "octree_load": _sig("svo_octree_load", C.c_int, C.c_char_p, C.POINTER(_u32p),
                         C.POINTER(C.c_size_t), C.POINTER(C.c_uint32), C.POINTER(C.c_int)),
@@ -140,6 +142,7 @@ EXPORTED = [
     "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
+    "svo_build_octree_gpu",
 ]
 
 
@@ -179,6 +182,20 @@ def build_octree(xyz, rgba, depth):
     root = C.c_uint32()
     _check(_f["build_octree"](xyz.ctypes.data_as(_u16p), rgba.ctypes.data_as(_u32p), len(rgba),
                               int(depth), C.byref(out), C.byref(n), C.byref(root)))
+    return _take_u32(out, n.value), root.value
+
+
+def build_octree_gpu(xyz, rgba, depth, device=0):
+    """build_octree on the GPU: identical words and root ref."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.uint16).reshape(-1, 3)
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint32).reshape(-1)
+    if len(xyz) != len(rgba):
+        raise ValueError("xyz and rgba lengths differ")
+    out = _u32p()
+    n = C.c_size_t()
+    root = C.c_uint32()
+    _check(_f["build_octree_gpu"](int(device), xyz.ctypes.data_as(_u16p), rgba.ctypes.data_as(_u32p), len(rgba),
+                                  int(depth), C.byref(out), C.byref(n), C.byref(root)))
     return _take_u32(out, n.value), root.value
 
 

@@ -27,11 +27,18 @@ WORKLOADS = {
 }
 
 
+@functools.lru_cache(maxsize=2)
+def voxels(name):
+    """(xyz, rgba) voxel stream of a workload's scene."""
+    w = WORKLOADS[name]
+    return scene(w["kind"], w["seed"], w["depth"], w["spheres"])
+
+
 @functools.lru_cache(maxsize=4)
 def octree(name):
     """(words, root_ref, depth, n_voxels) of a workload's scene."""
     w = WORKLOADS[name]
-    xyz, rgba = scene(w["kind"], w["seed"], w["depth"], w["spheres"])
+    xyz, rgba = voxels(name)
     words, root = build_octree(xyz, rgba, w["depth"])
     return words, root, w["depth"], len(rgba)
 

@@ -95,13 +95,13 @@ def test_batch_interleaved_with_single_frames_vs_oracle():
 
 def test_batch_rejects_mismatched_octrees():
     a_words, a_root = scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
-    b_words, b_root = scene_octree(1, 7, 7)
+    b_words, b_root = scene_octree(1, 7, 9)
     a = S.Context(0)
     b = S.Context(0)
     try:
         a.upload_octree(a_words, a_root, CONFIG1["depth"])
         a.resize(64, 64)
-        b.upload_octree(b_words, b_root, 7)
+        b.upload_octree(b_words, b_root, 9)
         b.resize(64, 64)
         with pytest.raises(S.SvoError):
             S.Batch([a, b])

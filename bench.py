@@ -351,6 +351,14 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "per_launch_bytes": round(bytes_launch), "avg_launch_ms": round(ray_ms, 4),
                 "fetch_words_per_ray": round(fbar, 2)}
+        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # runs of this workload, scaled by a copy kernel of known size
+        tpath = os.path.join(ROOT, "profiles", "k_rays_traffic.json")
+        if os.path.exists(tpath) and args.workload == "flythrough_2048":
+            with open(tpath) as fh:
+                t = json.load(fh)
+            roof["traffic"] = t["traffic_bytes"]
+            roof["traffic_source"] = t["source"]
     wl = g
     out = {
         "metric": METRIC,

@@ -257,11 +257,13 @@ int svo_cnn_gradients(svo_cnn* cnn, const uint16_t* x, const float* target, cons
  * and returns the step's loss. */
 int svo_cnn_train_step(svo_cnn* cnn, svo_ctx* warp, svo_ctx* full, const svo_frame_params* p,
                        int n_crops, int crop, uint64_t seed, float lr, float* loss_out);
-/* svo_frame_warp with the learned fill and per-tile confidence gate: after
- * the hole and tile rays, each 16x16 tile with more than max_cnn_holes hole
- * pixels has them raycast; tiles with fewer are filled by the CNN (image
- * only; screen words keep fillhole2's result).  cnn may be NULL: the gate
- * alone, fillhole2 colours everywhere. */
+/* svo_frame_warp with the learned fill and per-tile confidence gate.  After
+ * hole compaction each 16x16 tile counts its residual holes: hole pixels that
+ * neither the hole list nor the refresh tile will trace.  A tile with more
+ * than max_cnn_holes of them has them raycast in the same ray launch as the
+ * hole and tile rays; a tile with fewer is filled by the CNN (image only;
+ * screen words keep fillhole2's result).  cnn may be NULL: the gate alone,
+ * fillhole2 colours everywhere. */
 int svo_frame_warp_learned(svo_ctx* ctx, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
 
 /* ---- batched views (no counterpart in the reference) -------------------

@@ -809,10 +809,13 @@ void orc_frame_warp(orc_state* s, const orc_params* p, int inplace_filter)
     orc_colorize(s);
 }
 
-/* Gated warp frame (no reference counterpart): after the hole and tile rays,
- * every 16x16 tile (partial tiles included) with more than max_holes hole
- * pixels has those pixels raycast, in tile order; the CNN part of the learned
- * fill is not restated here. */
+/* Gated warp frame (no reference counterpart).  After hole compaction, the
+ * residual holes of a 16x16 tile (partial tiles included) are its hole pixels
+ * that neither the hole list (all-hole 2x2 quads of full 16x16 blocks) nor the
+ * refresh rectangle will trace; a tile with more than max_holes of them has
+ * those pixels raycast along with the hole and tile rays.  Rays write only
+ * their own pixel, so their order does not matter.  The CNN part of the
+ * learned fill is not restated here. */
 void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes)
 {
     const int rx = s->res_x, ry = s->res_y;
@@ -824,10 +827,25 @@ void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes)
     orc_counthole(s);
     orc_sumids(s);
     orc_writeids(s);
-    if (s->ids[0] > 0) orc_raycast_holes(s, p);
     int ax, ay, ex, ey;
     orc_tile_schedule(rx, ry, p->frame, &ax, &ay, &ex, &ey);
-    orc_raycast_rect(s, p, ax, ay, ex, ey);
+    /* residual holes: hole pixels that neither the hole list (all-hole 2x2
+     * quads of the full 16x16 blocks) nor the refresh rectangle traces */
+    const int fx = (rx / 16) * 16, fy = (ry / 16) * 16;
+    uint8_t* resid = (uint8_t*)calloc(N, 1);
+    for (int y = 0; y < ry; y++)
+        for (int x = 0; x < rx; x++) {
+            size_t o = (size_t)y * rx + x;
+            if (s->screen[o] != SCREEN_HOLE) continue;
+            if (x < fx && y < fy) {
+                size_t q = (size_t)(y & ~1) * rx + (x & ~1);
+                if (s->screen[q] == SCREEN_HOLE && s->screen[q + 1] == SCREEN_HOLE && s->screen[q + rx] == SCREEN_HOLE &&
+                    s->screen[q + rx + 1] == SCREEN_HOLE)
+                    continue;
+            }
+            if (x >= ax && x < ax + ex && y >= ay && y < ay + ey) continue;
+            resid[o] = 1;
+        }
     int tx = (rx + 15) / 16, ty = (ry + 15) / 16;
     uint32_t* list = (uint32_t*)malloc(N * 4);
     size_t n = 0;
@@ -835,13 +853,16 @@ void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes)
         int x0 = (t % tx) * 16, y0 = (t / tx) * 16, h = 0;
         for (int j = 0; j < 16; j++)
             for (int i = 0; i < 16; i++)
-                if (x0 + i < rx && y0 + j < ry && s->screen[(size_t)(y0 + j) * rx + x0 + i] == SCREEN_HOLE) h++;
+                if (x0 + i < rx && y0 + j < ry && resid[(size_t)(y0 + j) * rx + x0 + i]) h++;
         if (h <= max_holes) continue;
         for (int j = 0; j < 16; j++)
             for (int i = 0; i < 16; i++)
-                if (x0 + i < rx && y0 + j < ry && s->screen[(size_t)(y0 + j) * rx + x0 + i] == SCREEN_HOLE)
+                if (x0 + i < rx && y0 + j < ry && resid[(size_t)(y0 + j) * rx + x0 + i])
                     list[n++] = (uint32_t)(x0 + i) | ((uint32_t)(y0 + j) << 16);
     }
+    free(resid);
+    if (s->ids[0] > 0) orc_raycast_holes(s, p);
+    orc_raycast_rect(s, p, ax, ay, ex, ey);
     for (size_t k = 0; k < n; k++) {
         int x = (int)(list[k] & 0xffff), y = (int)(list[k] >> 16);
         float xyz[3];

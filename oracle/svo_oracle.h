@@ -115,6 +115,10 @@ void orc_tile_schedule(int res_x, int res_y, int frame, int* add_x, int* add_y,
  * inplace_filter == 0) and full raycast (raycast_fine_2 over the screen). */
 void orc_frame_warp(orc_state* s, const orc_params* p, int inplace_filter);
 void orc_frame_full(orc_state* s, const orc_params* p);
+/* gated warp frame (learned-fill gate, no reference counterpart): tiles with
+ * more than max_holes residual holes -- hole pixels that neither the hole
+ * list nor the refresh rectangle traces -- have them raycast with the frame's
+ * hole and tile rays */
 void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes);
 
 /* exhaustive-check helpers */

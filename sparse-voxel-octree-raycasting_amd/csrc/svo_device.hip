@@ -119,6 +119,8 @@ struct View {
     unsigned long long* keys;  // warp keys (N)
     const uint32_t* list;       // ray pixel list (x | y << 16), or null
     const uint32_t* list_count; // its length, read on the device
+    const uint32_t* list2;       // second ray pixel list, or null
+    const uint32_t* list2_count; // its length, read on the device
     int res_x, res_y;
     uint32_t N, S;
     float m0x, m0y, m0z;
@@ -183,7 +185,7 @@ __device__ __forceinline__ uint32_t colorize_word(uint32_t word)
 __device__ __forceinline__ uint32_t view_rays(const View& V)
 {
     // a pixel list holds at most N entries
-    return (V.list ? min(*V.list_count, V.N) : 0u) + V.n_rect;
+    return (V.list ? min(*V.list_count, V.N) : 0u) + (V.list2 ? min(*V.list2_count, V.N) : 0u) + V.n_rect;
 }
 
 // Ray set-up: kernel.cl:883-908 and CAST_RAY prologue :115-150.
@@ -202,13 +204,14 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
     }
     const View& V = a.views ? a.views[vi] : a.v;
     const uint32_t H = V.list ? min(*V.list_count, V.N) : 0u;
+    const uint32_t H2 = V.list2 ? min(*V.list2_count, V.N) : 0u;
     int x, y;
-    if (k < H) {
-        uint32_t w = V.list[k];
+    if (k < H + H2) {
+        const uint32_t w = (k < H) ? V.list[k] : V.list2[k - H];
         x = (int)(w & 0xffffu);
         y = (int)(w >> 16);
     } else {
-        k -= H;
+        k -= H + H2;
         uint32_t t = k >> 6, w = k & 63u;
         int tx = (int)(t % (uint32_t)V.tiles_x), ty = (int)(t / (uint32_t)V.tiles_x);
         int lx = tx * 8 + (int)(w & 7u), ly = ty * 8 + (int)(w >> 3);
@@ -647,16 +650,22 @@ __global__ __launch_bounds__(256) void k_fill_color(View v, const View* views, c
 // ---------------------------------------------------------------- learned-fill gate
 __global__ void k_gate_reset(uint32_t* gate) { gate[0] = 0u; gate[1] = 0u; }
 
-// One 16x16 tile per block: count its holes (plane 0 after the hole and tile
-// rays); 0 < holes <= max_cnn -> tile goes to the CNN, more -> its hole pixels
-// go back to raycasting.
-__global__ __launch_bounds__(256) void k_gate(const uint32_t* __restrict__ screen, int res_x, int res_y,
-                                              int tiles_x, int max_cnn, uint32_t* gate, uint32_t* tiles,
+// One 16x16 tile per block, after compaction and before the rays: count the
+// tile's residual holes -- hole pixels that neither the hole list (all-hole
+// 2x2 quads of the full 16x16 blocks) nor the refresh rectangle will trace.
+// 0 < residual <= max_cnn -> the tile goes to the CNN; more -> its residual
+// pixels join the frame's ray launch as a second list.
+__global__ __launch_bounds__(256) void k_gate(View v, int tiles_x, int max_cnn, uint32_t* gate, uint32_t* tiles,
                                               uint32_t* rays, uint32_t max_tiles)
 {
     const int t = blockIdx.x;
     const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
-    const bool hole = x < res_x && y < res_y && screen[(size_t)y * res_x + x] == HOLE;
+    const int res_x = v.res_x, res_y = v.res_y;
+    bool hole = x < res_x && y < res_y && v.screen[(size_t)y * res_x + x] == HOLE;
+    if (hole && x < (res_x / 16) * 16 && y < (res_y / 16) * 16 && quad_is_hole(v.screen, res_x, x & ~1, y & ~1))
+        hole = false;  // in the hole list
+    if (hole && x >= v.add_x && x < v.add_x + v.ext_x && y >= v.add_y && y < v.add_y + v.ext_y)
+        hole = false;  // in the refresh rectangle
     const int n = __syncthreads_count(hole);
     if (n == 0) return;
     if (n <= max_cnn) {
@@ -926,17 +935,25 @@ struct Launch {
 };
 static Launch single(svo_ctx* c) { return Launch{c->stream, nullptr, 1, (uint32_t)c->N, (uint32_t)c->S}; }
 
-static int launch_scatter_resolve(const Launch& L, const View& v, int max_planes, int clear)
+static int launch_scatter(const Launch& L, const View& v, int max_planes)
 {
-    const unsigned gx = (L.maxN + 255) / 256;
-    if (max_planes > 0) {
-        hipLaunchKernelGGL(k_proj_scatter, dim3(gx, (unsigned)max_planes, (unsigned)L.n), dim3(256), 0, L.stream, v,
-                           L.views);
-        HIPCHECK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_proj_resolve, dim3(gx, 1, (unsigned)L.n), dim3(256), 0, L.stream, v, L.views, clear);
+    if (max_planes <= 0) return SVO_OK;
+    hipLaunchKernelGGL(k_proj_scatter, dim3((L.maxN + 255) / 256, (unsigned)max_planes, (unsigned)L.n), dim3(256), 0,
+                       L.stream, v, L.views);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
+}
+static int launch_resolve(const Launch& L, const View& v, int clear)
+{
+    hipLaunchKernelGGL(k_proj_resolve, dim3((L.maxN + 255) / 256, 1, (unsigned)L.n), dim3(256), 0, L.stream, v,
+                       L.views, clear);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
+static int launch_scatter_resolve(const Launch& L, const View& v, int max_planes, int clear)
+{
+    RC(launch_scatter(L, v, max_planes));
+    return launch_resolve(L, v, clear);
 }
 
 static int launch_count(const Launch& L, const View& v)
@@ -1014,8 +1031,8 @@ static int launch_rays(svo_ctx* c, const svo_frame_params* p, const uint32_t* li
     return launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq);
 }
 
-// the warp stages of raycast_draw up to the hole and tile rays
-static int warp_front(svo_ctx* c, const svo_frame_params* p)
+// the warp stages of raycast_draw up to the hole compaction
+static int warp_compact(svo_ctx* c, const svo_frame_params* p)
 {
     const Launch L = single(c);
     mark(c, ST_CLEAR);
@@ -1029,13 +1046,21 @@ static int warp_front(svo_ctx* c, const svo_frame_params* p)
     v.plane_lo = c->plane1_holes ? 2 : 1;
     v.n_planes = 3 - v.plane_lo;
     mark(c, ST_SCATTER);
-    RC(launch_scatter_resolve(L, v, v.n_planes, 1));
+    RC(launch_scatter(L, v, v.n_planes));
+    mark(c, ST_RESOLVE);
+    RC(launch_resolve(L, v, 1));
     mark(c, ST_COUNT);
     RC(launch_count(L, v));
     mark(c, ST_SCAN);
     RC(launch_sum(L, v));
     mark(c, ST_WRITE);
-    RC(launch_write(L, v));
+    return launch_write(L, v);
+}
+
+// the warp stages of raycast_draw up to and including the hole and tile rays
+static int warp_front(svo_ctx* c, const svo_frame_params* p)
+{
+    RC(warp_compact(c, p));
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
     mark(c, ST_RAYS);
@@ -1135,21 +1160,27 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     if (max_cnn_holes < 0 || max_cnn_holes > 256) return svo_fail(SVO_ERR_ARG, "max_cnn_holes outside 0..256");
     HIPCHECK(hipSetDevice(c->device));
     frame_begin(c);
-    RC(warp_front(c, p));
+    RC(warp_compact(c, p));
+    int ax, ay, ex, ey;
+    svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
+    View v = make_view(c, p);
+    set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
+    v.list2 = c->d_gate_rays;
+    v.list2_count = c->d_gate + 1;
     const int tiles = c->n_tiles_x * c->n_tiles_y;
-    const int max_tiles = std::min(tiles, 4096);
     hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, c->d_gate);
-    hipLaunchKernelGGL(k_gate, dim3((unsigned)tiles), dim3(256), 0, c->stream, c->d_screen, c->res_x, c->res_y,
-                       c->n_tiles_x, max_cnn_holes, c->d_gate, c->d_gate_tiles, c->d_gate_rays, (uint32_t)max_tiles);
+    hipLaunchKernelGGL(k_gate, dim3((unsigned)tiles), dim3(256), 0, c->stream, v, c->n_tiles_x, max_cnn_holes,
+                       c->d_gate, c->d_gate_tiles, c->d_gate_rays, (uint32_t)tiles);
     HIPCHECK(hipGetLastError());
-    RC(launch_rays(c, p, c->d_gate_rays, c->d_gate + 1, 0, 0, 0, 0, nullptr));
+    // hole list, gate rays and the refresh tile in one ray launch
+    mark(c, ST_RAYS);
+    RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
     const Launch L = single(c);
-    const View v = make_view(c, p);
     mark(c, ST_COPY);
     RC(launch_copy(L, v, 2));
     mark(c, ST_FILTER);
     RC(launch_fill(L, v, nullptr, 1, 1));
-    if (cnn) RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, max_tiles));
+    if (cnn) RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles));
     return frame_end(c);
 }
 

@@ -299,6 +299,209 @@ __global__ __launch_bounds__(256) void k_write_fill(const float* __restrict__ ou
     }
 }
 
+// ---------------------------------------------------------------- fused tile inference
+// One workgroup (4 waves) runs the whole network on one 24x24 crop around a
+// 16x16 gate tile, activations in LDS.  Each layer is evaluated only where
+// the centre 16x16 output needs it: layer 1 on 22x22, layer 2 on 20x20,
+// layer 3 on 18x18, layer 4 on 16x16 (the crop border's zero padding never
+// reaches the centre, so this equals the full-crop network there).  An
+// M-tile is 16 consecutive output pixels in raster order; layer 1 uses
+// v_mfma_f32_16x16x16_bf16 over K = 16 channels per tap, layers 2-4
+// v_mfma_f32_16x16x32_bf16 over K = 32 (A lane: pixel l%16, channels
+// 8*(l/16)..+7; B lane: output channel l%16, the same 8 input channels).
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+// 32-channel pixels: the 16-byte channel group g of pixel p sits in slot
+// g ^ ((p >> 2) & 3), so 16 consecutive pixels cover all LDS banks
+__device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 2) & 3)) << 3); }
+
+struct TileArgs {
+    // frame mode (x == null): features of the crop at tile*16 - 4 from plane 2
+    const uint32_t* words;
+    const float4* coords;
+    int res_x, res_y, tiles_x;
+    const uint32_t* tiles;
+    const uint32_t* count;
+    int max_tiles;
+    float m0x, m0y, m0z, r20, r21, r22;
+    uint32_t* image;
+    // explicit mode: x [n][24][24][16] bf16 -> out [n][16][16][3] float
+    const bf16* x;
+    float* out;
+    int n;
+    const bf16* w[4];  // [9][32][16], [9][32][32], [9][32][32], [9][16][32]
+    const float* b[4];
+};
+
+// layers 2-4: in = swizzled 32-channel pixels of a win x win region, Wl =
+// swizzled [9][COUT][32] weights
+template <int COUT, bool LAST>
+__device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out, float* outf, int wout,
+                                             const bf16* Wl, const float* __restrict__ bias, int wave, int lane)
+{
+    constexpr int NT = COUT / 16;
+    const int npix = wout * wout;
+    const int row = lane & 15, kg = lane >> 4;
+    for (int m0 = wave * 2; m0 * 16 < npix; m0 += 8) {
+        f4v acc[2][NT];
+        int q0[2];
+        for (int m = 0; m < 2; m++) {
+            for (int nn = 0; nn < NT; nn++) acc[m][nn] = f4v{0.f, 0.f, 0.f, 0.f};
+            const int p = min((m0 + m) * 16 + row, npix - 1);  // past the end: recomputed, not stored
+            q0[m] = (p / wout) * win + p % wout;
+        }
+        for (int tap = 0; tap < 9; tap++) {
+            const int dq = (tap / 3) * win + tap % 3;
+            bf8v bfr[NT];
+            for (int nn = 0; nn < NT; nn++) bfr[nn] = *(const bf8v*)(Wl + swz(tap * COUT + nn * 16 + row, kg));
+            for (int m = 0; m < 2; m++) {
+                const bf8v afr = *(const bf8v*)(in + swz(q0[m] + dq, kg));
+                for (int nn = 0; nn < NT; nn++)
+                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nn], acc[m][nn], 0, 0, 0);
+            }
+        }
+        for (int m = 0; m < 2; m++)
+            for (int nn = 0; nn < NT; nn++) {
+                const int co = nn * 16 + row;
+                const float bv = bias[co];
+                for (int i = 0; i < 4; i++) {
+                    const int q = (m0 + m) * 16 + kg * 4 + i;
+                    if (q >= npix) continue;
+                    const float v = acc[m][nn][i] + bv;
+                    if (LAST) {
+                        if (co < 3) outf[q * 3 + co] = v;
+                    } else {
+                        out[swz(q, co >> 3) + (co & 7)] = f2bf(v > 0.f ? v : 0.f);
+                    }
+                }
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cnn_tiles(TileArgs a)
+{
+    __shared__ __align__(16) bf16 P[400 * 32];      // input 576 x 16 ch / layer-2 output 400 x 32 ch
+    __shared__ __align__(16) bf16 Q[484 * 32];      // layer-1 output 484 x 32 / layer-3 output 324 x 32
+    __shared__ __align__(16) bf16 Wl[9 * 32 * 32];  // the current layer's weights
+    __shared__ float OF[256 * 3];                   // layer-4 RGB of the centre 16x16
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int row = lane & 15, kg = lane >> 4;
+    const int n = a.x ? a.n : min((int)*a.count, a.max_tiles);
+    for (int c = blockIdx.x; c < n; c += gridDim.x) {
+        __syncthreads();  // the previous crop is done with every buffer
+        int ox = 0, oy = 0;
+        if (!a.x) {
+            const int t = (int)a.tiles[c];
+            ox = (t % a.tiles_x) * 16 - 4;
+            oy = (t / a.tiles_x) * 16 - 4;
+        }
+        for (int q = tid; q < 576; q += 256) {
+            uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
+            if (a.x) {
+                const uint4* src = (const uint4*)(a.x + ((size_t)c * 576 + q) * 16);
+                lo = src[0];
+                hi = src[1];
+            } else {
+                const int x = ox + q % 24, y = oy + q / 24;
+                float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+                if (x >= 0 && y >= 0 && x < a.res_x && y < a.res_y) {
+                    const size_t o = (size_t)y * a.res_x + x;
+                    const uint32_t w = a.words[o];
+                    if (w == HOLE_WORD) {
+                        f[4] = 1.f;
+                    } else {
+                        rgb_of(w, f[0], f[1], f[2]);
+                        const float4 cc = a.coords[o];
+                        const float qx = cc.x - a.m0x, qy = cc.y - a.m0y, qz = cc.z - a.m0z;
+                        const float z = fmaxf((qx * a.r20 + qy * a.r21) + qz * a.r22, 0.f);
+                        f[3] = z / (z + 64.f);
+                    }
+                }
+                lo.x = (uint32_t)f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16);
+                lo.y = (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16);
+                lo.z = (uint32_t)f2bf(f[4]);
+            }
+            *(uint4*)(P + q * 16) = lo;
+            *(uint4*)(P + q * 16 + 8) = hi;
+        }
+        for (int i = tid; i < 9 * 32 * 16 / 8; i += 256) *(uint4*)(Wl + i * 8) = *(const uint4*)(a.w[0] + i * 8);
+        __syncthreads();
+        // layer 1: 16 -> 32 on 22x22, input region 24x24 (plain 16-channel pixels)
+        for (int m0 = wave * 2; m0 * 16 < 484; m0 += 8) {
+            f4v acc[2][2];
+            int q0[2];
+            for (int m = 0; m < 2; m++) {
+                acc[m][0] = acc[m][1] = f4v{0.f, 0.f, 0.f, 0.f};
+                const int p = min((m0 + m) * 16 + row, 483);
+                q0[m] = (p / 22) * 24 + p % 22;
+            }
+            for (int tap = 0; tap < 9; tap++) {
+                const int dq = (tap / 3) * 24 + tap % 3;
+                s4v bfr[2];
+                for (int nn = 0; nn < 2; nn++) bfr[nn] = *(const s4v*)(Wl + (tap * 32 + nn * 16 + row) * 16 + kg * 4);
+                for (int m = 0; m < 2; m++) {
+                    const s4v afr = *(const s4v*)(P + (q0[m] + dq) * 16 + kg * 4);
+                    for (int nn = 0; nn < 2; nn++)
+                        acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[nn], acc[m][nn], 0, 0, 0);
+                }
+            }
+            for (int m = 0; m < 2; m++)
+                for (int nn = 0; nn < 2; nn++) {
+                    const int co = nn * 16 + row;
+                    const float bv = a.b[0][co];
+                    for (int i = 0; i < 4; i++) {
+                        const int q = (m0 + m) * 16 + kg * 4 + i;
+                        if (q >= 484) continue;
+                        const float v = acc[m][nn][i] + bv;
+                        Q[swz(q, co >> 3) + (co & 7)] = f2bf(v > 0.f ? v : 0.f);
+                    }
+                }
+        }
+        __syncthreads();
+        for (int i = tid; i < 9 * 32 * 4; i += 256) {  // layer-2 weights, swizzled rows
+            const int r = i >> 2, g = i & 3;
+            *(uint4*)(Wl + swz(r, g)) = *(const uint4*)(a.w[1] + r * 32 + g * 8);
+        }
+        __syncthreads();
+        tile_layer32<32, false>(Q, 22, P, nullptr, 20, Wl, a.b[1], wave, lane);
+        __syncthreads();
+        for (int i = tid; i < 9 * 32 * 4; i += 256) {
+            const int r = i >> 2, g = i & 3;
+            *(uint4*)(Wl + swz(r, g)) = *(const uint4*)(a.w[2] + r * 32 + g * 8);
+        }
+        __syncthreads();
+        tile_layer32<32, false>(P, 20, Q, nullptr, 18, Wl, a.b[2], wave, lane);
+        __syncthreads();
+        for (int i = tid; i < 9 * 16 * 4; i += 256) {
+            const int r = i >> 2, g = i & 3;
+            *(uint4*)(Wl + swz(r, g)) = *(const uint4*)(a.w[3] + r * 32 + g * 8);
+        }
+        __syncthreads();
+        tile_layer32<16, true>(Q, 18, nullptr, OF, 16, Wl, a.b[3], wave, lane);
+        __syncthreads();
+        {
+            const float r = OF[tid * 3], g = OF[tid * 3 + 1], b = OF[tid * 3 + 2];
+            if (a.x) {
+                float* o = a.out + ((size_t)c * 256 + tid) * 3;
+                o[0] = r;
+                o[1] = g;
+                o[2] = b;
+            } else {
+                const int x = ox + 4 + (tid & 15), y = oy + 4 + (tid >> 4);
+                if (x < a.res_x && y < a.res_y) {
+                    const size_t o = (size_t)y * a.res_x + x;
+                    if (a.words[o] == HOLE_WORD) {
+                        const uint32_t R = (uint32_t)fminf(fmaxf(r * 255.f + 0.5f, 0.f), 255.f);
+                        const uint32_t G = (uint32_t)fminf(fmaxf(g * 255.f + 0.5f, 0.f), 255.f);
+                        const uint32_t B = (uint32_t)fminf(fmaxf(b * 255.f + 0.5f, 0.f), 255.f);
+                        a.image[o] = B | (G << 8) | (R << 16);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // W'[t][ci][co] = W[8-t][co][ci] (backward-data weights), bf16
 __global__ void k_flip(const float* __restrict__ W, bf16* Wf, int cin, int cout)
 {
@@ -362,6 +565,10 @@ struct svo_cnn {
     float* mask_count = nullptr;
     int2* origins = nullptr;
     int max_origins = 0;
+    bool fused = true;     // per-tile fused inference kernel for the frame fill
+    bf16* xt = nullptr;    // explicit-mode tile inputs / outputs
+    float* ot = nullptr;
+    size_t cap_tiles = 0;
 };
 
 #define HIPCHECK(expr)                                                                    \
@@ -434,6 +641,8 @@ extern "C" int svo_cnn_destroy(svo_cnn* n)
     (void)hipFree(n->loss);
     (void)hipFree(n->mask_count);
     (void)hipFree(n->origins);
+    (void)hipFree(n->xt);
+    (void)hipFree(n->ot);
     if (n->stream) (void)hipStreamDestroy(n->stream);
     delete n;
     return SVO_OK;
@@ -736,9 +945,46 @@ extern "C" int svo_cnn_train_step(svo_cnn* n, svo_ctx* warp, svo_ctx* full, cons
 
 // Inference on the gate's tile list (24x24 crops around each 16x16 tile),
 // queued on the context's stream.
+static TileArgs tile_args(svo_cnn* n)
+{
+    TileArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int i = 0; i < 4; i++) {
+        a.w[i] = n->L[i].wbf;
+        a.b[i] = n->L[i].b;
+    }
+    return a;
+}
+
+static unsigned tile_grid(int max_tiles, int device)
+{
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    return (unsigned)std::max(1, std::min(max_tiles, 2 * cus));  // two resident workgroups per CU
+}
+
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
                    const uint32_t* tile_count, int max_tiles)
 {
+    if (n->fused) {
+        TileArgs a = tile_args(n);
+        a.words = c->d_screen + 2 * c->N;
+        a.coords = c->d_coords + 2 * c->N;
+        a.res_x = c->res_x;
+        a.res_y = c->res_y;
+        a.tiles_x = c->n_tiles_x;
+        a.tiles = tile_list;
+        a.count = tile_count;
+        a.max_tiles = max_tiles;
+        a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
+        a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
+        a.image = c->d_image;
+        hipLaunchKernelGGL(k_cnn_tiles, dim3(tile_grid(max_tiles, c->device)), dim3(256), 0, c->stream, a);
+        HIPCHECK(hipGetLastError());
+        return SVO_OK;
+    }
     int rc = ensure_ws(n, (size_t)max_tiles * 576);
     if (rc) return rc;
     hipStream_t keep = n->stream;
@@ -756,4 +1002,40 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
     }
     n->stream = keep;
     return rc;
+}
+
+// The fused tile network on explicit 24x24 crops: x [n][24][24][16] bf16
+// features -> out [n][16][16][3] float (the centre 16x16, RGB).
+extern "C" int svo_cnn_forward_tiles(svo_cnn* n, const uint16_t* x, int count, float* out)
+{
+    if (!n || !x || !out || count < 1 || count > (1 << 20)) return svo_fail(SVO_ERR_ARG, "bad tile batch");
+    HIPCHECK(hipSetDevice(n->device));
+    if ((size_t)count > n->cap_tiles) {
+        (void)hipFree(n->xt);
+        (void)hipFree(n->ot);
+        n->xt = nullptr;
+        n->ot = nullptr;
+        n->cap_tiles = 0;
+        if (hipMalloc(&n->xt, (size_t)count * 576 * 16 * 2) != hipSuccess ||
+            hipMalloc(&n->ot, (size_t)count * 256 * 3 * 4) != hipSuccess)
+            return svo_fail(SVO_ERR_NOMEM, "tile batch of %d", count);
+        n->cap_tiles = (size_t)count;
+    }
+    HIPCHECK(hipMemcpyAsync(n->xt, x, (size_t)count * 576 * 16 * 2, hipMemcpyHostToDevice, n->stream));
+    TileArgs a = tile_args(n);
+    a.x = n->xt;
+    a.out = n->ot;
+    a.n = count;
+    hipLaunchKernelGGL(k_cnn_tiles, dim3(tile_grid(count, n->device)), dim3(256), 0, n->stream, a);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, n->ot, (size_t)count * 256 * 3 * 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    return SVO_OK;
+}
+
+extern "C" int svo_cnn_set_fused(svo_cnn* n, int enable)
+{
+    if (!n) return svo_fail(SVO_ERR_ARG, "null network");
+    n->fused = enable != 0;
+    return SVO_OK;
 }

@@ -116,6 +116,8 @@ _f = {
                           C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float)),
     "cnn_train_step": _sig("svo_cnn_train_step", C.c_int, _ctxp, _ctxp, _ctxp, C.POINTER(FrameParams), C.c_int,
                            C.c_int, C.c_uint64, C.c_float, C.POINTER(C.c_float)),
+    "cnn_forward_tiles": _sig("svo_cnn_forward_tiles", C.c_int, _ctxp, _u16p, C.c_int, C.POINTER(C.c_float)),
+    "cnn_set_fused": _sig("svo_cnn_set_fused", C.c_int, _ctxp, C.c_int),
     "frame_warp_learned": _sig("svo_frame_warp_learned", C.c_int, _ctxp, _ctxp, C.POINTER(FrameParams), C.c_int),
     "batch_create": _sig("svo_batch_create", C.c_int, C.POINTER(_ctxp), C.POINTER(_ctxp), C.c_int),
     "batch_destroy": _sig("svo_batch_destroy", C.c_int, _ctxp),
@@ -142,7 +144,7 @@ EXPORTED = [
     "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
-    "svo_build_octree_gpu",
+    "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused",
 ]
 
 
@@ -468,6 +470,19 @@ class Cnn:
         out = np.zeros((B, H, W, 16), np.float32)
         _check(_f["cnn_forward"](self._h, x.ctypes.data_as(_u16p), B, H, W, out.ctypes.data_as(_fp)))
         return out
+
+    def forward_tiles(self, x_bf16):
+        """Fused tile network: x [n, 24, 24, 16] bf16 bits -> centre RGB [n, 16, 16, 3]."""
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16)
+        if x.shape[1:] != (24, 24, 16):
+            raise ValueError("x must be [n, 24, 24, 16]")
+        out = np.zeros((x.shape[0], 16, 16, 3), np.float32)
+        _check(_f["cnn_forward_tiles"](self._h, x.ctypes.data_as(_u16p), x.shape[0], out.ctypes.data_as(_fp)))
+        return out
+
+    def set_fused(self, on=True):
+        """Frame fill with the fused tile kernel (default) or the per-layer kernels."""
+        _check(_f["cnn_set_fused"](self._h, 1 if on else 0))
 
     def train_step(self, warp, full, p, n_crops=16, crop=64, seed=0, lr=1e-3, want_loss=False):
         """One Adam step on crops of the frame pair held by warp/full."""

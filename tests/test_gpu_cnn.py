@@ -64,3 +64,25 @@ def test_gradients_match_reference(cnn):
         assert rel(gb, gbr) < 3e-2, (li, rel(gb, gbr))
     # padded channels receive no gradient
     assert np.all(g[:9 * 32 * 16].reshape(9, 32, 16)[:, :, 5:] == 0)
+
+
+def test_fused_tile_forward_matches_reference():
+    """The fused per-tile inference kernel (all four layers in LDS, layer 1 on
+    16x16x16 MFMAs, layers 2-4 on 16x16x32 MFMAs) against the FP64 network on
+    full 24x24 crops, compared on the centre 16x16 it returns."""
+    rng = np.random.default_rng(5)
+    c = S.Cnn(0, seed=11)
+    try:
+        p = c.params()
+        o = 0
+        for ci, co in zip(R.CIN, R.COUT):  # non-zero biases on the real output channels
+            o += 9 * ci * co
+            p[o:o + co] = rng.uniform(-0.2, 0.2, co) * (np.arange(co) < (3 if co == 16 else co))
+            o += co
+        c.set_params(p)
+        bits, x = _batch(rng, 5, 24, 24)
+        out = c.forward_tiles(bits)
+        ref, _ = R.forward(c.params().astype(np.float64), x)
+        assert rel(out, ref[:, 4:20, 4:20, :3]) < 5e-3
+    finally:
+        c.close()

@@ -49,3 +49,43 @@ def test_training_improves_fill():
     assert after_all["learned"] > after_all["gate"]
     assert after_all["gate"] >= after_all["plain"]
     cnn.close()
+
+
+def test_fused_tile_fill_matches_per_layer_fill():
+    """The frame fill with the fused tile kernel and with the per-layer
+    kernels: identical screens, images equal outside CNN pixels and within
+    bf16 rounding (a few 8-bit levels) on them."""
+    cnn = S.Cnn(0)
+    L.load(cnn, L.WEIGHTS)
+    words, root, depth, _ = W.octree(NAME)
+    rx, ry = W.WORKLOADS[NAME]["res"]
+    rows = W.path(NAME, 12)
+    a, b = S.Context(0), S.Context(0)
+    try:
+        for c in (a, b):
+            c.upload_octree(words, root, depth)
+            c.resize(rx, ry)
+        n_cnn = 0
+        for f in range(12):
+            p = W.frame_params(NAME, rows, f)
+            cnn.set_fused(True)
+            a.frame_warp_learned(cnn, p, 64)
+            a.sync()
+            cnn.set_fused(False)
+            b.frame_warp_learned(cnn, p, 64)
+            b.sync()
+            assert_equal_u32(a.screen(), b.screen(), f"frame {f} screen")
+            holes = (a.screen(2) == 0xFFFFFF00).reshape(-1)
+            ia, ib = a.image().reshape(-1), b.image().reshape(-1)
+            assert np.array_equal(ia[~holes], ib[~holes]), f"frame {f}: non-CNN pixels differ"
+            ca = np.stack([(ia[holes] >> s) & 255 for s in (0, 8, 16)], 1).astype(int)
+            cb = np.stack([(ib[holes] >> s) & 255 for s in (0, 8, 16)], 1).astype(int)
+            if holes.any():
+                d = np.abs(ca - cb)
+                assert d.max() <= 4 and d.mean() < 0.5, (f, d.max(), d.mean())
+            n_cnn += int(holes.sum())
+        assert n_cnn > 0
+    finally:
+        a.close()
+        b.close()
+        cnn.close()

@@ -33,6 +33,7 @@ def pct(v, q):
 
 def run_gpu(args, name):
     import svo_amd as S
+    from svo_amd import autotune as A
     from svo_amd import workloads as W
 
     w = W.WORKLOADS[name]
@@ -110,7 +111,7 @@ def run_gpu(args, name):
         c.resize(rx, ry)
         if "tuned_warp" in res:
             t = res["tuned_warp"]
-            c.set_tuning(t["refill"], t["wg_per_cu"], t["iters"], t["lds_cache_words"])
+            A.apply(c, t)
         for f in range(args.warmup):
             c.frame_warp(params[f])
         c.sync()
@@ -169,7 +170,7 @@ def run_gpu(args, name):
             lw.resize(rx, ry)
             if "tuned_warp" in res:
                 t = res["tuned_warp"]
-                lw.set_tuning(t["refill"], t["wg_per_cu"], t["iters"], t["lds_cache_words"])
+                A.apply(lw, t)
             for f in range(args.warmup):
                 lw.frame_warp_learned(cnn, params[f], args.max_cnn_holes)
             lw.sync()
@@ -196,6 +197,7 @@ def run_batch(args):
     context, back to back) and once as batched frames (one launch per stage
     over all views, one shared ray launch)."""
     import svo_amd as S
+    from svo_amd import autotune as A
     from svo_amd import workloads as W
 
     views = W.batch_views()
@@ -253,7 +255,7 @@ def run_batch(args):
         out["tuned"] = dict(best, ms=round(ms, 4))
         log(f"[bench] batch autotune: {best} -> {ms:.4f} ms/batch frame")
         for c in ctxs[1:]:  # the view-by-view run uses the same knobs
-            c.set_tuning(best["refill"], best["wg_per_cu"], best["iters"], best["lds_cache_words"])
+            A.apply(c, best)
     t_seq, rays_seq = seq_run()
     t_bat, rays_bat = batch_run()
     k = n_frames - warm

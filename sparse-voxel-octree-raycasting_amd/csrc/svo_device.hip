@@ -151,6 +151,7 @@ struct RayArgs {
     uint32_t cache_words;           // octree prefix held in LDS
     uint32_t cache_last;            // max(cache_words, 1) - 1
     uint32_t oct_bytes;             // buffer extent of the octree (words + pad)
+    int spread;                     // hand out rays in transposed queue order
 };
 
 struct Ray {
@@ -392,6 +393,14 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     const OctBuf ob = oct_buffer(a.oct, a.oct_bytes);
     const int lane = __lane_id();
     const uint32_t total = a.views ? a.view_first[a.n_views] : view_rays(a.v);
+    // Spread order: queue slot k takes ray (k % 64) * R + k / 64 (R = rows of
+    // 64), so the 64 lanes of a wave trace rays R apart in the ray list and
+    // the long rays of one screen region (near the horizon) land in different
+    // waves; a wave then never runs many of them through both branch bodies.
+    const uint32_t rows = (total + 63u) >> 6;
+    // mode 2: spread only when every ray gets a lane in the first hand-out
+    const bool spread = a.spread == 2 ? total <= gridDim.x * (uint32_t)RAY_BLOCK : a.spread != 0;
+    const uint32_t qtotal = spread ? rows * 64u : total;
     if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
 
     Ray r;
@@ -405,11 +414,12 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(&a.queue[a.slot], (uint32_t)n_idle);
             base = __shfl(base, leader);
-            if (base + (uint32_t)n_idle >= total) exhausted = true;
+            if (base + (uint32_t)n_idle >= qtotal) exhausted = true;
             if (!active) {
                 const int rank = __popcll(idle & ((1ull << lane) - 1ull));
                 const uint32_t k = base + (uint32_t)rank;
-                if (k < total) active = init_ray(r, k, a);
+                const uint32_t kr = spread ? (k & 63u) * rows + (k >> 6) : k;
+                if (k < qtotal && kr < total) active = init_ray(r, kr, a);
             }
             traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)(64 - n_idle);
         }
@@ -1009,6 +1019,7 @@ static int launch_rays_v(svo_ctx* c, hipStream_t stream, const View& v, const Vi
     seq++;
     a.refill = c->refill;
     a.iters = c->iters;
+    a.spread = c->spread;
     a.ray_count = (unsigned long long*)(c->d_queue + 8);
     a.cache_words = std::min(c->cache_req, c->normal_words);
     a.cache_last = a.cache_words ? a.cache_words - 1 : 0;
@@ -1508,6 +1519,17 @@ extern "C" int svo_frame_time(svo_ctx* c, float* ms)
 {
     if (!c || !ms) return svo_fail(SVO_ERR_ARG, "null argument");
     *ms = c->frame_ms;
+    return SVO_OK;
+}
+
+extern "C" int svo_set_ray_spread(svo_ctx* c, int enable)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    if (enable < 0 || enable > 2) return svo_fail(SVO_ERR_ARG, "spread mode %d outside 0..2", enable);
+    c->spread = enable;
     return SVO_OK;
 }
 

@@ -91,6 +91,7 @@ _f = {
     "frame_time": _sig("svo_frame_time", C.c_int, _ctxp, C.POINTER(C.c_float)),
     "set_graph": _sig("svo_set_graph", C.c_int, _ctxp, C.c_int),
     "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int, C.c_int),
+    "set_ray_spread": _sig("svo_set_ray_spread", C.c_int, _ctxp, C.c_int),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
     "tile_schedule": _sig("svo_tile_schedule", C.c_int, C.c_int, C.c_int, C.c_int,
@@ -144,7 +145,7 @@ EXPORTED = [
     "svo_cnn_forward", "svo_cnn_gradients", "svo_cnn_train_step", "svo_frame_warp_learned",
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
-    "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused",
+    "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
 ]
 
 
@@ -377,6 +378,11 @@ class Context:
     def set_tuning(self, refill=-1, wg_per_cu=-1, iters=-1, lds_cache_words=-1):
         """Ray-kernel knobs; -1 keeps the current value."""
         _check(_f["set_tuning"](self._h, int(refill), int(wg_per_cu), int(iters), int(lds_cache_words)))
+
+    def set_ray_spread(self, mode=2):
+        """Ray hand-out order: 0 list order, 1 spread across waves, 2 (default)
+        spread when all rays fit in the resident lanes."""
+        _check(_f["set_ray_spread"](self._h, int(mode)))
 
     def ray_count(self, reset=False):
         """Rays traced by the ray kernels since the last reset."""

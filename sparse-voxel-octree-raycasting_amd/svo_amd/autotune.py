@@ -1,9 +1,10 @@
 """Autotuner for the ray kernel's launch knobs.
 
-Knobs (svo_set_tuning): work-queue refill threshold (idle lanes per refill),
-persistent workgroups per CU, traversal steps between refill checks, and the
-LDS node-cache depth (octree words of the breadth-first top levels kept in
-LDS).  Every candidate replays the same frame window from the same starting
+Knobs: ray hand-out order (svo_set_ray_spread: spread across waves or list
+order) and, through svo_set_tuning, the work-queue refill threshold (idle
+lanes per refill), persistent workgroups per CU, traversal steps between
+refill checks, and the LDS node-cache depth (octree words of the
+breadth-first top levels kept in LDS).  Every candidate replays the same frame window from the same starting
 state (the window starts with the two clearing frames), so candidates see
 identical work; the score is the median GPU time per frame over the timed
 frames, measured with the library's per-frame HIP-event pair.  Results do not
@@ -16,6 +17,7 @@ import statistics
 import time
 
 DEFAULT_SPACE = {
+    "spread": [1, 2, 0],
     "refill": [8, 16, 32, 48],
     "wg_per_cu": [4, 8, 16],
     "iters": [8, 16, 64],
@@ -62,8 +64,11 @@ def _search(space, coordinate, measure):
     return best, best_ms
 
 
-def _apply(ctx, cfg):
+def apply(ctx, cfg):
+    """Apply a knob setting (as returned by tune / tune_batch) to a context."""
     ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
+    if "spread" in cfg:
+        ctx.set_ray_spread(int(cfg["spread"]))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):
@@ -77,7 +82,7 @@ def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=No
     trials = []
 
     def measure(cfg):
-        _apply(ctx, cfg)
+        apply(ctx, cfg)
         ms = _frame_time(ctx, params, warmup, mode)
         trials.append((dict(cfg), ms))
         if log:
@@ -85,7 +90,7 @@ def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=No
         return ms
 
     best, best_ms = _search(space, coordinate, measure)
-    _apply(ctx, best)
+    apply(ctx, best)
     return best, best_ms, trials
 
 
@@ -100,7 +105,7 @@ def tune_batch(batch, frames, warmup=4, space=None, coordinate=True, log=None):
     trials = []
 
     def measure(cfg):
-        _apply(ctx0, cfg)
+        apply(ctx0, cfg)
         for ps in frames[:warmup]:
             batch.frame_warp(ps)
         batch.sync()
@@ -117,5 +122,5 @@ def tune_batch(batch, frames, warmup=4, space=None, coordinate=True, log=None):
         return ms
 
     best, best_ms = _search(space, coordinate, measure)
-    _apply(ctx0, best)
+    apply(ctx0, best)
     return best, best_ms, trials

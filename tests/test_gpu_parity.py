@@ -109,3 +109,36 @@ def test_warp_graph_vs_direct_launches(scene1):
     finally:
         g.close()
         d.close()
+
+
+@pytest.mark.parametrize("spread,refill,wg,iters,cache", [(0, 16, 4, 64, 0), (1, 48, 8, 8, 736),
+                                                          (2, 1, 1, 1, 4096), (1, 64, 32, 4096, 96)])
+def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, cache):
+    """Ray hand-out order, refill threshold, workgroups per CU, trips per
+    refill check and the LDS node cache change only the schedule."""
+    words, root = scene1
+    rx, ry = CONFIG1["res"]
+    pr = path(CONFIG1, 6)
+    c = S.Context(0)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+    try:
+        c.upload_octree(words, root, 8)
+        c.resize(rx, ry)
+        c.set_tuning(refill, wg, iters, cache)
+        c.set_ray_spread(spread)
+        for f in range(6):
+            p = params(CONFIG1, pr, f)
+            print(f"knobs {spread, refill, wg, iters, cache} frame {f}", flush=True)
+            c.frame_warp(p)
+            o.frame_warp(O.params_from(p))
+            c.sync()
+            assert_equal_u32(c.screen(), o.screen, f"frame {f} screen")
+            assert_equal_f32_bits(c.coords(), o.coords, f"frame {f} coords")
+        p = params(CONFIG1, pr, 5)
+        c.frame_full(p)
+        o.frame_full(O.params_from(p))
+        c.sync()
+        assert_equal_u32(c.screen(0), o.screen[0], "full frame")
+    finally:
+        c.close()
+        o.close()

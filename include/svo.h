@@ -227,12 +227,13 @@ const char* svo_stage_name(int stage);
  *                     per workgroup (0..12288; 0 disables the cache) */
 int svo_set_tuning(svo_ctx* ctx, int refill_threshold, int wg_per_cu,
                    int iters_per_refill, int lds_cache_words);
-/* Ray hand-out order of the ray kernel.  1 = spread: the 64 lanes of a wave
- * take rays 1/64 of the ray list apart, so the long rays of one screen region
- * do not share a wave; 0 = list order (coherent); 2 (default) = spread when
- * the launch's rays all fit in its resident lanes (warp frames), list order
- * otherwise (full frames).  Results do not depend on it. */
-int svo_set_ray_spread(svo_ctx* ctx, int mode);
+/* Ray hand-out order of the ray kernel.  The 64 lanes of a wave take 64/group
+ * groups of `group` consecutive rays that lie 1/64 of the ray list apart, so
+ * the long rays of one screen region do not share a wave; group 64 is list
+ * order.  only_if_fits != 0 spreads only launches whose rays all get a lane
+ * at once (warp frames) and keeps list order for larger ones (full frames).
+ * Default: group 1, only_if_fits 1.  Results do not depend on it. */
+int svo_set_ray_spread(svo_ctx* ctx, int group, int only_if_fits);
 
 /* ---- learned hole filler (no counterpart in the reference) ------------
  * A 4-layer 3x3 CNN (5 -> 32 -> 32 -> 32 -> 3 channels, ReLU) run with BF16

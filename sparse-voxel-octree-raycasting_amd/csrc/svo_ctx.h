@@ -41,7 +41,8 @@ struct svo_ctx {
     // tuning
     int refill = 16, wg_per_cu = 4, iters = 64;
     uint32_t cache_req = 0;  // requested LDS node-cache words
-    int spread = 2;          // ray hand-out order: 0 list, 1 spread, 2 spread when all rays fit
+    int spread_shift = 0;    // ray hand-out: log2 of the spread group (6 = list order)
+    int spread_fit = 1;      // spread only launches whose rays all fit in the resident lanes
     // per-frame graph of the warp frame (frames >= 2): its kernels read the
     // frame's view from d_view, refreshed from a pinned 2-slot staging ring
     bool use_graph = true;

@@ -151,7 +151,8 @@ struct RayArgs {
     uint32_t cache_words;           // octree prefix held in LDS
     uint32_t cache_last;            // max(cache_words, 1) - 1
     uint32_t oct_bytes;             // buffer extent of the octree (words + pad)
-    int spread;                     // hand out rays in transposed queue order
+    int spread_shift;               // log2 of the spread group (6 = list order)
+    int spread_fit;                 // spread only when all rays get a lane at once
 };
 
 struct Ray {
@@ -393,14 +394,16 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     const OctBuf ob = oct_buffer(a.oct, a.oct_bytes);
     const int lane = __lane_id();
     const uint32_t total = a.views ? a.view_first[a.n_views] : view_rays(a.v);
-    // Spread order: queue slot k takes ray (k % 64) * R + k / 64 (R = rows of
-    // 64), so the 64 lanes of a wave trace rays R apart in the ray list and
-    // the long rays of one screen region (near the horizon) land in different
-    // waves; a wave then never runs many of them through both branch bodies.
+    // Spread order with groups of G = 2^spread_shift rays: queue slot k (wave
+    // w = k / 64, lane l = k % 64) takes ray ((l / G) * R + w) * G + l % G,
+    // R = ceil(total / 64).  The 64 / G groups of a wave lie R groups apart in
+    // the ray list, so the long rays of one screen region (near the horizon)
+    // land in different waves; a wave then rarely runs many of them through
+    // both branch bodies.  G = 64 is list order.
     const uint32_t rows = (total + 63u) >> 6;
-    // mode 2: spread only when every ray gets a lane in the first hand-out
-    const bool spread = a.spread == 2 ? total <= gridDim.x * (uint32_t)RAY_BLOCK : a.spread != 0;
+    const bool spread = a.spread_shift < 6 && (!a.spread_fit || total <= gridDim.x * (uint32_t)RAY_BLOCK);
     const uint32_t qtotal = spread ? rows * 64u : total;
+    const uint32_t gmask = (1u << a.spread_shift) - 1u;
     if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
 
     Ray r;
@@ -418,7 +421,9 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
             if (!active) {
                 const int rank = __popcll(idle & ((1ull << lane) - 1ull));
                 const uint32_t k = base + (uint32_t)rank;
-                const uint32_t kr = spread ? (k & 63u) * rows + (k >> 6) : k;
+                const uint32_t l = k & 63u;
+                const uint32_t kr =
+                    spread ? ((((l >> a.spread_shift) * rows + (k >> 6)) << a.spread_shift) | (l & gmask)) : k;
                 if (k < qtotal && kr < total) active = init_ray(r, kr, a);
             }
             traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)(64 - n_idle);
@@ -1019,7 +1024,8 @@ static int launch_rays_v(svo_ctx* c, hipStream_t stream, const View& v, const Vi
     seq++;
     a.refill = c->refill;
     a.iters = c->iters;
-    a.spread = c->spread;
+    a.spread_shift = c->spread_shift;
+    a.spread_fit = c->spread_fit;
     a.ray_count = (unsigned long long*)(c->d_queue + 8);
     a.cache_words = std::min(c->cache_req, c->normal_words);
     a.cache_last = a.cache_words ? a.cache_words - 1 : 0;
@@ -1522,14 +1528,18 @@ extern "C" int svo_frame_time(svo_ctx* c, float* ms)
     return SVO_OK;
 }
 
-extern "C" int svo_set_ray_spread(svo_ctx* c, int enable)
+extern "C" int svo_set_ray_spread(svo_ctx* c, int group, int only_if_fits)
 {
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (group < 1 || group > 64 || (group & (group - 1)))
+        return svo_fail(SVO_ERR_ARG, "spread group %d is not a power of two in 1..64", group);
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
     drop_graph(c);
-    if (enable < 0 || enable > 2) return svo_fail(SVO_ERR_ARG, "spread mode %d outside 0..2", enable);
-    c->spread = enable;
+    int shift = 0;
+    while ((1 << shift) < group) shift++;
+    c->spread_shift = shift;
+    c->spread_fit = only_if_fits != 0;
     return SVO_OK;
 }
 

@@ -91,7 +91,7 @@ _f = {
     "frame_time": _sig("svo_frame_time", C.c_int, _ctxp, C.POINTER(C.c_float)),
     "set_graph": _sig("svo_set_graph", C.c_int, _ctxp, C.c_int),
     "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int, C.c_int),
-    "set_ray_spread": _sig("svo_set_ray_spread", C.c_int, _ctxp, C.c_int),
+    "set_ray_spread": _sig("svo_set_ray_spread", C.c_int, _ctxp, C.c_int, C.c_int),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
     "tile_schedule": _sig("svo_tile_schedule", C.c_int, C.c_int, C.c_int, C.c_int,
@@ -379,10 +379,11 @@ class Context:
         """Ray-kernel knobs; -1 keeps the current value."""
         _check(_f["set_tuning"](self._h, int(refill), int(wg_per_cu), int(iters), int(lds_cache_words)))
 
-    def set_ray_spread(self, mode=2):
-        """Ray hand-out order: 0 list order, 1 spread across waves, 2 (default)
-        spread when all rays fit in the resident lanes."""
-        _check(_f["set_ray_spread"](self._h, int(mode)))
+    def set_ray_spread(self, group=1, only_if_fits=True):
+        """Ray hand-out order: groups of `group` consecutive rays spread across
+        waves (64 = list order); only_if_fits limits it to launches whose rays
+        all fit in the resident lanes."""
+        _check(_f["set_ray_spread"](self._h, int(group), 1 if only_if_fits else 0))
 
     def ray_count(self, reset=False):
         """Rays traced by the ray kernels since the last reset."""

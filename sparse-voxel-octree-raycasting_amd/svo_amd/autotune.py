@@ -1,7 +1,7 @@
 """Autotuner for the ray kernel's launch knobs.
 
-Knobs: ray hand-out order (svo_set_ray_spread: spread across waves or list
-order) and, through svo_set_tuning, the work-queue refill threshold (idle
+Knobs: ray hand-out order (svo_set_ray_spread: the size of the ray groups
+spread across waves, 64 = list order) and, through svo_set_tuning, the work-queue refill threshold (idle
 lanes per refill), persistent workgroups per CU, traversal steps between
 refill checks, and the LDS node-cache depth (octree words of the
 breadth-first top levels kept in LDS).  Every candidate replays the same frame window from the same starting
@@ -17,7 +17,7 @@ import statistics
 import time
 
 DEFAULT_SPACE = {
-    "spread": [1, 2, 0],
+    "spread": [1, 4, 16, 64],
     "refill": [8, 16, 32, 48],
     "wg_per_cu": [4, 8, 16],
     "iters": [8, 16, 64],
@@ -68,7 +68,7 @@ def apply(ctx, cfg):
     """Apply a knob setting (as returned by tune / tune_batch) to a context."""
     ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
     if "spread" in cfg:
-        ctx.set_ray_spread(int(cfg["spread"]))
+        ctx.set_ray_spread(int(cfg["spread"]), True)
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):

@@ -111,8 +111,8 @@ def test_warp_graph_vs_direct_launches(scene1):
         d.close()
 
 
-@pytest.mark.parametrize("spread,refill,wg,iters,cache", [(0, 16, 4, 64, 0), (1, 48, 8, 8, 736),
-                                                          (2, 1, 1, 1, 4096), (1, 64, 32, 4096, 96)])
+@pytest.mark.parametrize("spread,refill,wg,iters,cache", [(64, 16, 4, 64, 0), (1, 48, 8, 8, 736),
+                                                          (4, 1, 1, 1, 4096), (16, 64, 32, 4096, 96)])
 def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, cache):
     """Ray hand-out order, refill threshold, workgroups per CU, trips per
     refill check and the LDS node cache change only the schedule."""
@@ -125,7 +125,7 @@ def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, c
         c.upload_octree(words, root, 8)
         c.resize(rx, ry)
         c.set_tuning(refill, wg, iters, cache)
-        c.set_ray_spread(spread)
+        c.set_ray_spread(spread, only_if_fits=(spread != 16))
         for f in range(6):
             p = params(CONFIG1, pr, f)
             print(f"knobs {spread, refill, wg, iters, cache} frame {f}", flush=True)

@@ -394,15 +394,19 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     const OctBuf ob = oct_buffer(a.oct, a.oct_bytes);
     const int lane = __lane_id();
     const uint32_t total = a.views ? a.view_first[a.n_views] : view_rays(a.v);
-    // Spread order with groups of G = 2^spread_shift rays: queue slot k (wave
-    // w = k / 64, lane l = k % 64) takes ray ((l / G) * R + w) * G + l % G,
-    // R = ceil(total / 64).  The 64 / G groups of a wave lie R groups apart in
-    // the ray list, so the long rays of one screen region (near the horizon)
-    // land in different waves; a wave then rarely runs many of them through
-    // both branch bodies.  G = 64 is list order.
-    const uint32_t rows = (total + 63u) >> 6;
+    // Spread order for the pixel-list rays (the first P rays; in a batch all
+    // rays) in groups of G = 2^spread_shift: queue slot k < 64 R (wave w =
+    // k / 64, lane l = k % 64) takes ray ((l / G) * R + w) * G + l % G,
+    // R = ceil(P / 64); the refresh-rectangle rays that follow keep their
+    // 8x8-block order (neighbouring rays there take the same path, so a wave
+    // of them stays coherent).  The 64 / G groups of a wave lie R groups apart
+    // in the list, so the long hole rays of one screen region (near the
+    // horizon) land in different waves; a wave then rarely runs many of them
+    // through both branch bodies.  G = 64 is list order.
+    const uint32_t P = (a.views && a.n_views > 1) ? total : total - (a.views ? a.views[0].n_rect : a.v.n_rect);
+    const uint32_t rows = (P + 63u) >> 6;
     const bool spread = a.spread_shift < 6 && (!a.spread_fit || total <= gridDim.x * (uint32_t)RAY_BLOCK);
-    const uint32_t qtotal = spread ? rows * 64u : total;
+    const uint32_t qtotal = spread ? rows * 64u + (total - P) : total;
     const uint32_t gmask = (1u << a.spread_shift) - 1u;
     if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
 
@@ -422,9 +426,17 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
                 const int rank = __popcll(idle & ((1ull << lane) - 1ull));
                 const uint32_t k = base + (uint32_t)rank;
                 const uint32_t l = k & 63u;
-                const uint32_t kr =
-                    spread ? ((((l >> a.spread_shift) * rows + (k >> 6)) << a.spread_shift) | (l & gmask)) : k;
-                if (k < qtotal && kr < total) active = init_ray(r, kr, a);
+                uint32_t kr = k;
+                bool valid = k < qtotal;
+                if (spread) {
+                    if (k < rows * 64u) {
+                        kr = (((l >> a.spread_shift) * rows + (k >> 6)) << a.spread_shift) | (l & gmask);
+                        valid = kr < P;
+                    } else {
+                        kr = P + (k - rows * 64u);
+                    }
+                }
+                if (valid) active = init_ray(r, kr, a);
             }
             traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)(64 - n_idle);
         }

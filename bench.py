@@ -20,6 +20,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sparse-voxel-octree-raycasting_amd"))
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+# fused CNN tile network, per 16x16 tile: useful FLOPs = 2 * (484*9*5*32 +
+# 400*9*32*32 + 324*9*32*32 + 256*9*32*3) (real channels on the 22/20/18/16
+# pixel-square regions the centre needs); issued = the MFMAs the kernel runs
+# (32/26/22/16 M-tiles, channels padded to 16/32)
+CNN_TILE_FLOPS = 2 * (484 * 9 * 5 * 32 + 400 * 9 * 32 * 32 + 324 * 9 * 32 * 32 + 256 * 9 * 32 * 3)
+CNN_TILE_MFMA_FLOPS = 32 * 2 * 9 * 8192 + 26 * 2 * 9 * 16384 + 22 * 2 * 9 * 16384 + 16 * 9 * 16384
 METRIC = "frames/s & p99 frame ms at 1920x1024 (warp+fill vs full raycast); Mrays/s"
 
 
@@ -180,6 +187,26 @@ def run_gpu(args, name):
             lw.sync()
             res["learned_fps"] = args.steps / (time.perf_counter() - t0)
             lw.close()
+            # per-stage times of the same frames on fresh state: the CNN stage
+            # is the fused tile kernel; its work is the gate's CNN tile count
+            lw = fresh()
+            lw.enable_timing(mode=1)
+            cnn_ms, tiles, gate_rays, lstage = [], [], [], {}
+            for f in range(args.warmup, min(total, args.warmup + 40)):
+                lw.frame_warp_learned(cnn, params[f], args.max_cnn_holes)
+                lw.sync()
+                st = lw.stage_times()
+                for k, v in st.items():
+                    lstage.setdefault(k, []).append(v)
+                cnn_ms.append(st["cnn"])
+                t_, g_ = lw.gate_counts()
+                tiles.append(t_)
+                gate_rays.append(g_)
+            lw.close()
+            res["learned_stage_ms"] = {k: round(float(np.mean(v)), 4) for k, v in lstage.items() if np.mean(v) > 0}
+            res["cnn_ms"] = float(np.mean(cnn_ms))
+            res["cnn_tiles"] = float(np.mean(tiles))
+            res["gate_rays"] = float(np.mean(gate_rays))
             allp, holes = L.evaluate(cnn, name, seed_off=1000, n_frames=40, max_cnn_holes=args.max_cnn_holes)
             res["psnr_modes"] = {k: round(v, 2) for k, v in allp.items()}
             res["psnr_hole_pixels"] = {k: round(v, 2) for k, v in holes.items()}
@@ -385,7 +412,18 @@ def main():
         "learned_fill": None if "learned_fps" not in g else {
             "fps": round(g["learned_fps"], 2), "speedup_vs_full": round(g["learned_fps"] / g["full_fps"], 3),
             "psnr_vs_full_db": g["psnr_modes"], "psnr_hole_pixels_db": g["psnr_hole_pixels"],
-            "max_cnn_holes_per_tile": args.max_cnn_holes},
+            "max_cnn_holes_per_tile": args.max_cnn_holes,
+            "stage_ms": g["learned_stage_ms"], "cnn_tiles_per_frame": round(g["cnn_tiles"]),
+            "gate_rays_per_frame": round(g["gate_rays"]),
+            # fused tile kernel, bf16 MFMA: useful FLOPs (real channels, the
+            # regions the centre 16x16 needs) and FLOPs the MFMAs issue
+            "roofline": {"kernel": "k_cnn_tiles", "bound": "mfma", "unit": "TFLOP/s",
+                         "achieved": round(g["cnn_tiles"] * CNN_TILE_FLOPS / (g["cnn_ms"] * 1e-3) / 1e12, 2),
+                         "issued": round(g["cnn_tiles"] * CNN_TILE_MFMA_FLOPS / (g["cnn_ms"] * 1e-3) / 1e12, 2),
+                         "peak": BF16_PEAK_TFLOPS,
+                         "frac": round(g["cnn_tiles"] * CNN_TILE_FLOPS / (g["cnn_ms"] * 1e-3) / 1e12
+                                       / BF16_PEAK_TFLOPS, 5),
+                         "avg_launch_ms": round(g["cnn_ms"], 4), "flops_per_tile": CNN_TILE_FLOPS}},
         "octree_build": g["octree_build"],
         "batched_views": batch,
         "roofline": roof,

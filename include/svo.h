@@ -279,6 +279,9 @@ int svo_cnn_set_fused(svo_cnn* cnn, int enable);
  * screen words keep fillhole2's result).  cnn may be NULL: the gate alone,
  * fillhole2 colours everywhere. */
 int svo_frame_warp_learned(svo_ctx* ctx, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
+/* Gate outcome of the last svo_frame_warp_learned (blocking): tiles sent to
+ * the CNN and residual-hole pixels sent back to raycasting. */
+int svo_gate_counts(svo_ctx* ctx, uint32_t* cnn_tiles, uint32_t* gate_rays);
 
 /* ---- batched views (no counterpart in the reference) -------------------
  * A batch runs the warp frame of many contexts that hold the same octree on

@@ -7,9 +7,10 @@
 
 #include "svo_internal.h"
 
-enum Stage { ST_CLEAR, ST_SCATTER, ST_RESOLVE, ST_COUNT, ST_SCAN, ST_WRITE, ST_RAYS, ST_COPY, ST_FILTER, ST_N };
-static const char* const kStageNames[ST_N] = {"clear", "proj_scatter", "proj_resolve", "counthole", "sumids",
-                                        "writeids",  "rays",        "copy",         "fill_colorize"};
+enum Stage { ST_CLEAR, ST_SCATTER, ST_RESOLVE, ST_COUNT, ST_SCAN, ST_WRITE, ST_GATE, ST_RAYS, ST_COPY, ST_FILTER, ST_CNN, ST_N };
+static const char* const kStageNames[ST_N] = {"clear",    "proj_scatter", "proj_resolve", "counthole",
+                                              "sumids",   "writeids",     "gate",         "rays",
+                                              "copy",     "fill_colorize", "cnn"};
 
 struct svo_ctx {
     int device = 0;

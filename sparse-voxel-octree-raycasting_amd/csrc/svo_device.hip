@@ -1197,6 +1197,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     v.list2 = c->d_gate_rays;
     v.list2_count = c->d_gate + 1;
     const int tiles = c->n_tiles_x * c->n_tiles_y;
+    mark(c, ST_GATE);
     hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, c->d_gate);
     hipLaunchKernelGGL(k_gate, dim3((unsigned)tiles), dim3(256), 0, c->stream, v, c->n_tiles_x, max_cnn_holes,
                        c->d_gate, c->d_gate_tiles, c->d_gate_rays, (uint32_t)tiles);
@@ -1209,7 +1210,10 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     RC(launch_copy(L, v, 2));
     mark(c, ST_FILTER);
     RC(launch_fill(L, v, nullptr, 1, 1));
-    if (cnn) RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles));
+    if (cnn) {
+        mark(c, ST_CNN);
+        RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles));
+    }
     return frame_end(c);
 }
 
@@ -1506,6 +1510,19 @@ extern "C" int svo_sync(svo_ctx* c)
         if (first >= 0 || c->timing == 2)
             (void)hipEventElapsedTime(&c->frame_ms, c->ev[first >= 0 ? first : 0], c->ev[ST_N]);
     }
+    return SVO_OK;
+}
+
+extern "C" int svo_gate_counts(svo_ctx* c, uint32_t* cnn_tiles, uint32_t* gate_rays)
+{
+    RC(need_frame(c));
+    if (!cnn_tiles || !gate_rays) return svo_fail(SVO_ERR_ARG, "null argument");
+    uint32_t g[2];
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(g, c->d_gate, 8, hipMemcpyDeviceToHost));
+    *cnn_tiles = g[0];
+    *gate_rays = g[1];
     return SVO_OK;
 }
 

@@ -119,6 +119,7 @@ _f = {
                            C.c_int, C.c_uint64, C.c_float, C.POINTER(C.c_float)),
     "cnn_forward_tiles": _sig("svo_cnn_forward_tiles", C.c_int, _ctxp, _u16p, C.c_int, C.POINTER(C.c_float)),
     "cnn_set_fused": _sig("svo_cnn_set_fused", C.c_int, _ctxp, C.c_int),
+    "gate_counts": _sig("svo_gate_counts", C.c_int, _ctxp, _u32p, _u32p),
     "frame_warp_learned": _sig("svo_frame_warp_learned", C.c_int, _ctxp, _ctxp, C.POINTER(FrameParams), C.c_int),
     "batch_create": _sig("svo_batch_create", C.c_int, C.POINTER(_ctxp), C.POINTER(_ctxp), C.c_int),
     "batch_destroy": _sig("svo_batch_destroy", C.c_int, _ctxp),
@@ -146,6 +147,7 @@ EXPORTED = [
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
+    "svo_gate_counts",
 ]
 
 
@@ -335,6 +337,12 @@ class Context:
         """Warp frame with the CNN fill and per-tile gate."""
         _check(_f["frame_warp_learned"](self._h, cnn._h if cnn is not None else None, C.byref(p),
                                         int(max_cnn_holes)))
+
+    def gate_counts(self):
+        """(CNN tiles, gate rays) of the last learned frame."""
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(_f["gate_counts"](self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # per-stage (reference kernel names)
     def memset(self, byte_ofs, val, nbytes):

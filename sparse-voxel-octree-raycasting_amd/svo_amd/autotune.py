@@ -17,7 +17,7 @@ import statistics
 import time
 
 DEFAULT_SPACE = {
-    "spread": [1, 4, 16, 64],
+    "spread": [1, 64],
     "refill": [8, 16, 32, 48],
     "wg_per_cu": [4, 8, 16],
     "iters": [8, 16, 64],

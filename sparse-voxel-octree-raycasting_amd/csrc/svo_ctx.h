@@ -32,7 +32,7 @@ struct svo_ctx {
     unsigned long long* d_keys = nullptr;
     uint32_t* d_snap = nullptr;
     uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [2..3] the graph's, [8..9] ray count (u64)
-    // learned-fill gate: [0] CNN tile count, [1] gate ray count
+    // learned-fill gate: [0] CNN tile count, [16] gate ray count
     uint32_t* d_gate = nullptr;
     uint32_t* d_gate_tiles = nullptr;  // tile indices sent to the CNN
     uint32_t* d_gate_rays = nullptr;   // x | y << 16 of holes sent back to raycasting

@@ -675,39 +675,79 @@ __global__ __launch_bounds__(256) void k_fill_color(View v, const View* views, c
 }
 
 // ---------------------------------------------------------------- learned-fill gate
-__global__ void k_gate_reset(uint32_t* gate) { gate[0] = 0u; gate[1] = 0u; }
+// gate counters: [0] CNN tiles, [GATE_RAYS] gate rays (a separate 64-byte line)
+constexpr int GATE_RAYS = 16;
+__global__ void k_gate_reset(uint32_t* gate) { gate[0] = 0u; gate[GATE_RAYS] = 0u; }
 
-// One 16x16 tile per block, after compaction and before the rays: count the
-// tile's residual holes -- hole pixels that neither the hole list (all-hole
-// 2x2 quads of the full 16x16 blocks) nor the refresh rectangle will trace.
-// 0 < residual <= max_cnn -> the tile goes to the CNN; more -> its residual
-// pixels join the frame's ray launch as a second list.
-__global__ __launch_bounds__(256) void k_gate(View v, int tiles_x, int max_cnn, uint32_t* gate, uint32_t* tiles,
-                                              uint32_t* rays, uint32_t max_tiles)
+// After compaction and before the rays, each 16x16 tile counts its residual
+// holes: hole pixels that neither the hole list (all-hole 2x2 quads of the
+// full 16x16 blocks) nor the refresh rectangle will trace.  0 < residual <=
+// max_cnn -> the tile goes to the CNN; more -> its residual pixels join the
+// frame's ray launch as a second list.  A workgroup handles GATE_TILES tiles
+// (each wave two, lane = 4 pixels) and appends its CNN tiles and gate pixels
+// with one atomic per list, so the counters see few, uncontended atomics.
+constexpr int GATE_TILES = 8;
+__global__ __launch_bounds__(256) void k_gate(View v, int tiles_x, int n_tiles, int max_cnn, uint32_t* gate,
+                                              uint32_t* tiles, uint32_t* rays, uint32_t max_tiles)
 {
-    const int t = blockIdx.x;
-    const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    __shared__ uint32_t s_n[GATE_TILES], s_base[2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int res_x = v.res_x, res_y = v.res_y;
-    bool hole = x < res_x && y < res_y && v.screen[(size_t)y * res_x + x] == HOLE;
-    if (hole && x < (res_x / 16) * 16 && y < (res_y / 16) * 16 && quad_is_hole(v.screen, res_x, x & ~1, y & ~1))
-        hole = false;  // in the hole list
-    if (hole && x >= v.add_x && x < v.add_x + v.ext_x && y >= v.add_y && y < v.add_y + v.ext_y)
-        hole = false;  // in the refresh rectangle
-    const int n = __syncthreads_count(hole);
-    if (n == 0) return;
-    if (n <= max_cnn) {
-        if (threadIdx.x == 0) {
-            const uint32_t k = atomicAdd(&gate[0], 1u);
-            if (k < max_tiles) tiles[k] = (uint32_t)t;
+    const int fx = (res_x / 16) * 16, fy = (res_y / 16) * 16;
+    unsigned long long m[2][4];
+    for (int h = 0; h < 2; h++) {
+        const int lt = wave * 2 + h, t = blockIdx.x * GATE_TILES + lt;
+        uint32_t n = 0;
+        for (int j = 0; j < 4; j++) {
+            const int i = lane + 64 * j;
+            const int x = (t % tiles_x) * 16 + (i & 15), y = (t / tiles_x) * 16 + (i >> 4);
+            bool hole = t < n_tiles && x < res_x && y < res_y && v.screen[(size_t)y * res_x + x] == HOLE;
+            if (hole && x < fx && y < fy && quad_is_hole(v.screen, res_x, x & ~1, y & ~1))
+                hole = false;  // in the hole list
+            if (hole && x >= v.add_x && x < v.add_x + v.ext_x && y >= v.add_y && y < v.add_y + v.ext_y)
+                hole = false;  // in the refresh rectangle
+            m[h][j] = __ballot(hole);
+            n += (uint32_t)__popcll(m[h][j]);
         }
-        return;
+        if (lane == 0) s_n[lt] = n;
     }
-    const unsigned long long m = __ballot(hole);
-    const int lane = __lane_id();
-    uint32_t base = 0;
-    if (lane == 0 && m) base = atomicAdd(&gate[1], (uint32_t)__popcll(m));
-    base = __shfl(base, 0);
-    if (hole) rays[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)x | ((uint32_t)y << 16);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t nc = 0, nr = 0;
+        for (int k = 0; k < GATE_TILES; k++) {
+            if (s_n[k] == 0) continue;
+            if ((int)s_n[k] <= max_cnn) nc++;
+            else nr += s_n[k];
+        }
+        s_base[0] = nc ? atomicAdd(&gate[0], nc) : 0u;
+        s_base[1] = nr ? atomicAdd(&gate[GATE_RAYS], nr) : 0u;
+    }
+    __syncthreads();
+    for (int h = 0; h < 2; h++) {
+        const int lt = wave * 2 + h, t = blockIdx.x * GATE_TILES + lt;
+        const uint32_t n = s_n[lt];
+        if (n == 0) continue;
+        // this tile's slot: tiles of lower index in the block come first
+        uint32_t oc = s_base[0], orr = s_base[1];
+        for (int k = 0; k < lt; k++) {
+            if (s_n[k] == 0) continue;
+            if ((int)s_n[k] <= max_cnn) oc++;
+            else orr += s_n[k];
+        }
+        if ((int)n <= max_cnn) {
+            if (lane == 0 && oc < max_tiles) tiles[oc] = (uint32_t)t;
+            continue;
+        }
+        for (int j = 0; j < 4; j++) {
+            const unsigned long long mm = m[h][j];
+            if ((mm >> lane) & 1ull) {
+                const int i = lane + 64 * j;
+                const int x = (t % tiles_x) * 16 + (i & 15), y = (t / tiles_x) * 16 + (i >> 4);
+                rays[orr + (uint32_t)__popcll(mm & ((1ull << lane) - 1ull))] = (uint32_t)x | ((uint32_t)y << 16);
+            }
+            orr += (uint32_t)__popcll(mm);
+        }
+    }
 }
 
 }  // namespace
@@ -843,7 +883,7 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
     if (hipMalloc(&c->d_screen, 4 * N * 4) != hipSuccess || hipMalloc(&c->d_coords, 4 * N * 16) != hipSuccess ||
         hipMalloc(&c->d_ids, ids_words * 4) != hipSuccess || hipMalloc(&c->d_image, N * 4) != hipSuccess ||
         hipMalloc(&c->d_keys, N * 8) != hipSuccess || hipMalloc(&c->d_snap, N * 4) != hipSuccess ||
-        hipMalloc(&c->d_gate, 64) != hipSuccess ||
+        hipMalloc(&c->d_gate, 128) != hipSuccess ||
         hipMalloc(&c->d_gate_tiles, ((size_t)(res_x + 15) / 16) * ((res_y + 15) / 16) * 4) != hipSuccess ||
         hipMalloc(&c->d_gate_rays, N * 4) != hipSuccess) {
         free_frame(c);
@@ -855,7 +895,7 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
     HIPCHECK(hipMemset(c->d_image, 0, N * 4));
     HIPCHECK(hipMemset(c->d_keys, 0xff, N * 8));
     HIPCHECK(hipMemset(c->d_snap, 0, N * 4));
-    HIPCHECK(hipMemset(c->d_gate, 0, 64));
+    HIPCHECK(hipMemset(c->d_gate, 0, 128));
     c->n_tiles_x = (res_x + 15) / 16;
     c->n_tiles_y = (res_y + 15) / 16;
     HIPCHECK(hipDeviceSynchronize());
@@ -1195,12 +1235,13 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     View v = make_view(c, p);
     set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
     v.list2 = c->d_gate_rays;
-    v.list2_count = c->d_gate + 1;
+    v.list2_count = c->d_gate + GATE_RAYS;
     const int tiles = c->n_tiles_x * c->n_tiles_y;
     mark(c, ST_GATE);
     hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, c->d_gate);
-    hipLaunchKernelGGL(k_gate, dim3((unsigned)tiles), dim3(256), 0, c->stream, v, c->n_tiles_x, max_cnn_holes,
-                       c->d_gate, c->d_gate_tiles, c->d_gate_rays, (uint32_t)tiles);
+    hipLaunchKernelGGL(k_gate, dim3((unsigned)((tiles + GATE_TILES - 1) / GATE_TILES)), dim3(256), 0, c->stream, v,
+                       c->n_tiles_x, tiles, max_cnn_holes, c->d_gate, c->d_gate_tiles, c->d_gate_rays,
+                       (uint32_t)tiles);
     HIPCHECK(hipGetLastError());
     // hole list, gate rays and the refresh tile in one ray launch
     mark(c, ST_RAYS);
@@ -1520,7 +1561,8 @@ extern "C" int svo_gate_counts(svo_ctx* c, uint32_t* cnn_tiles, uint32_t* gate_r
     uint32_t g[2];
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    HIPCHECK(hipMemcpy(g, c->d_gate, 8, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(&g[0], c->d_gate, 4, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(&g[1], c->d_gate + GATE_RAYS, 4, hipMemcpyDeviceToHost));
     *cnn_tiles = g[0];
     *gate_rays = g[1];
     return SVO_OK;

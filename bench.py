@@ -286,6 +286,21 @@ def run_batch(args):
     t_seq, rays_seq = seq_run()
     t_bat, rays_bat = batch_run()
     k = n_frames - warm
+    # the same frames with the learned fill on (gate + fused CNN per view)
+    from svo_amd import learned as L
+    if args.learned and os.path.exists(L.WEIGHTS):
+        cnn = S.Cnn(0)
+        L.load(cnn, L.WEIGHTS)
+        for ps in frames[:warm]:
+            batch.frame_warp_learned(cnn, ps, args.max_cnn_holes)
+        batch.sync()
+        t0 = time.perf_counter()
+        for ps in frames[warm:]:
+            batch.frame_warp_learned(cnn, ps, args.max_cnn_holes)
+        batch.sync()
+        t_bl = time.perf_counter() - t0
+        out["learned_batch_view_frames_per_s"] = round(k * len(views) / t_bl, 1)
+        cnn.close()
     out.update(
         batch_frames_per_s=round(k / t_bat, 2), batch_view_frames_per_s=round(k * len(views) / t_bat, 1),
         sequential_view_frames_per_s=round(k * len(views) / t_seq, 1),

@@ -299,6 +299,12 @@ int svo_batch_destroy(svo_batch* batch);
  * view's later work is ordered after the batch. */
 int svo_batch_frame_warp(svo_batch* batch, const svo_frame_params* params);
 int svo_batch_sync(svo_batch* batch);
+/* svo_frame_warp_learned for every view (same cnn and max_cnn_holes for all):
+ * the gate of all views in one launch, their gate pixels in the shared ray
+ * launch, then the CNN fill of each view.  Per view identical to
+ * svo_frame_warp_learned on that context. */
+int svo_batch_frame_warp_learned(svo_batch* batch, svo_cnn* cnn, const svo_frame_params* params,
+                                 int max_cnn_holes);
 
 #ifdef __cplusplus
 }

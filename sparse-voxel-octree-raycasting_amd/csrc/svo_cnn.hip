@@ -966,7 +966,7 @@ static unsigned tile_grid(int max_tiles, int device)
 }
 
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
-                   const uint32_t* tile_count, int max_tiles)
+                   const uint32_t* tile_count, int max_tiles, hipStream_t stream)
 {
     if (n->fused) {
         TileArgs a = tile_args(n);
@@ -981,14 +981,14 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
         a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
         a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
         a.image = c->d_image;
-        hipLaunchKernelGGL(k_cnn_tiles, dim3(tile_grid(max_tiles, c->device)), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL(k_cnn_tiles, dim3(tile_grid(max_tiles, c->device)), dim3(256), 0, stream, a);
         HIPCHECK(hipGetLastError());
         return SVO_OK;
     }
     int rc = ensure_ws(n, (size_t)max_tiles * 576);
     if (rc) return rc;
     hipStream_t keep = n->stream;
-    n->stream = c->stream;  // run on the frame's stream
+    n->stream = stream;  // run on the frame's stream
     FeatArgs a = feat_args(n, c, p, 24);
     a.tiles = tile_list;
     a.count = tile_count;

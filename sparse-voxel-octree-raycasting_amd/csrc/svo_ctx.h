@@ -65,4 +65,4 @@ struct svo_ctx {
 
 // learned-fill hooks implemented in svo_cnn.hip
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
-                   const uint32_t* tile_count, int max_tiles);
+                   const uint32_t* tile_count, int max_tiles, hipStream_t stream);

@@ -131,6 +131,11 @@ struct View {
     int add_x, add_y, ext_x, ext_y, tiles_x;  // refresh rectangle
     uint32_t n_rect;
     uint32_t* ray_image;                   // colorize ray words straight into this, or null
+    // learned-fill gate of this view: counters, CNN tile list, gate pixel list
+    uint32_t* gate;
+    uint32_t* gate_tiles;
+    uint32_t* gate_rays;
+    int gate_tiles_x, gate_n_tiles;
 };
 
 #define VIEW_OF(idx) const View& V = views ? views[(idx)] : v
@@ -677,7 +682,12 @@ __global__ __launch_bounds__(256) void k_fill_color(View v, const View* views, c
 // ---------------------------------------------------------------- learned-fill gate
 // gate counters: [0] CNN tiles, [GATE_RAYS] gate rays (a separate 64-byte line)
 constexpr int GATE_RAYS = 16;
-__global__ void k_gate_reset(uint32_t* gate) { gate[0] = 0u; gate[GATE_RAYS] = 0u; }
+__global__ void k_gate_reset(View v, const View* views)
+{
+    VIEW_OF(blockIdx.y);
+    V.gate[0] = 0u;
+    V.gate[GATE_RAYS] = 0u;
+}
 
 // After compaction and before the rays, each 16x16 tile counts its residual
 // holes: hole pixels that neither the hole list (all-hole 2x2 quads of the
@@ -687,12 +697,17 @@ __global__ void k_gate_reset(uint32_t* gate) { gate[0] = 0u; gate[GATE_RAYS] = 0
 // (each wave two, lane = 4 pixels) and appends its CNN tiles and gate pixels
 // with one atomic per list, so the counters see few, uncontended atomics.
 constexpr int GATE_TILES = 8;
-__global__ __launch_bounds__(256) void k_gate(View v, int tiles_x, int n_tiles, int max_cnn, uint32_t* gate,
-                                              uint32_t* tiles, uint32_t* rays, uint32_t max_tiles)
+__global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max_cnn)
 {
+    VIEW_OF(blockIdx.y);
     __shared__ uint32_t s_n[GATE_TILES], s_base[2];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int res_x = v.res_x, res_y = v.res_y;
+    const int res_x = V.res_x, res_y = V.res_y;
+    const int tiles_x = V.gate_tiles_x, n_tiles = V.gate_n_tiles;
+    uint32_t* gate = V.gate;
+    uint32_t* tiles = V.gate_tiles;
+    uint32_t* rays = V.gate_rays;
+    const uint32_t max_tiles = (uint32_t)n_tiles;
     const int fx = (res_x / 16) * 16, fy = (res_y / 16) * 16;
     unsigned long long m[2][4];
     for (int h = 0; h < 2; h++) {
@@ -701,10 +716,10 @@ __global__ __launch_bounds__(256) void k_gate(View v, int tiles_x, int n_tiles, 
         for (int j = 0; j < 4; j++) {
             const int i = lane + 64 * j;
             const int x = (t % tiles_x) * 16 + (i & 15), y = (t / tiles_x) * 16 + (i >> 4);
-            bool hole = t < n_tiles && x < res_x && y < res_y && v.screen[(size_t)y * res_x + x] == HOLE;
-            if (hole && x < fx && y < fy && quad_is_hole(v.screen, res_x, x & ~1, y & ~1))
+            bool hole = t < n_tiles && x < res_x && y < res_y && V.screen[(size_t)y * res_x + x] == HOLE;
+            if (hole && x < fx && y < fy && quad_is_hole(V.screen, res_x, x & ~1, y & ~1))
                 hole = false;  // in the hole list
-            if (hole && x >= v.add_x && x < v.add_x + v.ext_x && y >= v.add_y && y < v.add_y + v.ext_y)
+            if (hole && x >= V.add_x && x < V.add_x + V.ext_x && y >= V.add_y && y < V.add_y + V.ext_y)
                 hole = false;  // in the refresh rectangle
             m[h][j] = __ballot(hole);
             n += (uint32_t)__popcll(m[h][j]);
@@ -955,6 +970,11 @@ static View make_view(svo_ctx* c, const svo_frame_params* p)
     v.N = (uint32_t)c->N;
     v.S = (uint32_t)c->S;
     v.tiles_x = 1;
+    v.gate = c->d_gate;
+    v.gate_tiles = c->d_gate_tiles;
+    v.gate_rays = c->d_gate_rays;
+    v.gate_tiles_x = c->n_tiles_x;
+    v.gate_n_tiles = c->n_tiles_x * c->n_tiles_y;
     if (p) {
         v.m0x = p->m0[0]; v.m0y = p->m0[1]; v.m0z = p->m0[2];
         v.r00 = p->row0[0]; v.r01 = p->row0[1]; v.r02 = p->row0[2];
@@ -1238,10 +1258,9 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     v.list2_count = c->d_gate + GATE_RAYS;
     const int tiles = c->n_tiles_x * c->n_tiles_y;
     mark(c, ST_GATE);
-    hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, c->d_gate);
+    hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, v, (const View*)nullptr);
     hipLaunchKernelGGL(k_gate, dim3((unsigned)((tiles + GATE_TILES - 1) / GATE_TILES)), dim3(256), 0, c->stream, v,
-                       c->n_tiles_x, tiles, max_cnn_holes, c->d_gate, c->d_gate_tiles, c->d_gate_rays,
-                       (uint32_t)tiles);
+                       (const View*)nullptr, max_cnn_holes);
     HIPCHECK(hipGetLastError());
     // hole list, gate rays and the refresh tile in one ray launch
     mark(c, ST_RAYS);
@@ -1253,7 +1272,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     RC(launch_fill(L, v, nullptr, 1, 1));
     if (cnn) {
         mark(c, ST_CNN);
-        RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles));
+        RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles, c->stream));
     }
     return frame_end(c);
 }
@@ -1430,7 +1449,23 @@ extern "C" int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_vi
 // svo_frame_warp for every view of the batch (params[i] for view i), with all
 // stages batched over the views.  Results per view are identical to
 // svo_frame_warp on that view alone.
+static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cnn, int max_cnn_holes, bool learned);
+
 extern "C" int svo_batch_frame_warp(svo_batch* b, const svo_frame_params* params)
+{
+    return batch_frame(b, params, nullptr, 0, false);
+}
+
+// svo_frame_warp_learned for every view of the batch: the gate of all views
+// in one launch, their gate pixels in the shared ray launch, the CNN per view.
+extern "C" int svo_batch_frame_warp_learned(svo_batch* b, svo_cnn* cnn, const svo_frame_params* params,
+                                            int max_cnn_holes)
+{
+    if (max_cnn_holes < 0 || max_cnn_holes > 256) return svo_fail(SVO_ERR_ARG, "max_cnn_holes outside 0..256");
+    return batch_frame(b, params, cnn, max_cnn_holes, true);
+}
+
+static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cnn, int max_cnn_holes, bool learned)
 {
     if (!b || !params) return svo_fail(SVO_ERR_ARG, "null argument");
     const int n = (int)b->views.size();
@@ -1459,6 +1494,10 @@ extern "C" int svo_batch_frame_warp(svo_batch* b, const svo_frame_params* params
         int ax, ay, ex, ey;
         svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
         set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
+        if (learned) {
+            v.list2 = c->d_gate_rays;
+            v.list2_count = c->d_gate + GATE_RAYS;
+        }
         hv[i] = v;
     }
     HIPCHECK(hipMemcpyAsync(dv, hv, sizeof(View) * n, hipMemcpyHostToDevice, b->stream));
@@ -1468,12 +1507,26 @@ extern "C" int svo_batch_frame_warp(svo_batch* b, const svo_frame_params* params
     RC(launch_count(L, dummy));
     RC(launch_sum(L, dummy));
     RC(launch_write(L, dummy));
+    if (learned) {
+        uint32_t max_gate_tiles = 0;
+        for (auto* c : b->views) max_gate_tiles = std::max(max_gate_tiles, (uint32_t)(c->n_tiles_x * c->n_tiles_y));
+        hipLaunchKernelGGL(k_gate_reset, dim3(1, (unsigned)n), dim3(1), 0, b->stream, dummy, (const View*)dv);
+        hipLaunchKernelGGL(k_gate, dim3((max_gate_tiles + GATE_TILES - 1) / GATE_TILES, (unsigned)n), dim3(256), 0,
+                           b->stream, dummy, (const View*)dv, max_cnn_holes);
+        HIPCHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_view_first, dim3(1), dim3(1), 0, b->stream, (const View*)dv, n, b->d_first,
                        (uint32_t*)nullptr);
     HIPCHECK(hipGetLastError());
     RC(launch_rays_v(c0, b->stream, dummy, dv, b->d_first, n, b->d_queue, b->seq));
     RC(launch_copy(L, dummy, 2));
     RC(launch_fill(L, dummy, nullptr, 1, 1));
+    if (learned && cnn)
+        for (int i = 0; i < n; i++) {
+            svo_ctx* c = b->views[i];
+            RC(cnn_fill_tiles(cnn, c, &params[i], c->d_gate_tiles, c->d_gate, c->n_tiles_x * c->n_tiles_y,
+                              b->stream));
+        }
     HIPCHECK(hipEventRecord(b->slot_done[slot], b->stream));
     for (auto* c : b->views) HIPCHECK(hipStreamWaitEvent(c->stream, b->slot_done[slot], 0));
     b->frame_no++;

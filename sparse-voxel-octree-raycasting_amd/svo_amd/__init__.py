@@ -125,6 +125,8 @@ _f = {
     "batch_destroy": _sig("svo_batch_destroy", C.c_int, _ctxp),
     "batch_frame_warp": _sig("svo_batch_frame_warp", C.c_int, _ctxp, C.POINTER(FrameParams)),
     "batch_sync": _sig("svo_batch_sync", C.c_int, _ctxp),
+    "batch_frame_warp_learned": _sig("svo_batch_frame_warp_learned", C.c_int, _ctxp, _ctxp, C.POINTER(FrameParams),
+                                     C.c_int),
     "rle4_load": _sig("svo_rle4_load", C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                       C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
     "rle4_save": _sig("svo_rle4_save", C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)),
@@ -147,7 +149,7 @@ EXPORTED = [
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
-    "svo_gate_counts",
+    "svo_gate_counts", "svo_batch_frame_warp_learned",
 ]
 
 
@@ -549,6 +551,12 @@ class Batch:
             raise ValueError("one FrameParams per view")
         arr = (FrameParams * len(params))(*params)
         _check(_f["batch_frame_warp"](self._h, arr))
+
+    def frame_warp_learned(self, cnn, params, max_cnn_holes=64):
+        if len(params) != len(self.contexts):
+            raise ValueError("one FrameParams per view")
+        arr = (FrameParams * len(params))(*params)
+        _check(_f["batch_frame_warp_learned"](self._h, cnn._h if cnn is not None else None, arr, int(max_cnn_holes)))
 
     def sync(self):
         _check(_f["batch_sync"](self._h))

@@ -110,3 +110,34 @@ def test_batch_rejects_mismatched_octrees():
     finally:
         a.close()
         b.close()
+
+
+def test_batch_learned_matches_single_views():
+    """Batched learned frames (gate of all views in one launch, gate pixels in
+    the shared ray launch, CNN per view) equal per-view learned frames."""
+    from svo_amd import learned as L
+    words, root = scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
+    pr = path(CONFIG1, 32)
+    res = RES[:4]
+    batched = make_views(words, root, res)
+    single = make_views(words, root, res)
+    cnn = S.Cnn(0)
+    L.load(cnn, L.WEIGHTS)
+    batch = S.Batch(batched)
+    try:
+        for f in range(6):
+            ps = [view_params(pr, v, f) for v in range(len(res))]
+            print(f"batch learned frame {f}", flush=True)
+            batch.frame_warp_learned(cnn, ps, 64)
+            for c, p in zip(single, ps):
+                c.frame_warp_learned(cnn, p, 64)
+            for v, (a, b) in enumerate(zip(batched, single)):
+                a.sync()
+                b.sync()
+                compare(a, b, f"learned frame {f} view {v} {res[v]}")
+                assert a.gate_counts() == b.gate_counts()
+    finally:
+        batch.close()
+        for c in batched + single:
+            c.close()
+        cnn.close()

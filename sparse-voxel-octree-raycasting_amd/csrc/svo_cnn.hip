@@ -342,6 +342,10 @@ __device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out,
     constexpr int NT = COUT / 16;
     const int npix = wout * wout;
     const int row = lane & 15, kg = lane >> 4;
+    // the layer's B fragments stay in registers for all of the wave's M-tiles
+    bf8v bfr[9][NT];
+    for (int tap = 0; tap < 9; tap++)
+        for (int nn = 0; nn < NT; nn++) bfr[tap][nn] = *(const bf8v*)(Wl + swz(tap * COUT + nn * 16 + row, kg));
     for (int m0 = wave * 2; m0 * 16 < npix; m0 += 8) {
         f4v acc[2][NT];
         int q0[2];
@@ -352,12 +356,10 @@ __device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out,
         }
         for (int tap = 0; tap < 9; tap++) {
             const int dq = (tap / 3) * win + tap % 3;
-            bf8v bfr[NT];
-            for (int nn = 0; nn < NT; nn++) bfr[nn] = *(const bf8v*)(Wl + swz(tap * COUT + nn * 16 + row, kg));
             for (int m = 0; m < 2; m++) {
                 const bf8v afr = *(const bf8v*)(in + swz(q0[m] + dq, kg));
                 for (int nn = 0; nn < NT; nn++)
-                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nn], acc[m][nn], 0, 0, 0);
+                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[tap][nn], acc[m][nn], 0, 0, 0);
             }
         }
         for (int m = 0; m < 2; m++)
@@ -427,6 +429,9 @@ __global__ __launch_bounds__(256) void k_cnn_tiles(TileArgs a)
         for (int i = tid; i < 9 * 32 * 16 / 8; i += 256) *(uint4*)(Wl + i * 8) = *(const uint4*)(a.w[0] + i * 8);
         __syncthreads();
         // layer 1: 16 -> 32 on 22x22, input region 24x24 (plain 16-channel pixels)
+        s4v b1[9][2];
+        for (int tap = 0; tap < 9; tap++)
+            for (int nn = 0; nn < 2; nn++) b1[tap][nn] = *(const s4v*)(Wl + (tap * 32 + nn * 16 + row) * 16 + kg * 4);
         for (int m0 = wave * 2; m0 * 16 < 484; m0 += 8) {
             f4v acc[2][2];
             int q0[2];
@@ -437,12 +442,10 @@ __global__ __launch_bounds__(256) void k_cnn_tiles(TileArgs a)
             }
             for (int tap = 0; tap < 9; tap++) {
                 const int dq = (tap / 3) * 24 + tap % 3;
-                s4v bfr[2];
-                for (int nn = 0; nn < 2; nn++) bfr[nn] = *(const s4v*)(Wl + (tap * 32 + nn * 16 + row) * 16 + kg * 4);
                 for (int m = 0; m < 2; m++) {
                     const s4v afr = *(const s4v*)(P + (q0[m] + dq) * 16 + kg * 4);
                     for (int nn = 0; nn < 2; nn++)
-                        acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[nn], acc[m][nn], 0, 0, 0);
+                        acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, b1[tap][nn], acc[m][nn], 0, 0, 0);
                 }
             }
             for (int m = 0; m < 2; m++)

@@ -73,3 +73,46 @@ def test_workload_properties(big):
             assert np.all(lst[:, 3] == lst[:, 0] + (1 << 16))
             blk = (y // 16) * (rx // 16) + x // 16
             assert np.all(np.diff(blk.astype(np.int64)) >= 0)
+
+
+@pytest.mark.parametrize("res", [(8192, 8192), (8192, 16), (16, 8192)])
+def test_maximum_resolution_sampled_vs_oracle(big, res):
+    """The largest (and most lopsided) frames svo_resize accepts: a full frame
+    checked against the oracle's single-ray trace on 1500 seeded pixels, then
+    warp frames whose id list matches plane 0's hole quads and whose graph
+    replay matches direct launches on the same pixels."""
+    words, root, depth = big
+    rx, ry = res
+    rows = W.path(NAME, 4)
+    rng = np.random.default_rng(rx * 7 + ry)
+    xs = rng.integers(0, rx, 1500)
+    ys = rng.integers(0, ry, 1500)
+    with S.Context(0) as c:
+        c.upload_octree(words, root, depth)
+        c.resize(rx, ry)
+        p = W.frame_params(NAME, rows, 3)
+        print(f"full frame {rx}x{ry}", flush=True)
+        c.frame_full(p)
+        c.sync()
+        scr = c.screen(0).reshape(ry, rx)
+        crd = c.coords(0).reshape(ry, rx, 4)
+        op = O.params_from(p)
+        for x, y in zip(xs, ys):
+            w, xyz, _ = O.trace_pixel(words, root, depth, rx, ry, op, int(x), int(y))
+            assert int(scr[y, x]) == w, (x, y)
+            assert_equal_f32_bits(crd[y, x, :3], np.array(xyz, np.float32), f"pixel {x},{y}")
+        # warp frames: graph (frames >= 2) on c, direct launches on d
+        with S.Context(0) as d:
+            d.upload_octree(words, root, depth)
+            d.resize(rx, ry)
+            d.set_graph(False)
+            for f in range(4):
+                p = W.frame_params(NAME, rows, f)
+                print(f"warp frame {f} {rx}x{ry}", flush=True)
+                c.frame_warp(p)
+                d.frame_warp(p)
+                c.sync()
+                d.sync()
+                a, b = c.screen(0), d.screen(0)
+                assert np.array_equal(a[ys * rx + xs], b[ys * rx + xs])
+                assert int(c.ids()[0]) == int(d.ids()[0])

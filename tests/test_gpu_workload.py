@@ -113,6 +113,7 @@ def test_maximum_resolution_sampled_vs_oracle(big, res):
                 d.frame_warp(p)
                 c.sync()
                 d.sync()
-                a, b = c.screen(0), d.screen(0)
-                assert np.array_equal(a[ys * rx + xs], b[ys * rx + xs])
+                a, b = c.screen(0).reshape(-1), d.screen(0).reshape(-1)
+                sel = ys.astype(np.int64) * rx + xs
+                assert np.array_equal(a[sel], b[sel])
                 assert int(c.ids()[0]) == int(d.ids()[0])

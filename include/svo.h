@@ -186,6 +186,10 @@ int svo_image_save_ppm(const char* path, const uint32_t* image, int w, int h);
  * *xyz / *rgba are malloc'ed (svo_free). */
 int svo_scene_generate(int kind, uint64_t seed, int depth, int n_spheres,
                        uint16_t** xyz, uint32_t** rgba, size_t* n);
+/* svo_scene_generate for kind 3 (the cave) on a GPU: the same voxel stream,
+ * bit for bit, in seconds at depth 12 (the host needs hours there). */
+int svo_scene_generate_gpu(int device, int kind, uint64_t seed, int depth,
+                           uint16_t** xyz, uint32_t** rgba, size_t* n);
 
 /* Seeded Catmull-Rom flythrough: 6 floats per frame
  * (yaw, pitch, roll, pos_x, pos_y, pos_z), world units (1 voxel = 1/8).

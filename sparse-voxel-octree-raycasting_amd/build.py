@@ -38,7 +38,7 @@ def build(verbose=True):
     host_o = os.path.join(BUILD, "svo_host.o")
     cmds = [["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "svo_host.cpp"), "-o", host_o]]
     objs = [host_o]
-    for src in ("svo_device.hip", "svo_cnn.hip", "svo_build.hip"):
+    for src in ("svo_device.hip", "svo_cnn.hip", "svo_build.hip", "svo_scene.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -105,6 +105,8 @@ _f = {
     "octree_save": _sig("svo_octree_save", C.c_int, C.c_char_p, _u32p, C.c_size_t, C.c_uint32, C.c_int),
     "octree_load": _sig("svo_octree_load", C.c_int, C.c_char_p, C.POINTER(_u32p),
                         C.POINTER(C.c_size_t), C.POINTER(C.c_uint32), C.POINTER(C.c_int)),
+    "scene_generate_gpu": _sig("svo_scene_generate_gpu", C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int,
+                               C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
     "scene_generate": _sig("svo_scene_generate", C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
                            C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
     "cnn_create": _sig("svo_cnn_create", C.c_int, C.POINTER(_ctxp), C.c_int, C.c_uint64),
@@ -149,7 +151,7 @@ EXPORTED = [
     "svo_batch_create", "svo_batch_destroy", "svo_batch_frame_warp", "svo_batch_sync",
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
-    "svo_gate_counts", "svo_batch_frame_warp_learned",
+    "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
 ]
 
 
@@ -233,6 +235,16 @@ def scene(kind, seed, depth, n_spheres=0):
     _f["free"](C.cast(xyz, _vp))
     _f["free"](C.cast(rgba, _vp))
     return a, b
+
+
+def scene_gpu(kind, seed, depth, device=0):
+    """scene() for the cave (kind 3) generated on the GPU: the same stream."""
+    xyz = _u16p()
+    rgba = _u32p()
+    n = C.c_size_t()
+    _check(_f["scene_generate_gpu"](int(device), int(kind), int(seed), int(depth), C.byref(xyz), C.byref(rgba),
+                                    C.byref(n)))
+    return _take_stream(xyz, rgba, n)
 
 
 def _take_stream(xyz, rgba, n):

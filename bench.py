@@ -218,6 +218,74 @@ def run_gpu(args, name):
     return res
 
 
+def run_stress(args, name="cave_4096", n_timed=60):
+    """Divergence stress: the depth-12 cave with a fast camera (up to 32
+    voxels and 5 deg yaw per frame).  Scene generated and octree built on the
+    GPU; warp frames (throughput and per-frame events, hole fraction read per
+    frame) and full frames."""
+    import svo_amd as S
+    from svo_amd import workloads as W
+
+    w = W.WORKLOADS[name]
+    t = time.perf_counter()
+    xyz, rgba = W.voxels(name)
+    t_gen = time.perf_counter() - t
+    t = time.perf_counter()
+    words, root, depth, nvox = W.octree(name)
+    t_build = time.perf_counter() - t
+    log(f"[bench] {name}: {nvox} voxels generated in {t_gen:.2f}s, {len(words)} words built in {t_build:.2f}s")
+    rx, ry = w["res"]
+    total = 10 + n_timed
+    rows = W.path(name, total)
+    params = [W.frame_params(name, rows, f) for f in range(total)]
+    warp = S.Context(0)
+    full = S.Context(0)
+    for c in (warp, full):
+        c.upload_octree(words, root, depth)
+        c.resize(rx, ry)
+    for f in range(10):
+        warp.frame_warp(params[f])
+    warp.sync()
+    warp.ray_count(reset=True)
+    t0 = time.perf_counter()
+    for f in range(10, total):
+        warp.frame_warp(params[f])
+    warp.sync()
+    t_warp = time.perf_counter() - t0
+    rays = warp.ray_count(reset=True)
+    # per-frame GPU time and hole fraction on fresh state
+    w2 = S.Context(0)
+    w2.upload_octree(words, root, depth)
+    w2.resize(rx, ry)
+    for f in range(10):
+        w2.frame_warp(params[f])
+    w2.sync()
+    w2.enable_timing(mode=2)
+    ms, holes = [], []
+    for f in range(10, total):
+        w2.frame_warp(params[f])
+        w2.sync()
+        ms.append(w2.frame_time())
+        holes.append(w2.hole_count() / (rx * ry))
+    full.enable_timing(mode=2)
+    fm = []
+    for f in range(10, min(total, 30)):
+        full.frame_full(params[f])
+        full.sync()
+        fm.append(full.frame_time())
+    for c in (warp, full, w2):
+        c.close()
+    return {"workload": f"{name}: {rx}x{ry}, SVO depth {depth}, 3-D noise cave shell, up to {w['vox']:.0f} voxels "
+                        f"and {w['yaw']:.0f} deg yaw per frame",
+            "voxels": int(nvox), "octree_words": int(len(words)), "scene_gpu_s": round(t_gen, 2),
+            "build_gpu_s": round(t_build, 2), "warp_fps": round(n_timed / t_warp, 2),
+            "warp_p50_ms": round(pct(ms, 50), 4), "warp_p99_ms": round(pct(ms, 99), 4),
+            "full_p50_ms": round(pct(fm, 50), 4), "full_fps_from_p50": round(1000.0 / pct(fm, 50), 2),
+            "hole_list_fraction_mean": round(float(np.mean(holes)), 4),
+            "hole_list_fraction_max": round(float(np.max(holes)), 4),
+            "rays_per_warp_frame": round(rays / n_timed)}
+
+
 def run_batch(args):
     """Batched views: 24 seeded cameras of uneven resolution over the depth-11
     scene.  The same frames run once view by view (svo_frame_warp per
@@ -365,6 +433,8 @@ def main():
     ap.add_argument("--no-learned", dest="learned", action="store_false", help="skip the learned-fill leg")
     ap.add_argument("--max-cnn-holes", type=int, default=64)
     ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the batched-views leg")
+    ap.add_argument("--no-stress", dest="stress", action="store_false",
+                    help="skip the divergence-stress leg (depth-12 cave)")
     ap.add_argument("--batch-warmup", type=int, default=8)
     ap.add_argument("--legs", default="all", choices=["all", "warp"],
                     help="warp: only the warp-frame legs (for profiling the warp kernels)")
@@ -378,6 +448,9 @@ def main():
 
     g = run_gpu(args, args.workload)
     batch = run_batch(args) if (args.batch and args.legs == "all") else None
+    stress = run_stress(args) if (args.stress and args.legs == "all") else None
+    if stress:
+        log(f"[bench] divergence stress: {stress}")
     # dominant kernel: ray traversal in the warp frames
     ray_ms = g["stage_mean"].get("rays", float("nan"))
     cpu = None
@@ -441,6 +514,7 @@ def main():
                          "avg_launch_ms": round(g["cnn_ms"], 4), "flops_per_tile": CNN_TILE_FLOPS}},
         "octree_build": g["octree_build"],
         "batched_views": batch,
+        "divergence_stress": stress,
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["warp_fps"], 3), "unit": "frames/s", "cores": threads, "kind": "port",

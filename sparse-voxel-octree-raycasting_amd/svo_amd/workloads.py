@@ -6,6 +6,9 @@ arguments.  Scene seed = 1000 + n, path seed = 2000 + n for workload n.
   flythrough_2048: depth 11 (2048^3) 4-octave fractal terrain shell + 2000
                    sphere shells (radius 4..64 voxels), 1920x1024, 300 frames,
                    1..8 voxels and <= 1 deg yaw per frame
+  cave_4096:       depth 12 (4096^3) 5-octave 3-D noise cave (boundary voxels of
+                   a slab 512 voxels high), 1920x1024, 200 frames, up to 32
+                   voxels and 5 deg yaw per frame (divergence stress)
   views_24:        24 independent cameras over the flythrough_2048 scene,
                    per-view resolution drawn from {320x240, 640x480,
                    1280x720, 1920x1024}, per-view motion up to 0.5..16
@@ -17,29 +20,36 @@ import functools
 
 import numpy as np
 
-from . import build_octree, camera, camera_path, scene
+from . import build_octree, build_octree_gpu, camera, camera_path, scene, scene_gpu
 
 WORKLOADS = {
     "flythrough_256": dict(kind=1, seed=1001, depth=8, spheres=0, res=(512, 256), frames=64,
                            vox=2.0, yaw=0.5, height=240.0, pitch=0.35, path_seed=2001),
     "flythrough_2048": dict(kind=2, seed=1002, depth=11, spheres=2000, res=(1920, 1024), frames=300,
                             vox=8.0, yaw=1.0, height=1400.0, pitch=0.30, path_seed=2002),
+    "cave_4096": dict(kind=3, seed=1003, depth=12, spheres=0, res=(1920, 1024), frames=200,
+                      vox=32.0, yaw=5.0, height=256.0, pitch=0.05, path_seed=2003),
 }
 
 
 @functools.lru_cache(maxsize=2)
 def voxels(name):
-    """(xyz, rgba) voxel stream of a workload's scene."""
+    """(xyz, rgba) voxel stream of a workload's scene (the cave on the GPU:
+    the host generator needs hours at depth 12; both give the same stream)."""
     w = WORKLOADS[name]
+    if w["kind"] == 3:
+        return scene_gpu(3, w["seed"], w["depth"])
     return scene(w["kind"], w["seed"], w["depth"], w["spheres"])
 
 
 @functools.lru_cache(maxsize=4)
 def octree(name):
-    """(words, root_ref, depth, n_voxels) of a workload's scene."""
+    """(words, root_ref, depth, n_voxels) of a workload's scene (the cave's
+    octree is built on the GPU; the host builder gives the same words)."""
     w = WORKLOADS[name]
     xyz, rgba = voxels(name)
-    words, root = build_octree(xyz, rgba, w["depth"])
+    build = build_octree_gpu if w["kind"] == 3 else build_octree
+    words, root = build(xyz, rgba, w["depth"])
     return words, root, w["depth"], len(rgba)
 
 

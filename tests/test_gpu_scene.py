@@ -42,3 +42,28 @@ def test_cave_depth12_stream_properties():
     assert len(words) < (1 << 29)
     with S.Context(0) as c:
         c.upload_octree(words, root, 12)
+
+
+def test_cave_depth9_warp_frames_vs_oracle():
+    """The divergence-stress scene kind (cave, fast camera) through the warp
+    pipeline: bit-exact against the oracle at depth 9."""
+    import oracle as O
+    from helpers import assert_equal_f32_bits, assert_equal_u32
+    xyz, rgba = S.scene_gpu(3, 1003, 9)
+    words, root = S.build_octree_gpu(xyz, rgba, 9)
+    rows = S.camera_path(2003, 9, 8, 4.0, 5.0, 32.0, 0.05)
+    rx, ry = 320, 160
+    o = O.Frame(rx, ry, words, root, 9, nthreads=8)
+    with S.Context(0) as c:
+        c.upload_octree(words, root, 9)
+        c.resize(rx, ry)
+        for f in range(8):
+            yaw, pitch, roll, x, y, z = [float(v) for v in rows[f]]
+            p, _ = S.camera(yaw, pitch, roll, (x, y, z), 9, f)
+            c.frame_warp(p)
+            o.frame_warp(O.params_from(p))
+            c.sync()
+            assert_equal_u32(c.screen(), o.screen, f"cave frame {f} screen")
+            assert_equal_f32_bits(c.coords(), o.coords, f"cave frame {f} coords")
+            assert_equal_u32(c.image(), o.image, f"cave frame {f} image")
+    o.close()

@@ -238,6 +238,12 @@ int svo_set_tuning(svo_ctx* ctx, int refill_threshold, int wg_per_cu,
  * at once (warp frames) and keeps list order for larger ones (full frames).
  * Default: group 1, only_if_fits 1.  Results do not depend on it. */
 int svo_set_ray_spread(svo_ctx* ctx, int group, int only_if_fits);
+/* Lanes per wave that take rays (power of two, 1..64; default 64).  Fewer
+ * lanes spread the same rays over more waves, so fewer long rays share one
+ * (a wave runs both branch bodies while its rays disagree).  The spread order
+ * then places the rays of a wave 1/lanes of the list apart.  Results do not
+ * depend on it. */
+int svo_set_ray_lanes(svo_ctx* ctx, int lanes);
 
 /* ---- learned hole filler (no counterpart in the reference) ------------
  * A 4-layer 3x3 CNN (5 -> 32 -> 32 -> 32 -> 3 channels, ReLU) run with BF16

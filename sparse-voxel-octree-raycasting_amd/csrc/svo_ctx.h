@@ -44,6 +44,7 @@ struct svo_ctx {
     uint32_t cache_req = 0;  // requested LDS node-cache words
     int spread_shift = 0;    // ray hand-out: log2 of the spread group (6 = list order)
     int spread_fit = 1;      // spread only launches whose rays all fit in the resident lanes
+    int lanes_shift = 6;     // log2 of the lanes per wave that take rays
     // per-frame graph of the warp frame (frames >= 2): its kernels read the
     // frame's view from d_view, refreshed from a pinned 2-slot staging ring
     bool use_graph = true;

@@ -156,8 +156,9 @@ struct RayArgs {
     uint32_t cache_words;           // octree prefix held in LDS
     uint32_t cache_last;            // max(cache_words, 1) - 1
     uint32_t oct_bytes;             // buffer extent of the octree (words + pad)
-    int spread_shift;               // log2 of the spread group (6 = list order)
+    int spread_shift;               // log2 of the spread group (>= lanes_shift: list order)
     int spread_fit;                 // spread only when all rays get a lane at once
+    int lanes_shift;                // log2 of the lanes of a wave that take rays (6: all 64)
 };
 
 struct Ray {
@@ -387,6 +388,15 @@ __device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, OctBuf ob, 
     if (V.ray_image) V.ray_image[r.pix] = colorize_word(word);
 }
 
+// lanes 0, 64/K, 2*64/K, ... of a wave (K = 2^ks lanes)
+__device__ __forceinline__ unsigned long long lane_pattern(int ks)
+{
+    unsigned long long m = 0;
+    const int step = 64 >> ks;
+    for (int l = 0; l < 64; l += step) m |= 1ull << l;
+    return m;
+}
+
 template <bool kCache>
 __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
 {
@@ -408,42 +418,51 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     // in the list, so the long hole rays of one screen region (near the
     // horizon) land in different waves; a wave then rarely runs many of them
     // through both branch bodies.  G = 64 is list order.
+    // Only K = 2^lanes_shift lanes of a wave take rays (every 64/K-th lane):
+    // with K < 64 the same rays occupy more waves, so fewer long rays share
+    // one.  The spread mapping then works on K-lane queue waves.
+    const int ks = a.lanes_shift;
+    const uint32_t K = 1u << ks;
+    const unsigned long long elig = ks >= 6 ? ~0ull : lane_pattern(ks);
     const uint32_t P = (a.views && a.n_views > 1) ? total : total - (a.views ? a.views[0].n_rect : a.v.n_rect);
-    const uint32_t rows = (P + 63u) >> 6;
-    const bool spread = a.spread_shift < 6 && (!a.spread_fit || total <= gridDim.x * (uint32_t)RAY_BLOCK);
-    const uint32_t qtotal = spread ? rows * 64u + (total - P) : total;
+    const uint32_t rows = (P + K - 1u) >> ks;
+    const bool spread = a.spread_shift < ks &&
+                        (!a.spread_fit || total <= gridDim.x * (uint32_t)(RAY_BLOCK / 64) * K);
+    const uint32_t qtotal = spread ? rows * K + (total - P) : total;
     const uint32_t gmask = (1u << a.spread_shift) - 1u;
+    const int refill = min(a.refill, (int)K);
     if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
 
     Ray r;
     bool active = false, exhausted = false;
     uint32_t traced = 0;  // lane 0 of each wave: rays started by the wave
     for (;;) {
-        const unsigned long long idle = __ballot(!active);
+        const unsigned long long idle_all = __ballot(!active);
+        const unsigned long long idle = idle_all & elig;
         const int n_idle = __popcll(idle);
-        if (!exhausted && (n_idle >= a.refill || n_idle == 64)) {
+        if (!exhausted && (n_idle >= refill || n_idle == (int)K)) {
             const int leader = __ffsll((long long)idle) - 1;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(&a.queue[a.slot], (uint32_t)n_idle);
             base = __shfl(base, leader);
             if (base + (uint32_t)n_idle >= qtotal) exhausted = true;
-            if (!active) {
+            if ((idle >> lane) & 1ull) {
                 const int rank = __popcll(idle & ((1ull << lane) - 1ull));
                 const uint32_t k = base + (uint32_t)rank;
-                const uint32_t l = k & 63u;
+                const uint32_t l = k & (K - 1u);
                 uint32_t kr = k;
                 bool valid = k < qtotal;
                 if (spread) {
-                    if (k < rows * 64u) {
-                        kr = (((l >> a.spread_shift) * rows + (k >> 6)) << a.spread_shift) | (l & gmask);
+                    if (k < rows * K) {
+                        kr = (((l >> a.spread_shift) * rows + (k >> ks)) << a.spread_shift) | (l & gmask);
                         valid = kr < P;
                     } else {
-                        kr = P + (k - rows * 64u);
+                        kr = P + (k - rows * K);
                     }
                 }
                 if (valid) active = init_ray(r, kr, a);
             }
-            traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)(64 - n_idle);
+            traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)__popcll(~idle_all);
         }
         if (__ballot(active) == 0ull) {
             if (exhausted) break;
@@ -1098,6 +1117,7 @@ static int launch_rays_v(svo_ctx* c, hipStream_t stream, const View& v, const Vi
     a.iters = c->iters;
     a.spread_shift = c->spread_shift;
     a.spread_fit = c->spread_fit;
+    a.lanes_shift = c->lanes_shift;
     a.ray_count = (unsigned long long*)(c->d_queue + 8);
     a.cache_words = std::min(c->cache_req, c->normal_words);
     a.cache_last = a.cache_words ? a.cache_words - 1 : 0;
@@ -1664,6 +1684,20 @@ extern "C" int svo_set_ray_spread(svo_ctx* c, int group, int only_if_fits)
     while ((1 << shift) < group) shift++;
     c->spread_shift = shift;
     c->spread_fit = only_if_fits != 0;
+    return SVO_OK;
+}
+
+extern "C" int svo_set_ray_lanes(svo_ctx* c, int lanes)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)))
+        return svo_fail(SVO_ERR_ARG, "lanes %d is not a power of two in 1..64", lanes);
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    int shift = 0;
+    while ((1 << shift) < lanes) shift++;
+    c->lanes_shift = shift;
     return SVO_OK;
 }
 

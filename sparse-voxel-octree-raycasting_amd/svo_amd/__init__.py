@@ -92,6 +92,7 @@ _f = {
     "set_graph": _sig("svo_set_graph", C.c_int, _ctxp, C.c_int),
     "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int, C.c_int),
     "set_ray_spread": _sig("svo_set_ray_spread", C.c_int, _ctxp, C.c_int, C.c_int),
+    "set_ray_lanes": _sig("svo_set_ray_lanes", C.c_int, _ctxp, C.c_int),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
     "tile_schedule": _sig("svo_tile_schedule", C.c_int, C.c_int, C.c_int, C.c_int,
@@ -152,6 +153,7 @@ EXPORTED = [
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
+    "svo_set_ray_lanes",
 ]
 
 
@@ -406,6 +408,10 @@ class Context:
         waves (64 = list order); only_if_fits limits it to launches whose rays
         all fit in the resident lanes."""
         _check(_f["set_ray_spread"](self._h, int(group), 1 if only_if_fits else 0))
+
+    def set_ray_lanes(self, lanes=64):
+        """Lanes per wave that take rays (64 = all)."""
+        _check(_f["set_ray_lanes"](self._h, int(lanes)))
 
     def ray_count(self, reset=False):
         """Rays traced by the ray kernels since the last reset."""

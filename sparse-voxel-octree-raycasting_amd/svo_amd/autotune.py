@@ -1,7 +1,8 @@
 """Autotuner for the ray kernel's launch knobs.
 
 Knobs: ray hand-out order (svo_set_ray_spread: the size of the ray groups
-spread across waves, 64 = list order) and, through svo_set_tuning, the work-queue refill threshold (idle
+spread across waves, 64 = list order), lanes per wave that take rays
+(svo_set_ray_lanes) and, through svo_set_tuning, the work-queue refill threshold (idle
 lanes per refill), persistent workgroups per CU, traversal steps between
 refill checks, and the LDS node-cache depth (octree words of the
 breadth-first top levels kept in LDS).  Every candidate replays the same frame window from the same starting
@@ -18,6 +19,7 @@ import time
 
 DEFAULT_SPACE = {
     "spread": [1, 64],
+    "lanes": [64, 32, 16],
     "refill": [8, 16, 32, 48],
     "wg_per_cu": [4, 8, 16],
     "iters": [8, 16, 64],
@@ -69,6 +71,8 @@ def apply(ctx, cfg):
     ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
     if "spread" in cfg:
         ctx.set_ray_spread(int(cfg["spread"]), True)
+    if "lanes" in cfg:
+        ctx.set_ray_lanes(int(cfg["lanes"]))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):

@@ -111,9 +111,10 @@ def test_warp_graph_vs_direct_launches(scene1):
         d.close()
 
 
-@pytest.mark.parametrize("spread,refill,wg,iters,cache", [(64, 16, 4, 64, 0), (1, 48, 8, 8, 736),
-                                                          (4, 1, 1, 1, 4096), (16, 64, 32, 4096, 96)])
-def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, cache):
+@pytest.mark.parametrize("spread,refill,wg,iters,cache,lanes", [(64, 16, 4, 64, 0, 64), (1, 48, 8, 8, 736, 32),
+                                                                (4, 1, 1, 1, 4096, 16), (16, 64, 32, 4096, 96, 8),
+                                                                (1, 8, 2, 16, 0, 1)])
+def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, cache, lanes):
     """Ray hand-out order, refill threshold, workgroups per CU, trips per
     refill check and the LDS node cache change only the schedule."""
     words, root = scene1
@@ -126,9 +127,10 @@ def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, c
         c.resize(rx, ry)
         c.set_tuning(refill, wg, iters, cache)
         c.set_ray_spread(spread, only_if_fits=(spread != 16))
+        c.set_ray_lanes(lanes)
         for f in range(6):
             p = params(CONFIG1, pr, f)
-            print(f"knobs {spread, refill, wg, iters, cache} frame {f}", flush=True)
+            print(f"knobs {spread, refill, wg, iters, cache, lanes} frame {f}", flush=True)
             c.frame_warp(p)
             o.frame_warp(O.params_from(p))
             c.sync()

@@ -7,8 +7,9 @@ lanes per refill), persistent workgroups per CU, traversal steps between
 refill checks, and the LDS node-cache depth (octree words of the
 breadth-first top levels kept in LDS).  Every candidate replays the same frame window from the same starting
 state (the window starts with the two clearing frames), so candidates see
-identical work; the score is the median GPU time per frame over the timed
-frames, measured with the library's per-frame HIP-event pair.  Results do not
+identical work; the score is the mean GPU time per frame over the timed
+frames (frames/s follows the mean), measured with the library's per-frame
+HIP-event pair.  Results do not
 depend on the knobs (the tests check bit-exactness under several settings).
 """
 from __future__ import annotations
@@ -39,7 +40,7 @@ def _frame_time(ctx, params, warmup, mode):
         ctx.sync()
         ts.append(ctx.frame_time())
     ctx.enable_timing(False)
-    return statistics.median(ts)
+    return statistics.mean(ts)  # frames/s is set by the mean
 
 
 def _search(space, coordinate, measure):
@@ -102,7 +103,7 @@ def tune_batch(batch, frames, warmup=4, space=None, coordinate=True, log=None):
     """Tune the shared ray launch of a Batch (its first view's knobs).
 
     frames: list of per-frame FrameParams lists, one entry per view, starting
-    at frame 0.  The score is the median wall time of one batched frame
+    at frame 0.  The score is the mean wall time of one batched frame
     (launch + device completion); returns (best_setting, best_ms, trials)."""
     space = dict(DEFAULT_SPACE if space is None else space)
     ctx0 = batch.contexts[0]
@@ -119,7 +120,7 @@ def tune_batch(batch, frames, warmup=4, space=None, coordinate=True, log=None):
             batch.frame_warp(ps)
             batch.sync()
             ts.append((time.perf_counter() - t0) * 1e3)
-        ms = statistics.median(ts)
+        ms = statistics.mean(ts)
         trials.append((dict(cfg), ms))
         if log:
             log(f"[autotune batch] {cfg} -> {ms:.4f} ms")

@@ -299,8 +299,8 @@ int svo_gate_counts(svo_ctx* ctx, uint32_t* cnn_tiles, uint32_t* gate_rays);
  * tile rays of all views share one persistent ray launch, so the long-ray
  * tails of different views overlap.  Each view's outputs equal those of
  * svo_frame_warp on that context.  The views keep their own resolution and
- * frame counter; the contexts must not be resized or given a new octree while
- * the batch exists.  The ray kernel uses the first view's tuning and counts
+ * frame counter; the contexts must not be resized, given a new octree or
+ * destroyed while the batch exists.  The ray kernel uses the first view's tuning and counts
  * rays on the first view. */
 typedef struct svo_batch svo_batch;
 int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_views);

@@ -337,7 +337,8 @@ struct TileArgs {
 // swizzled [9][COUT][32] weights
 template <int COUT, bool LAST>
 __device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out, float* outf, int wout,
-                                             const bf16* Wl, const float* __restrict__ bias, int wave, int lane)
+                                             const bf16* Wl, const float* __restrict__ bias, int wave, int lane,
+                                             int nw)
 {
     constexpr int NT = COUT / 16;
     const int npix = wout * wout;
@@ -346,7 +347,7 @@ __device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out,
     bf8v bfr[9][NT];
     for (int tap = 0; tap < 9; tap++)
         for (int nn = 0; nn < NT; nn++) bfr[tap][nn] = *(const bf8v*)(Wl + swz(tap * COUT + nn * 16 + row, kg));
-    for (int m0 = wave * 2; m0 * 16 < npix; m0 += 8) {
+    for (int m0 = wave * 2; m0 * 16 < npix; m0 += 2 * nw) {
         f4v acc[2][NT];
         int q0[2];
         for (int m = 0; m < 2; m++) {
@@ -380,24 +381,90 @@ __device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out,
     }
 }
 
-__global__ __launch_bounds__(256) void k_cnn_tiles(TileArgs a)
+// weights of the four layers as the tile kernel keeps them in LDS: layer 1
+// [9][32][16] plain, layers 2-4 [9][COUT][32] with swizzled rows
+constexpr int W_OFS1 = 9 * 32 * 16, W_OFS2 = W_OFS1 + 9 * 32 * 32, W_OFS3 = W_OFS2 + 9 * 32 * 32;
+constexpr int W_ALL = W_OFS3 + 9 * 16 * 32;
+
+__device__ __forceinline__ void stage_weights(bf16* dst, const bf16* __restrict__ src, int rows, bool swizzled,
+                                              int tid, int nthreads)
 {
-    __shared__ __align__(16) bf16 P[400 * 32];      // input 576 x 16 ch / layer-2 output 400 x 32 ch
-    __shared__ __align__(16) bf16 Q[484 * 32];      // layer-1 output 484 x 32 / layer-3 output 324 x 32
-    __shared__ __align__(16) bf16 Wl[9 * 32 * 32];  // the current layer's weights
-    __shared__ float OF[256 * 3];                   // layer-4 RGB of the centre 16x16
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (!swizzled) {
+        for (int i = tid; i < rows * 16 / 8; i += nthreads) *(uint4*)(dst + i * 8) = *(const uint4*)(src + i * 8);
+        return;
+    }
+    for (int i = tid; i < rows * 4; i += nthreads) {
+        const int r = i >> 2, g = i & 3;
+        *(uint4*)(dst + swz(r, g)) = *(const uint4*)(src + r * 32 + g * 8);
+    }
+}
+
+// layer 1: 16 -> 32 on 22x22 from the 24x24 plain 16-channel input
+__device__ __forceinline__ void tile_layer1(const bf16* P, bf16* Q, const bf16* Wl, const float* __restrict__ bias,
+                                            int wave, int lane, int nw)
+{
     const int row = lane & 15, kg = lane >> 4;
+    s4v b1[9][2];
+    for (int tap = 0; tap < 9; tap++)
+        for (int nn = 0; nn < 2; nn++) b1[tap][nn] = *(const s4v*)(Wl + (tap * 32 + nn * 16 + row) * 16 + kg * 4);
+    for (int m0 = wave * 2; m0 * 16 < 484; m0 += 2 * nw) {
+        f4v acc[2][2];
+        int q0[2];
+        for (int m = 0; m < 2; m++) {
+            acc[m][0] = acc[m][1] = f4v{0.f, 0.f, 0.f, 0.f};
+            const int p = min((m0 + m) * 16 + row, 483);
+            q0[m] = (p / 22) * 24 + p % 22;
+        }
+        for (int tap = 0; tap < 9; tap++) {
+            const int dq = (tap / 3) * 24 + tap % 3;
+            for (int m = 0; m < 2; m++) {
+                const s4v afr = *(const s4v*)(P + (q0[m] + dq) * 16 + kg * 4);
+                for (int nn = 0; nn < 2; nn++)
+                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, b1[tap][nn], acc[m][nn], 0, 0, 0);
+            }
+        }
+        for (int m = 0; m < 2; m++)
+            for (int nn = 0; nn < 2; nn++) {
+                const int co = nn * 16 + row;
+                const float bv = bias[co];
+                for (int i = 0; i < 4; i++) {
+                    const int q = (m0 + m) * 16 + kg * 4 + i;
+                    if (q >= 484) continue;
+                    const float v = acc[m][nn][i] + bv;
+                    Q[swz(q, co >> 3) + (co & 7)] = f2bf(v > 0.f ? v : 0.f);
+                }
+            }
+    }
+}
+
+// NW waves per workgroup.  RESIDENT: the four layers' weights are staged in
+// LDS once per workgroup (8 waves, one workgroup per CU); otherwise each
+// layer's weights are staged per tile (4 waves, two workgroups per CU).
+template <int NW, bool RESIDENT>
+__global__ __launch_bounds__(NW * 64) void k_cnn_tiles(TileArgs a)
+{
+    __shared__ __align__(16) bf16 P[400 * 32];  // input 576 x 16 ch / layer-2 output 400 x 32 ch / layer-4 RGB
+    __shared__ __align__(16) bf16 Q[484 * 32];  // layer-1 output 484 x 32 / layer-3 output 324 x 32
+    __shared__ __align__(16) bf16 Wl[RESIDENT ? W_ALL : 9 * 32 * 32];
+    float* OF = (float*)P;  // 256 x 3 floats, written by layer 4 after layer 3 has consumed P
+    constexpr int NT = NW * 64;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = a.x ? a.n : min((int)*a.count, a.max_tiles);
+    if (RESIDENT) {
+        stage_weights(Wl, a.w[0], 9 * 32, false, tid, NT);
+        stage_weights(Wl + W_OFS1, a.w[1], 9 * 32, true, tid, NT);
+        stage_weights(Wl + W_OFS2, a.w[2], 9 * 32, true, tid, NT);
+        stage_weights(Wl + W_OFS3, a.w[3], 9 * 16, true, tid, NT);
+    }
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
-        __syncthreads();  // the previous crop is done with every buffer
+        __syncthreads();  // the previous crop is done with every buffer (and the weights are staged)
         int ox = 0, oy = 0;
         if (!a.x) {
             const int t = (int)a.tiles[c];
             ox = (t % a.tiles_x) * 16 - 4;
             oy = (t / a.tiles_x) * 16 - 4;
         }
-        for (int q = tid; q < 576; q += 256) {
+        for (int q = tid; q < 576; q += NT) {
             uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
             if (a.x) {
                 const uint4* src = (const uint4*)(a.x + ((size_t)c * 576 + q) * 16);
@@ -426,71 +493,37 @@ __global__ __launch_bounds__(256) void k_cnn_tiles(TileArgs a)
             *(uint4*)(P + q * 16) = lo;
             *(uint4*)(P + q * 16 + 8) = hi;
         }
-        for (int i = tid; i < 9 * 32 * 16 / 8; i += 256) *(uint4*)(Wl + i * 8) = *(const uint4*)(a.w[0] + i * 8);
+        if (!RESIDENT) stage_weights(Wl, a.w[0], 9 * 32, false, tid, NT);
         __syncthreads();
-        // layer 1: 16 -> 32 on 22x22, input region 24x24 (plain 16-channel pixels)
-        s4v b1[9][2];
-        for (int tap = 0; tap < 9; tap++)
-            for (int nn = 0; nn < 2; nn++) b1[tap][nn] = *(const s4v*)(Wl + (tap * 32 + nn * 16 + row) * 16 + kg * 4);
-        for (int m0 = wave * 2; m0 * 16 < 484; m0 += 8) {
-            f4v acc[2][2];
-            int q0[2];
-            for (int m = 0; m < 2; m++) {
-                acc[m][0] = acc[m][1] = f4v{0.f, 0.f, 0.f, 0.f};
-                const int p = min((m0 + m) * 16 + row, 483);
-                q0[m] = (p / 22) * 24 + p % 22;
-            }
-            for (int tap = 0; tap < 9; tap++) {
-                const int dq = (tap / 3) * 24 + tap % 3;
-                for (int m = 0; m < 2; m++) {
-                    const s4v afr = *(const s4v*)(P + (q0[m] + dq) * 16 + kg * 4);
-                    for (int nn = 0; nn < 2; nn++)
-                        acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, b1[tap][nn], acc[m][nn], 0, 0, 0);
-                }
-            }
-            for (int m = 0; m < 2; m++)
-                for (int nn = 0; nn < 2; nn++) {
-                    const int co = nn * 16 + row;
-                    const float bv = a.b[0][co];
-                    for (int i = 0; i < 4; i++) {
-                        const int q = (m0 + m) * 16 + kg * 4 + i;
-                        if (q >= 484) continue;
-                        const float v = acc[m][nn][i] + bv;
-                        Q[swz(q, co >> 3) + (co & 7)] = f2bf(v > 0.f ? v : 0.f);
-                    }
-                }
+        tile_layer1(P, Q, Wl, a.b[0], wave, lane, NW);
+        __syncthreads();
+        if (!RESIDENT) {
+            stage_weights(Wl, a.w[1], 9 * 32, true, tid, NT);
+            __syncthreads();
         }
+        tile_layer32<32, false>(Q, 22, P, nullptr, 20, RESIDENT ? Wl + W_OFS1 : Wl, a.b[1], wave, lane, NW);
         __syncthreads();
-        for (int i = tid; i < 9 * 32 * 4; i += 256) {  // layer-2 weights, swizzled rows
-            const int r = i >> 2, g = i & 3;
-            *(uint4*)(Wl + swz(r, g)) = *(const uint4*)(a.w[1] + r * 32 + g * 8);
+        if (!RESIDENT) {
+            stage_weights(Wl, a.w[2], 9 * 32, true, tid, NT);
+            __syncthreads();
         }
+        tile_layer32<32, false>(P, 20, Q, nullptr, 18, RESIDENT ? Wl + W_OFS2 : Wl, a.b[2], wave, lane, NW);
         __syncthreads();
-        tile_layer32<32, false>(Q, 22, P, nullptr, 20, Wl, a.b[1], wave, lane);
-        __syncthreads();
-        for (int i = tid; i < 9 * 32 * 4; i += 256) {
-            const int r = i >> 2, g = i & 3;
-            *(uint4*)(Wl + swz(r, g)) = *(const uint4*)(a.w[2] + r * 32 + g * 8);
+        if (!RESIDENT) {
+            stage_weights(Wl, a.w[3], 9 * 16, true, tid, NT);
+            __syncthreads();
         }
+        tile_layer32<16, true>(Q, 18, nullptr, OF, 16, RESIDENT ? Wl + W_OFS3 : Wl, a.b[3], wave, lane, NW);
         __syncthreads();
-        tile_layer32<32, false>(P, 20, Q, nullptr, 18, Wl, a.b[2], wave, lane);
-        __syncthreads();
-        for (int i = tid; i < 9 * 16 * 4; i += 256) {
-            const int r = i >> 2, g = i & 3;
-            *(uint4*)(Wl + swz(r, g)) = *(const uint4*)(a.w[3] + r * 32 + g * 8);
-        }
-        __syncthreads();
-        tile_layer32<16, true>(Q, 18, nullptr, OF, 16, Wl, a.b[3], wave, lane);
-        __syncthreads();
-        {
-            const float r = OF[tid * 3], g = OF[tid * 3 + 1], b = OF[tid * 3 + 2];
+        for (int q = tid; q < 256; q += NT) {
+            const float r = OF[q * 3], g = OF[q * 3 + 1], b = OF[q * 3 + 2];
             if (a.x) {
-                float* o = a.out + ((size_t)c * 256 + tid) * 3;
+                float* o = a.out + ((size_t)c * 256 + q) * 3;
                 o[0] = r;
                 o[1] = g;
                 o[2] = b;
             } else {
-                const int x = ox + 4 + (tid & 15), y = oy + 4 + (tid >> 4);
+                const int x = ox + 4 + (q & 15), y = oy + 4 + (q >> 4);
                 if (x < a.res_x && y < a.res_y) {
                     const size_t o = (size_t)y * a.res_x + x;
                     if (a.words[o] == HOLE_WORD) {
@@ -569,10 +602,17 @@ struct svo_cnn {
     int2* origins = nullptr;
     int max_origins = 0;
     bool fused = true;     // per-tile fused inference kernel for the frame fill
+    int tile_mode = 1;     // fused kernel variant (see launch_tiles)
     bf16* xt = nullptr;    // explicit-mode tile inputs / outputs
     float* ot = nullptr;
     size_t cap_tiles = 0;
 };
+
+#define RCC(x)             \
+    do {                   \
+        int _r = (x);      \
+        if (_r) return _r; \
+    } while (0)
 
 #define HIPCHECK(expr)                                                                    \
     do {                                                                                  \
@@ -959,13 +999,25 @@ static TileArgs tile_args(svo_cnn* n)
     return a;
 }
 
-static unsigned tile_grid(int max_tiles, int device)
+static unsigned tile_grid(int max_tiles, int device, int per_cu)
 {
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         cus = prop.multiProcessorCount;
-    return (unsigned)std::max(1, std::min(max_tiles, 2 * cus));  // two resident workgroups per CU
+    return (unsigned)std::max(1, std::min(max_tiles, per_cu * cus));
+}
+
+// tile kernel variant: 1 = 8 waves, weights resident (one workgroup per CU);
+// 0 = 4 waves, weights staged per layer (two workgroups per CU)
+static int launch_tiles(const TileArgs& a, int mode, int max_tiles, int device, hipStream_t stream)
+{
+    if (mode == 1)
+        hipLaunchKernelGGL((k_cnn_tiles<8, true>), dim3(tile_grid(max_tiles, device, 1)), dim3(512), 0, stream, a);
+    else
+        hipLaunchKernelGGL((k_cnn_tiles<4, false>), dim3(tile_grid(max_tiles, device, 2)), dim3(256), 0, stream, a);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
 }
 
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
@@ -984,9 +1036,7 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
         a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
         a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
         a.image = c->d_image;
-        hipLaunchKernelGGL(k_cnn_tiles, dim3(tile_grid(max_tiles, c->device)), dim3(256), 0, stream, a);
-        HIPCHECK(hipGetLastError());
-        return SVO_OK;
+        return launch_tiles(a, n->tile_mode, max_tiles, c->device, stream);
     }
     int rc = ensure_ws(n, (size_t)max_tiles * 576);
     if (rc) return rc;
@@ -1029,16 +1079,17 @@ extern "C" int svo_cnn_forward_tiles(svo_cnn* n, const uint16_t* x, int count, f
     a.x = n->xt;
     a.out = n->ot;
     a.n = count;
-    hipLaunchKernelGGL(k_cnn_tiles, dim3(tile_grid(count, n->device)), dim3(256), 0, n->stream, a);
-    HIPCHECK(hipGetLastError());
+    RCC(launch_tiles(a, n->tile_mode, count, n->device, n->stream));
     HIPCHECK(hipMemcpyAsync(out, n->ot, (size_t)count * 256 * 3 * 4, hipMemcpyDeviceToHost, n->stream));
     HIPCHECK(hipStreamSynchronize(n->stream));
     return SVO_OK;
 }
 
-extern "C" int svo_cnn_set_fused(svo_cnn* n, int enable)
+extern "C" int svo_cnn_set_fused(svo_cnn* n, int mode)
 {
     if (!n) return svo_fail(SVO_ERR_ARG, "null network");
-    n->fused = enable != 0;
+    if (mode < 0 || mode > 2) return svo_fail(SVO_ERR_ARG, "fused mode %d outside 0..2", mode);
+    n->fused = mode != 0;
+    n->tile_mode = mode == 2 ? 0 : 1;
     return SVO_OK;
 }

@@ -66,7 +66,8 @@ def test_gradients_match_reference(cnn):
     assert np.all(g[:9 * 32 * 16].reshape(9, 32, 16)[:, :, 5:] == 0)
 
 
-def test_fused_tile_forward_matches_reference():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fused_tile_forward_matches_reference(mode):
     """The fused per-tile inference kernel (all four layers in LDS, layer 1 on
     16x16x16 MFMAs, layers 2-4 on 16x16x32 MFMAs) against the FP64 network on
     full 24x24 crops, compared on the centre 16x16 it returns."""
@@ -80,6 +81,7 @@ def test_fused_tile_forward_matches_reference():
             p[o:o + co] = rng.uniform(-0.2, 0.2, co) * (np.arange(co) < (3 if co == 16 else co))
             o += co
         c.set_params(p)
+        c.set_fused(mode=mode)
         bits, x = _batch(rng, 5, 24, 24)
         out = c.forward_tiles(bits)
         ref, _ = R.forward(c.params().astype(np.float64), x)

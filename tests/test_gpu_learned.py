@@ -51,7 +51,8 @@ def test_training_improves_fill():
     cnn.close()
 
 
-def test_fused_tile_fill_matches_per_layer_fill():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fused_tile_fill_matches_per_layer_fill(mode):
     """The frame fill with the fused tile kernel and with the per-layer
     kernels: identical screens, images equal outside CNN pixels and within
     bf16 rounding (a few 8-bit levels) on them."""
@@ -68,7 +69,7 @@ def test_fused_tile_fill_matches_per_layer_fill():
         n_cnn = 0
         for f in range(12):
             p = W.frame_params(NAME, rows, f)
-            cnn.set_fused(True)
+            cnn.set_fused(mode=mode)
             a.frame_warp_learned(cnn, p, 64)
             a.sync()
             cnn.set_fused(False)

@@ -277,10 +277,11 @@ int svo_cnn_train_step(svo_cnn* cnn, svo_ctx* warp, svo_ctx* full, const svo_fra
 /* The frame fill's inference kernel: one workgroup runs all four layers on
  * a 24x24 crop with activations in LDS and returns the centre 16x16 RGB.
  * x: [count][24][24][16] bf16 features; out: [count][16][16][3] float.
- * svo_cnn_set_fused(cnn, mode): 1 (default) the fused kernel with 8 waves
- * per tile and all weights resident in LDS; 2 the fused kernel with 4 waves
- * staging each layer's weights; 0 the per-layer kernels of svo_cnn_forward
- * (same network, bf16 rounding differences). */
+ * svo_cnn_set_fused(cnn, mode): 1 (default) the fused kernel with 4 waves
+ * per tile staging each layer's weights (two workgroups per CU); 2 the fused
+ * kernel with 8 waves and all weights resident in LDS (one workgroup per
+ * CU, about 10% slower on MI355X); 0 the per-layer kernels of
+ * svo_cnn_forward (same network, bf16 rounding differences). */
 int svo_cnn_forward_tiles(svo_cnn* cnn, const uint16_t* x, int count, float* out);
 int svo_cnn_set_fused(svo_cnn* cnn, int mode);
 /* svo_frame_warp with the learned fill and per-tile confidence gate.  After

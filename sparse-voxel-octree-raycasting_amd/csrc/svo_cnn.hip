@@ -602,7 +602,7 @@ struct svo_cnn {
     int2* origins = nullptr;
     int max_origins = 0;
     bool fused = true;     // per-tile fused inference kernel for the frame fill
-    int tile_mode = 1;     // fused kernel variant (see launch_tiles)
+    int tile_mode = 0;     // fused kernel variant (see launch_tiles)
     bf16* xt = nullptr;    // explicit-mode tile inputs / outputs
     float* ot = nullptr;
     size_t cap_tiles = 0;
@@ -1008,8 +1008,8 @@ static unsigned tile_grid(int max_tiles, int device, int per_cu)
     return (unsigned)std::max(1, std::min(max_tiles, per_cu * cus));
 }
 
-// tile kernel variant: 1 = 8 waves, weights resident (one workgroup per CU);
-// 0 = 4 waves, weights staged per layer (two workgroups per CU)
+// tile kernel variant: 0 = 4 waves, weights staged per layer (two workgroups
+// per CU); 1 = 8 waves, weights resident (one workgroup per CU)
 static int launch_tiles(const TileArgs& a, int mode, int max_tiles, int device, hipStream_t stream)
 {
     if (mode == 1)
@@ -1090,6 +1090,6 @@ extern "C" int svo_cnn_set_fused(svo_cnn* n, int mode)
     if (!n) return svo_fail(SVO_ERR_ARG, "null network");
     if (mode < 0 || mode > 2) return svo_fail(SVO_ERR_ARG, "fused mode %d outside 0..2", mode);
     n->fused = mode != 0;
-    n->tile_mode = mode == 2 ? 0 : 1;
+    n->tile_mode = mode == 2 ? 1 : 0;
     return SVO_OK;
 }

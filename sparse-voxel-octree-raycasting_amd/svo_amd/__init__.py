@@ -516,9 +516,9 @@ class Cnn:
         return out
 
     def set_fused(self, on=True, mode=None):
-        """Frame fill with the fused tile kernel (default; mode 1: 8 waves with
-        resident weights, mode 2: 4 waves staging weights) or, with on=False,
-        the per-layer kernels."""
+        """Frame fill with the fused tile kernel (default; mode 1: 4 waves
+        staging each layer's weights, mode 2: 8 waves with resident weights)
+        or, with on=False, the per-layer kernels."""
         _check(_f["cnn_set_fused"](self._h, int(mode) if mode is not None else (1 if on else 0)))
 
     def train_step(self, warp, full, p, n_crops=16, crop=64, seed=0, lr=1e-3, want_loss=False):

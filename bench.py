@@ -76,7 +76,7 @@ def run_gpu(args, name):
     res = {"octree_build": build}
     if args.tune:
         from svo_amd import autotune as A
-        tp = [W.frame_params(name, rows, f) for f in range(min(total, 40))]
+        tp = [W.frame_params(name, rows, f) for f in range(min(total, 70))]
         for c, mode in ((warp, "warp"), (full, "full")):
             best, ms, _ = A.tune(c, tp, warmup=10, mode=mode)
             res[f"tuned_{mode}"] = dict(best, ms=round(ms, 4))

@@ -1,8 +1,9 @@
 """Autotuner for the ray kernel's launch knobs.
 
 Knobs: ray hand-out order (svo_set_ray_spread: the size of the ray groups
-spread across waves, 64 = list order), lanes per wave that take rays
-(svo_set_ray_lanes) and, through svo_set_tuning, the work-queue refill threshold (idle
+spread across waves, 64 = list order; lanes per wave, svo_set_ray_lanes, is
+applied when a setting names it but is not searched by default: fewer lanes
+helped short windows and hurt long ones) and, through svo_set_tuning, the work-queue refill threshold (idle
 lanes per refill), persistent workgroups per CU, traversal steps between
 refill checks, and the LDS node-cache depth (octree words of the
 breadth-first top levels kept in LDS).  Every candidate replays the same frame window from the same starting
@@ -20,7 +21,6 @@ import time
 
 DEFAULT_SPACE = {
     "spread": [1, 64],
-    "lanes": [64, 32, 16],
     "refill": [8, 16, 32, 48],
     "wg_per_cu": [4, 8, 16],
     "iters": [8, 16, 64],

@@ -111,6 +111,15 @@ int svo_raycast_holes(svo_ctx* ctx, const svo_frame_params* p);
  * i < ext_x, j < ext_y, clipped to the screen. */
 int svo_raycast_rect(svo_ctx* ctx, const svo_frame_params* p, int add_x,
                      int add_y, int ext_x, int ext_y);
+/* raycast_fine (kernel.cl:696-843; launched by raycast.h:236-259 under an
+ * if(0)): one ray per 2x2 quad at (2 gx + add_x, 2 gy + add_y) over the
+ * roundup16(res/4) grid; the quad's first hole pixel is traced, else pixel
+ * ((frame>>2)&1, (frame>>3)&1) of the quad.  The reference host used
+ * add_x = (res_x/2)*(frame&1), add_y = (res_y/2)*((frame>>1)&1).  Picks use
+ * flat offsets, so quads on the last column/row may read and write the first
+ * pixel of the next row or of plane 1, as in the reference; all picks are made
+ * before any ray writes.  Uses the learned-fill gate lists as scratch. */
+int svo_raycast_quads(svo_ctx* ctx, const svo_frame_params* p, int add_x, int add_y);
 /* ocl_memcpy plane 0 -> dst_plane of screen and coords (raycast.h:394-405) */
 int svo_copy_cache(svo_ctx* ctx, int dst_plane);
 /* raycast_fillhole2 (kernel.cl:404-470), defined on a pre-filter snapshot of

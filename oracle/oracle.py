@@ -75,6 +75,7 @@ _sum = _sig("orc_sumids", None, _S)
 _write = _sig("orc_writeids", None, _S)
 _holes = _sig("orc_raycast_holes", None, _S, _P)
 _rect = _sig("orc_raycast_rect", None, _S, _P, C.c_int, C.c_int, C.c_int, C.c_int)
+_quads = _sig("orc_raycast_quads", None, _S, _P, C.c_int, C.c_int)
 _copy = _sig("orc_copy_plane", None, _S, C.c_int)
 _fill_snap = _sig("orc_fillhole2_snapshot", None, _S, _u32p)
 _fill_inpl = _sig("orc_fillhole2_inplace", None, _S)
@@ -228,6 +229,9 @@ class Frame:
 
     def raycast_rect(self, p, ax, ay, ex, ey):
         _rect(C.byref(self.s), C.byref(p), ax, ay, ex, ey)
+
+    def raycast_quads(self, p, ax, ay):
+        _quads(C.byref(self.s), C.byref(p), ax, ay)
 
     def copy_plane(self, dst):
         _copy(C.byref(self.s), dst)

@@ -674,6 +674,63 @@ void orc_raycast_rect(orc_state* s, const orc_params* p, int add_x, int add_y,
     add_stats(&s->st, &tot);
 }
 
+/* raycast_fine (kernel.cl:696-843; its launch at raycast.h:236-259 is
+ * disabled): one ray per 2x2 quad at (2 gx + add_x, 2 gy + add_y) for gx <
+ * roundup16(res_x/4), gy < roundup16(res_y/4) (ocl.h:188-198 global-size
+ * rounding), skipped when that corner is off the screen (:722).  The traced
+ * pixel is the quad's first hole in the order (0,0), (1,0), (0,1), (1,1)
+ * (:726-737), else ((frame>>2)&1, (frame>>3)&1) (:738-742); the reads and the
+ * write use flat offsets y*res_x + x (:727, :804), so at the last column or
+ * row they reach the next row or plane 1.  All picks are made before any ray
+ * writes (defined order; the reference's work-items race only at those
+ * edges). */
+void orc_raycast_quads(orc_state* s, const orc_params* p, int add_x, int add_y)
+{
+    const int res_x = s->res_x, res_y = s->res_y;
+    const int gx_n = ((res_x / 4 + 15) / 16) * 16, gy_n = ((res_y / 4 + 15) / 16) * 16;
+    const long nq = (long)gx_n * gy_n;
+    int* pick = (int*)malloc(sizeof(int) * 2 * (size_t)nq);
+    for (long g = 0; g < nq; g++) {
+        int idx = 2 * (int)(g % gx_n) + add_x, idy = 2 * (int)(g / gx_n) + add_y;
+        if (idx >= res_x || idy >= res_y) {
+            pick[2 * g] = -1;
+            continue;
+        }
+        int found = 0;
+        for (int i = 0; i < 4 && !found; i++) {
+            size_t o = (size_t)(idy + (i >> 1)) * res_x + idx + (i & 1);
+            if (s->screen[o] == SCREEN_HOLE) {
+                idx += i & 1;
+                idy += i >> 1;
+                found = 1;
+            }
+        }
+        if (!found) {
+            idx += (p->frame >> 2) & 1;
+            idy += (p->frame >> 3) & 1;
+        }
+        pick[2 * g] = idx;
+        pick[2 * g + 1] = idy;
+    }
+    orc_stats tot; memset(&tot, 0, sizeof(tot));
+#pragma omp parallel num_threads(s->nthreads)
+    {
+        orc_stats loc; memset(&loc, 0, sizeof(loc));
+#pragma omp for schedule(dynamic, 64)
+        for (long g = 0; g < nq; g++) {
+            if (pick[2 * g] < 0) continue;
+            int idx = pick[2 * g], idy = pick[2 * g + 1];
+            float xyz[3];
+            uint32_t w = orc_trace_pixel(s->oct, s->n_words, s->root, s->depth, res_x, res_y, p, idx, idy, xyz, &loc);
+            write_ray(s, (size_t)idy * res_x + idx, w, xyz);
+        }
+#pragma omp critical
+        add_stats(&tot, &loc);
+    }
+    add_stats(&s->st, &tot);
+    free(pick);
+}
+
 /* raycast.h:394-405 (ocl_memcpy of the screen words then the float4 coords) */
 void orc_copy_plane(orc_state* s, int dst_plane)
 {

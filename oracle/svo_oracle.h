@@ -108,6 +108,7 @@ void orc_fillhole2_snapshot(orc_state* s, const uint32_t* snapshot); /* kernel.c
 void orc_fillhole2_inplace(orc_state* s);                         /* serial 16x16-group order */
 void orc_colorize(orc_state* s);                                  /* kernel.cl:944-974 */
 
+void orc_raycast_quads(orc_state* s, const orc_params* p, int add_x, int add_y); /* kernel.cl:696-843 */
 void orc_tile_schedule(int res_x, int res_y, int frame, int* add_x, int* add_y,
                        int* ext_x, int* ext_y);                   /* raycast.h:363-366 */
 

@@ -131,12 +131,15 @@ struct View {
     int add_x, add_y, ext_x, ext_y, tiles_x;  // refresh rectangle
     uint32_t n_rect;
     uint32_t* ray_image;                   // colorize ray words straight into this, or null
+    int edge_ok;  // list pixels may be (res_x, y) or (x, res_y) (quad rays)
     // learned-fill gate of this view: counters, CNN tile list, gate pixel list
     uint32_t* gate;
     uint32_t* gate_tiles;
     uint32_t* gate_rays;
     int gate_tiles_x, gate_n_tiles;
 };
+
+constexpr uint32_t QUAD_NONE = 0xffffffffu;  // list entry without a ray
 
 #define VIEW_OF(idx) const View& V = views ? views[(idx)] : v
 
@@ -216,6 +219,7 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
     int x, y;
     if (k < H + H2) {
         const uint32_t w = (k < H) ? V.list[k] : V.list2[k - H];
+        if (w == QUAD_NONE) return false;
         x = (int)(w & 0xffffu);
         y = (int)(w >> 16);
     } else {
@@ -227,7 +231,9 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
         x = V.add_x + lx;
         y = V.add_y + ly;
     }
-    if (x >= V.res_x || y >= V.res_y) return false;
+    // quad rays may pick the pixel one past the row or frame (their flat
+    // offset then lies in the next row or in plane 1, inside the buffers)
+    if (!V.edge_ok && (x >= V.res_x || y >= V.res_y)) return false;
     r.pix = (uint32_t)(y * V.res_x + x);
     r.view = vi;
 
@@ -699,6 +705,41 @@ __global__ __launch_bounds__(256) void k_fill_color(View v, const View* views, c
 }
 
 // ---------------------------------------------------------------- learned-fill gate
+// raycast_fine (kernel.cl:696-843, disabled in raycast.h:236): one ray per
+// 2x2 quad at (2 gx + add_x, 2 gy + add_y) over the rounded-up (res/4) grid
+// (ocl.h:188-198).  The quad's first hole pixel in the order (0,0), (1,0),
+// (0,1), (1,1) is traced; without one, pixel ((frame>>2)&1, (frame>>3)&1).
+// The reads are flat offsets, so a quad on the last column or row reads and
+// may pick the first pixel of the next row or of plane 1 (kernel.cl:727-736).
+// The pick is made for all quads before any ray writes (the reference's
+// threads read only their own quad except at those edges).
+__global__ __launch_bounds__(256) void k_quad_select(View v, int add_x, int add_y, int frame, int gx_n, int gy_n,
+                                                     uint32_t* list, uint32_t* count)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g == 0) *count = (uint32_t)(gx_n * gy_n);
+    if (g >= gx_n * gy_n) return;
+    int idx = 2 * (g % gx_n) + add_x, idy = 2 * (g / gx_n) + add_y;
+    if (idx >= v.res_x || idy >= v.res_y) {
+        list[g] = QUAD_NONE;
+        return;
+    }
+    bool found = false;
+    for (int i = 0; i < 4 && !found; i++) {
+        const size_t o = (size_t)(idy + (i >> 1)) * v.res_x + idx + (i & 1);
+        if (v.screen[o] == HOLE) {
+            idx += i & 1;
+            idy += i >> 1;
+            found = true;
+        }
+    }
+    if (!found) {
+        idx += (frame >> 2) & 1;
+        idy += (frame >> 3) & 1;
+    }
+    list[g] = (uint32_t)idx | ((uint32_t)idy << 16);
+}
+
 // gate counters: [0] CNN tiles, [GATE_RAYS] gate rays (a separate 64-byte line)
 constexpr int GATE_RAYS = 16;
 __global__ void k_gate_reset(View v, const View* views)
@@ -1366,6 +1407,23 @@ extern "C" int svo_raycast_rect(svo_ctx* c, const svo_frame_params* p, int add_x
     HIPCHECK(hipSetDevice(c->device));
     return launch_rays(c, p, nullptr, nullptr, add_x, add_y, ext_x, ext_y, nullptr);
 }
+extern "C" int svo_raycast_quads(svo_ctx* c, const svo_frame_params* p, int add_x, int add_y)
+{
+    RC(need_octree(c));
+    if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
+    if (add_x < 0 || add_y < 0 || add_x > 65535 || add_y > 65535) return svo_fail(SVO_ERR_ARG, "bad quad offset");
+    const int gx_n = ((c->res_x / 4 + 15) / 16) * 16, gy_n = ((c->res_y / 4 + 15) / 16) * 16;
+    if ((size_t)gx_n * gy_n > c->N) return svo_fail(SVO_ERR_ARG, "quad grid larger than the frame");
+    HIPCHECK(hipSetDevice(c->device));
+    View v = make_view(c, p);
+    hipLaunchKernelGGL(k_quad_select, dim3((unsigned)((gx_n * gy_n + 255) / 256)), dim3(256), 0, c->stream, v, add_x,
+                       add_y, p->frame, gx_n, gy_n, c->d_gate_rays, c->d_gate + GATE_RAYS);
+    HIPCHECK(hipGetLastError());
+    set_rays(v, c->d_gate_rays, c->d_gate + GATE_RAYS, 0, 0, 0, 0, nullptr);
+    v.edge_ok = 1;
+    return launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq);
+}
+
 extern "C" int svo_copy_cache(svo_ctx* c, int dst_plane)
 {
     RC(need_frame(c));

@@ -78,6 +78,7 @@ _f = {
     "raycast_rect": _sig("svo_raycast_rect", C.c_int, _ctxp, C.POINTER(FrameParams),
                          C.c_int, C.c_int, C.c_int, C.c_int),
     "copy_cache": _sig("svo_copy_cache", C.c_int, _ctxp, C.c_int),
+    "raycast_quads": _sig("svo_raycast_quads", C.c_int, _ctxp, C.POINTER(FrameParams), C.c_int, C.c_int),
     "fillhole": _sig("svo_fillhole", C.c_int, _ctxp),
     "colorize": _sig("svo_colorize", C.c_int, _ctxp),
     "read": _sig("svo_read", C.c_int, _ctxp, C.c_int, C.c_size_t, C.c_size_t, _vp),
@@ -153,7 +154,7 @@ EXPORTED = [
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
-    "svo_set_ray_lanes",
+    "svo_set_ray_lanes", "svo_raycast_quads",
 ]
 
 
@@ -381,6 +382,10 @@ class Context:
 
     def raycast_rect(self, p, add_x, add_y, ext_x, ext_y):
         _check(_f["raycast_rect"](self._h, C.byref(p), int(add_x), int(add_y), int(ext_x), int(ext_y)))
+
+    def raycast_quads(self, p, add_x, add_y):
+        """raycast_fine: one ray per 2x2 quad of the (res/4)-rounded grid."""
+        _check(_f["raycast_quads"](self._h, C.byref(p), int(add_x), int(add_y)))
 
     def copy_cache(self, dst_plane):
         _check(_f["copy_cache"](self._h, int(dst_plane)))

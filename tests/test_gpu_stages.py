@@ -170,3 +170,27 @@ def test_error_paths(ctx):
         ctx.proj(0, S.FrameParams())
     with pytest.raises(S.SvoError):
         ctx.set_tuning(refill=65)
+
+
+@pytest.mark.parametrize("res,add,frame", [((96, 64), (0, 0), 5), ((98, 66), (49, 33), 12), ((200, 136), (100, 0), 8)])
+def test_quad_raycast_vs_oracle(ctx, scene1, res, add, frame):
+    """svo_raycast_quads (raycast_fine, kernel.cl:696-843) on a random
+    hole/non-hole screen: all four planes and coordinates bit-exact against
+    the oracle, odd offsets included (picks past the row or frame)."""
+    words, root = scene1
+    rx, ry = res
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    p, _ = S.camera(0.7, 0.3, 0.0, (4.0, 30.0, 9.0), 8, frame)
+    rng = np.random.default_rng(rx + frame)
+    before = np.where(rng.random(4 * rx * ry) < 0.4, 0xFFFFFF00, 0x12345600).astype(np.uint32)
+    ctx.write(S.BUF_SCREEN, 0, before)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+    o.screen.reshape(-1)[:] = before
+    print(f"quads {res} add {add}", flush=True)
+    ctx.raycast_quads(p, *add)
+    ctx.sync()
+    o.raycast_quads(O.params_from(p), *add)
+    assert_equal_u32(ctx.screen(), o.screen, "screen planes")
+    assert_equal_f32_bits(ctx.coords()[..., :3], o.coords[..., :3], "coords")
+    o.close()

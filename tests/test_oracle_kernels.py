@@ -251,3 +251,46 @@ def test_proj_first_smallest_wins_random():
             else:
                 assert out[t] == HOLE
         f.close()
+
+
+@pytest.mark.parametrize("res,add,frame", [((96, 64), (0, 0), 5), ((98, 66), (49, 33), 12), ((96, 64), (48, 32), 8)])
+def test_quad_raycast_picks_and_flat_writes(res, add, frame):
+    """raycast_fine (kernel.cl:696-843): the traced pixel of each quad is its
+    first hole in the order (0,0), (1,0), (0,1), (1,1), else the frame's
+    fallback corner; the ray of pixel (x, y) lands at flat offset y*res_x + x,
+    also when x == res_x or y == res_y (odd offsets reach the next row and
+    plane 1).  Checked by brute force against single-ray traces."""
+    xyz, rgba = S.scene(1, 1001, 8)
+    words, root = S.build_octree(xyz, rgba, 8)
+    rx, ry = res
+    ax, ay = add
+    p, _ = O.camera(0.7, 0.3, 0.0, (4.0, 30.0, 9.0), 8, frame)
+    f = O.Frame(rx, ry, words, root, 8, nthreads=4)
+    rng = np.random.default_rng(rx + frame)
+    before = np.where(rng.random(4 * rx * ry) < 0.4, HOLE, 0x12345600).astype(np.uint32)
+    f.screen.reshape(-1)[:] = before
+    f.raycast_quads(p, ax, ay)
+    after = f.screen.reshape(-1).copy()
+    gx_n, gy_n = ((rx // 4 + 15) // 16) * 16, ((ry // 4 + 15) // 16) * 16
+    picks = {}
+    for gy in range(gy_n):
+        for gx in range(gx_n):
+            x, y = 2 * gx + ax, 2 * gy + ay
+            if x >= rx or y >= ry:
+                continue
+            for dx, dy in ((0, 0), (1, 0), (0, 1), (1, 1)):
+                if before[(y + dy) * rx + x + dx] == HOLE:
+                    x, y = x + dx, y + dy
+                    break
+            else:
+                x, y = x + ((frame >> 2) & 1), y + ((frame >> 3) & 1)
+            picks[y * rx + x] = (x, y)
+    changed = set(np.nonzero(after != before)[0].tolist())
+    assert changed == set(picks)
+    assert any(x == rx or y == ry for x, y in picks.values()) == (ax % 2 == 1)
+    crd = f.coords.reshape(-1, 4)
+    for ofs, (x, y) in list(picks.items())[::7]:
+        w, pos, _ = O.trace_pixel(words, root, 8, rx, ry, p, x, y)
+        assert int(after[ofs]) == w
+        assert np.array_equal(crd[ofs, :3].view(np.uint32), np.array(pos, np.float32).view(np.uint32))
+    f.close()

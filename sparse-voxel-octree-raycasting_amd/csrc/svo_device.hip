@@ -1452,6 +1452,8 @@ extern "C" int svo_colorize(svo_ctx* c)
 // persistent ray launch, so the long-ray tails of the views overlap.
 struct svo_batch {
     std::vector<svo_ctx*> views;
+    std::vector<size_t> view_n;  // each view's pixel count and octree at creation
+    std::vector<uint32_t*> view_oct;
     int device = 0;
     hipStream_t stream = nullptr;
     View* d_views = nullptr;     // 2 slots x n
@@ -1501,6 +1503,10 @@ extern "C" int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_vi
     b->device = v0->device;
     b->views.assign(views, views + n_views);
     for (auto* c : b->views) {
+        b->view_n.push_back(c->N);
+        b->view_oct.push_back(c->d_oct);
+    }
+    for (auto* c : b->views) {
         b->maxN = std::max(b->maxN, (uint32_t)c->N);
         b->maxS = std::max(b->maxS, (uint32_t)c->S);
     }
@@ -1546,6 +1552,9 @@ extern "C" int svo_batch_frame_warp_learned(svo_batch* b, svo_cnn* cnn, const sv
 static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cnn, int max_cnn_holes, bool learned)
 {
     if (!b || !params) return svo_fail(SVO_ERR_ARG, "null argument");
+    for (size_t i = 0; i < b->views.size(); i++)
+        if (b->views[i]->N != b->view_n[i] || b->views[i]->d_oct != b->view_oct[i] || !b->views[i]->d_screen)
+            return svo_fail(SVO_ERR_STATE, "view %zu was resized or given a new octree after svo_batch_create", i);
     const int n = (int)b->views.size();
     svo_ctx* c0 = b->views[0];
     HIPCHECK(hipSetDevice(b->device));

@@ -141,3 +141,20 @@ def test_batch_learned_matches_single_views():
         for c in batched + single:
             c.close()
         cnn.close()
+
+
+def test_batch_refuses_a_resized_view():
+    words, root = scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
+    views = make_views(words, root, [(128, 64), (64, 64)])
+    batch = S.Batch(views)
+    try:
+        pr = path(CONFIG1, 4)
+        batch.frame_warp([view_params(pr, v, 0) for v in range(2)])
+        batch.sync()
+        views[1].resize(96, 64)
+        with pytest.raises(S.SvoError):
+            batch.frame_warp([view_params(pr, v, 1) for v in range(2)])
+    finally:
+        batch.close()
+        for c in views:
+            c.close()

@@ -1454,6 +1454,8 @@ struct svo_batch {
     std::vector<svo_ctx*> views;
     std::vector<size_t> view_n;  // each view's pixel count and octree at creation
     std::vector<uint32_t*> view_oct;
+    size_t n_words = 0;
+    uint32_t root = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     View* d_views = nullptr;     // 2 slots x n
@@ -1506,6 +1508,8 @@ extern "C" int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_vi
         b->view_n.push_back(c->N);
         b->view_oct.push_back(c->d_oct);
     }
+    b->n_words = v0->n_words;
+    b->root = v0->root;
     for (auto* c : b->views) {
         b->maxN = std::max(b->maxN, (uint32_t)c->N);
         b->maxS = std::max(b->maxS, (uint32_t)c->S);
@@ -1553,7 +1557,8 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
 {
     if (!b || !params) return svo_fail(SVO_ERR_ARG, "null argument");
     for (size_t i = 0; i < b->views.size(); i++)
-        if (b->views[i]->N != b->view_n[i] || b->views[i]->d_oct != b->view_oct[i] || !b->views[i]->d_screen)
+        if (b->views[i]->N != b->view_n[i] || b->views[i]->d_oct != b->view_oct[i] || !b->views[i]->d_screen ||
+            b->views[i]->n_words != b->n_words || b->views[i]->root != b->root)
             return svo_fail(SVO_ERR_STATE, "view %zu was resized or given a new octree after svo_batch_create", i);
     const int n = (int)b->views.size();
     svo_ctx* c0 = b->views[0];

@@ -253,6 +253,15 @@ int svo_set_ray_spread(svo_ctx* ctx, int group, int only_if_fits);
  * then places the rays of a wave 1/lanes of the list apart.  Results do not
  * depend on it. */
 int svo_set_ray_lanes(svo_ctx* ctx, int lanes);
+/* Full frames (svo_frame_full): the 8x8 screen blocks in which the previous
+ * full frame had a ray of at least long_trips traversal trips are raycast
+ * first (as a pixel list ahead of the raster-order blocks), so the long
+ * horizon rays overlap the rest of the frame; 0 = raster order only.  The
+ * first full frame after a resize, an upload or a change of this setting
+ * only collects the flags.  The reference launches raycast_fine_2 as a fixed
+ * 16x16-workgroup grid over its rectangle (raycast.h:361-366), so no
+ * counterpart.  Default 1024; results do not depend on it. */
+int svo_set_long_first(svo_ctx* ctx, int long_trips);
 
 /* ---- learned hole filler (no counterpart in the reference) ------------
  * A 4-layer 3x3 CNN (5 -> 32 -> 32 -> 32 -> 3 channels, ReLU) run with BF16

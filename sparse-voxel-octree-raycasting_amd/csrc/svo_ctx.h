@@ -45,6 +45,14 @@ struct svo_ctx {
     int spread_shift = 0;    // ray hand-out: log2 of the spread group (6 = list order)
     int spread_fit = 1;      // spread only launches whose rays all fit in the resident lanes
     int lanes_shift = 6;     // log2 of the lanes per wave that take rays
+    // full frames: 8x8 blocks whose previous full frame had a ray of at least
+    // long_trips trips are raycast first (0 = raster order only)
+    int long_trips = 1024;
+    uint32_t* d_block_long = nullptr;  // two flag arrays (previous / current frame)
+    uint32_t* d_long_list = nullptr;   // 64 pixel words per long block
+    uint32_t* d_long_count = nullptr;
+    int long_parity = 0;
+    bool long_valid = false;
     // per-frame graph of the warp frame (frames >= 2): its kernels read the
     // frame's view from d_view, refreshed from a pinned 2-slot staging ring
     bool use_graph = true;

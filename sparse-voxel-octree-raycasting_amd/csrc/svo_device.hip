@@ -132,6 +132,9 @@ struct View {
     uint32_t n_rect;
     uint32_t* ray_image;                   // colorize ray words straight into this, or null
     int edge_ok;  // list pixels may be (res_x, y) or (x, res_y) (quad rays)
+    const uint32_t* rect_skip;   // per 8x8 block: nonzero = raycast from the list instead, or null
+    uint32_t* block_long;        // per 8x8 block: set to 1 by a ray of >= long_trips trips, or null
+    uint32_t long_trips;
     // learned-fill gate of this view: counters, CNN tile list, gate pixel list
     uint32_t* gate;
     uint32_t* gate_tiles;
@@ -140,6 +143,7 @@ struct View {
 };
 
 constexpr uint32_t QUAD_NONE = 0xffffffffu;  // list entry without a ray
+constexpr uint32_t NO_BLOCK = 0xffffffffu;
 
 #define VIEW_OF(idx) const View& V = views ? views[(idx)] : v
 
@@ -175,6 +179,7 @@ struct Ray {
     uint32_t pix;
     uint32_t guard;
     int view;
+    uint32_t blk;  // 8x8 screen block of the ray (long-ray flags of full frames), or NO_BLOCK
 };
 
 __device__ __forceinline__ uint32_t colorize_word(uint32_t word)
@@ -222,9 +227,12 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
         if (w == QUAD_NONE) return false;
         x = (int)(w & 0xffffu);
         y = (int)(w >> 16);
+        r.blk = V.block_long ? (uint32_t)((y >> 3) * V.tiles_x + (x >> 3)) : NO_BLOCK;
     } else {
         k -= H + H2;
-        uint32_t t = k >> 6, w = k & 63u;
+        const uint32_t t = k >> 6, w = k & 63u;
+        if (V.rect_skip && V.rect_skip[t]) return false;
+        r.blk = t;
         int tx = (int)(t % (uint32_t)V.tiles_x), ty = (int)(t / (uint32_t)V.tiles_x);
         int lx = tx * 8 + (int)(w & 7u), ly = ty * 8 + (int)(w >> 3);
         if (lx >= V.ext_x || ly >= V.ext_y) return false;
@@ -392,6 +400,7 @@ __device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, OctBuf ob, 
     c[1] = py / 16.0f;
     c[2] = pz / 16.0f;
     if (V.ray_image) V.ray_image[r.pix] = colorize_word(word);
+    if (V.block_long && r.blk != NO_BLOCK && r.guard + 1u >= V.long_trips) V.block_long[r.blk] = 1u;
 }
 
 // lanes 0, 64/K, 2*64/K, ... of a wave (K = 2^ks lanes)
@@ -740,6 +749,47 @@ __global__ __launch_bounds__(256) void k_quad_select(View v, int add_x, int add_
     list[g] = (uint32_t)idx | ((uint32_t)idy << 16);
 }
 
+// Long blocks of a full frame go first.  An 8x8 block whose previous full
+// frame had a ray of at least long_trips trips (flag in `prev`) has its
+// on-screen pixels written to the front list (count in pixels, at most the
+// screen) and the rectangle pass skips it; the flags for this frame (`cur`)
+// are cleared.  In a flythrough such blocks sit at the horizon, half-way
+// through raster order, and their rays alone take a large part of the frame;
+// started first, they overlap the rest.  Rays are independent, so the
+// results do not depend on the order.  count[0] is this frame's list length
+// (zero on entry); count[1], the next frame's, is cleared here.
+__global__ __launch_bounds__(256) void k_long_blocks(const uint32_t* __restrict__ prev, uint32_t* __restrict__ cur,
+                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                                                     uint32_t* __restrict__ next_count, uint32_t n, uint32_t tiles_x,
+                                                     uint32_t res_x, uint32_t res_y)
+{
+    __shared__ uint32_t s_wave[4], s_base;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t i = blockIdx.x * 256u + (uint32_t)tid;
+    if (i == 0) *next_count = 0u;
+    const bool hv = i < n && prev && prev[i] != 0u;
+    if (i < n) cur[i] = 0u;
+    const uint32_t x0 = (i % tiles_x) * 8u, y0 = (i / tiles_x) * 8u;
+    const uint32_t w = hv ? min(8u, res_x - x0) : 0u, h = hv ? min(8u, res_y - y0) : 0u;
+    const uint32_t mine = w * h;
+    uint32_t incl = mine;  // inclusive scan of pixel counts over the wave
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_wave[wv] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        s_base = tot ? atomicAdd(count, tot) : 0u;  // one atomic per workgroup
+    }
+    __syncthreads();
+    if (!hv) return;
+    uint32_t base = s_base + incl - mine;
+    for (int k = 0; k < wv; k++) base += s_wave[k];
+    for (uint32_t j = 0; j < mine; j++) list[base + j] = (x0 + j % w) | ((y0 + j / w) << 16);
+}
+
 // gate counters: [0] CNN tiles, [GATE_RAYS] gate rays (a separate 64-byte line)
 constexpr int GATE_RAYS = 16;
 __global__ void k_gate_reset(View v, const View* views)
@@ -859,6 +909,11 @@ static int free_frame(svo_ctx* c)
     (void)hipFree(c->d_gate_tiles);
     (void)hipFree(c->d_gate_rays);
     c->d_gate = c->d_gate_tiles = c->d_gate_rays = nullptr;
+    (void)hipFree(c->d_block_long);
+    (void)hipFree(c->d_long_list);
+    (void)hipFree(c->d_long_count);
+    c->d_block_long = c->d_long_list = c->d_long_count = nullptr;
+    c->long_valid = false;
     c->res_x = c->res_y = 0;
     c->N = c->S = 0;
     return SVO_OK;
@@ -932,6 +987,7 @@ extern "C" int svo_upload_octree(svo_ctx* c, const uint32_t* words, size_t n_wor
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
     drop_graph(c);
+    c->long_valid = false;  // the next full frame only collects long-ray flags
     (void)hipFree(c->d_oct);
     c->d_oct = nullptr;
     HIPCHECK(hipMalloc(&c->d_oct, (n_words + kOctPad) * 4));
@@ -960,7 +1016,10 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
         hipMalloc(&c->d_keys, N * 8) != hipSuccess || hipMalloc(&c->d_snap, N * 4) != hipSuccess ||
         hipMalloc(&c->d_gate, 128) != hipSuccess ||
         hipMalloc(&c->d_gate_tiles, ((size_t)(res_x + 15) / 16) * ((res_y + 15) / 16) * 4) != hipSuccess ||
-        hipMalloc(&c->d_gate_rays, N * 4) != hipSuccess) {
+        hipMalloc(&c->d_gate_rays, N * 4) != hipSuccess ||
+        hipMalloc(&c->d_block_long, ((size_t)(res_x + 7) / 8) * ((res_y + 7) / 8) * 8) != hipSuccess ||
+        hipMalloc(&c->d_long_list, N * 4) != hipSuccess ||
+        hipMalloc(&c->d_long_count, 512) != hipSuccess) {
         free_frame(c);
         return svo_fail(SVO_ERR_NOMEM, "frame buffers for %dx%d", res_x, res_y);
     }
@@ -971,6 +1030,7 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
     HIPCHECK(hipMemset(c->d_keys, 0xff, N * 8));
     HIPCHECK(hipMemset(c->d_snap, 0, N * 4));
     HIPCHECK(hipMemset(c->d_gate, 0, 128));
+    HIPCHECK(hipMemset(c->d_long_count, 0, 512));
     c->n_tiles_x = (res_x + 15) / 16;
     c->n_tiles_y = (res_y + 15) / 16;
     HIPCHECK(hipDeviceSynchronize());
@@ -1344,9 +1404,41 @@ extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     HIPCHECK(hipSetDevice(c->device));
     frame_begin(c);
+    View v = make_view(c, p);
+    set_rays(v, nullptr, nullptr, 0, 0, c->res_x, c->res_y, c->d_image);
     mark(c, ST_RAYS);
-    RC(launch_rays(c, p, nullptr, nullptr, 0, 0, c->res_x, c->res_y, c->d_image));
+    if (c->long_trips) {
+        // the first full frame after a resize, upload or setting change only collects flags
+        const uint32_t nb = (uint32_t)(((c->res_x + 7) / 8) * ((c->res_y + 7) / 8));
+        uint32_t* cur = c->d_block_long + (size_t)c->long_parity * nb;
+        const uint32_t* prev = c->long_valid ? c->d_block_long + (size_t)(c->long_parity ^ 1) * nb : nullptr;
+        uint32_t* count = c->d_long_count + 64 * c->long_parity;  // zero: cleared by the previous full frame
+        uint32_t* next_count = c->d_long_count + 64 * (c->long_parity ^ 1);
+        hipLaunchKernelGGL(k_long_blocks, dim3((nb + 255) / 256), dim3(256), 0, c->stream, prev, cur, c->d_long_list,
+                           count, next_count, nb, (uint32_t)v.tiles_x, (uint32_t)c->res_x, (uint32_t)c->res_y);
+        HIPCHECK(hipGetLastError());
+        v.list = c->d_long_list;
+        v.list_count = count;
+        v.rect_skip = prev;
+        v.block_long = cur;
+        v.long_trips = (uint32_t)c->long_trips;
+        c->long_parity ^= 1;
+        c->long_valid = true;
+    }
+    RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
     return frame_end(c);
+}
+
+extern "C" int svo_set_long_first(svo_ctx* c, int long_trips)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (long_trips < 0 || long_trips > SVO_MAX_RAY_ITERS)
+        return svo_fail(SVO_ERR_ARG, "long_trips %d outside 0..%d", long_trips, (int)SVO_MAX_RAY_ITERS);
+    if (c->long_trips != long_trips) {
+        c->long_trips = long_trips;
+        c->long_valid = false;  // flags are not kept while off and mean another threshold
+    }
+    return SVO_OK;
 }
 
 extern "C" int svo_memset(svo_ctx* c, size_t byte_ofs, uint32_t val, size_t bytes)

@@ -94,6 +94,7 @@ _f = {
     "set_tuning": _sig("svo_set_tuning", C.c_int, _ctxp, C.c_int, C.c_int, C.c_int, C.c_int),
     "set_ray_spread": _sig("svo_set_ray_spread", C.c_int, _ctxp, C.c_int, C.c_int),
     "set_ray_lanes": _sig("svo_set_ray_lanes", C.c_int, _ctxp, C.c_int),
+    "set_long_first": _sig("svo_set_long_first", C.c_int, _ctxp, C.c_int),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
     "tile_schedule": _sig("svo_tile_schedule", C.c_int, C.c_int, C.c_int, C.c_int,
@@ -154,7 +155,7 @@ EXPORTED = [
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
-    "svo_set_ray_lanes", "svo_raycast_quads",
+    "svo_set_ray_lanes", "svo_set_long_first", "svo_raycast_quads",
 ]
 
 
@@ -417,6 +418,12 @@ class Context:
     def set_ray_lanes(self, lanes=64):
         """Lanes per wave that take rays (64 = all)."""
         _check(_f["set_ray_lanes"](self._h, int(lanes)))
+
+    def set_long_first(self, long_trips=1024):
+        """Full frames raycast first the 8x8 blocks whose previous full frame
+        had a ray of at least long_trips trips (0 = raster order only).
+        Results are the same either way."""
+        _check(_f["set_long_first"](self._h, int(long_trips)))
 
     def ray_count(self, reset=False):
         """Rays traced by the ray kernels since the last reset."""

@@ -6,7 +6,9 @@ applied when a setting names it but is not searched by default: fewer lanes
 helped short windows and hurt long ones) and, through svo_set_tuning, the work-queue refill threshold (idle
 lanes per refill), persistent workgroups per CU, traversal steps between
 refill checks, and the LDS node-cache depth (octree words of the
-breadth-first top levels kept in LDS).  Every candidate replays the same frame window from the same starting
+breadth-first top levels kept in LDS); full frames also search the trip
+count that puts a screen block on the long-rays-first list
+(svo_set_long_first).  Every candidate replays the same frame window from the same starting
 state (the window starts with the two clearing frames), so candidates see
 identical work; the score is the mean GPU time per frame over the timed
 frames (frames/s follows the mean), measured with the library's per-frame
@@ -67,6 +69,10 @@ def _search(space, coordinate, measure):
     return best, best_ms
 
 
+# full frames also search the long-ray-first threshold (svo_set_long_first)
+FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
+
+
 def apply(ctx, cfg):
     """Apply a knob setting (as returned by tune / tune_batch) to a context."""
     ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
@@ -74,6 +80,8 @@ def apply(ctx, cfg):
         ctx.set_ray_spread(int(cfg["spread"]), True)
     if "lanes" in cfg:
         ctx.set_ray_lanes(int(cfg["lanes"]))
+    if "long_trips" in cfg:
+        ctx.set_long_first(int(cfg["long_trips"]))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):
@@ -83,7 +91,9 @@ def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=No
     coordinate=True the knobs are tuned one at a time (coordinate descent,
     two sweeps); otherwise the full grid is measured.  Applies and returns
     (best_setting, best_ms, trials)."""
-    space = dict(DEFAULT_SPACE if space is None else space)
+    if space is None:
+        space = FULL_SPACE if mode == "full" else DEFAULT_SPACE
+    space = dict(space)
     trials = []
 
     def measure(cfg):

@@ -116,7 +116,8 @@ def test_warp_graph_vs_direct_launches(scene1):
                                                                 (1, 8, 2, 16, 0, 1)])
 def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, cache, lanes):
     """Ray hand-out order, refill threshold, workgroups per CU, trips per
-    refill check and the LDS node cache change only the schedule."""
+    refill check, the LDS node cache and lanes per wave change only the
+    schedule."""
     words, root = scene1
     rx, ry = CONFIG1["res"]
     pr = path(CONFIG1, 6)
@@ -144,3 +145,34 @@ def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, c
     finally:
         c.close()
         o.close()
+
+
+@pytest.mark.parametrize("long_trips", [1, 16, 512])
+def test_full_frame_long_first_bitexact(scene1, long_trips):
+    """Full frames that raycast the previous frame's long-ray blocks first
+    (1: every block, so the front list is the whole screen): identical to the
+    oracle, with the setting switched off for one frame, across a warp frame
+    in between and a resize to a size that is not a multiple of 8."""
+    words, root = scene1
+    pr = path(CONFIG1, 6)
+    c = S.Context(0)
+    try:
+        c.upload_octree(words, root, 8)
+        for rx, ry in ((CONFIG1["res"][0], CONFIG1["res"][1]), (203, 133)):
+            c.resize(rx, ry)
+            o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+            for f in range(6):
+                p = params(CONFIG1, pr, f)
+                c.set_long_first(0 if f == 4 else long_trips)
+                print(f"long first {long_trips} {rx}x{ry} frame {f}", flush=True)
+                if f == 2 and rx % 8 == 0:
+                    c.frame_warp(p)  # warp state in between (sizes the warp tests cover)
+                c.frame_full(p)
+                c.sync()
+                o.frame_full(O.params_from(p))
+                assert_equal_u32(c.screen(0), o.screen[0], f"{rx}x{ry} frame {f} screen")
+                assert_equal_f32_bits(c.coords(0)[..., :3], o.coords[0][..., :3], f"{rx}x{ry} frame {f} coords")
+                assert_equal_u32(c.image(), o.image, f"{rx}x{ry} frame {f} image")
+            o.close()
+    finally:
+        c.close()

@@ -127,7 +127,7 @@ def run_gpu(args, name):
     warp2 = fresh()
     warp2.enable_timing(mode=2)
     full.enable_timing(mode=2)
-    frame_ms, full_ms, psnrs = [], [], []
+    frame_ms, full_ms, psnrs, maxabs, same = [], [], [], [], []
     for f in range(args.warmup, total):
         warp2.frame_warp(params[f])
         warp2.sync()
@@ -138,7 +138,11 @@ def run_gpu(args, name):
         full.sync()
         full_ms.append(full.frame_time())
         if (f - args.warmup) % max(1, args.steps // 10) == 0:
-            psnrs.append(S.psnr(S.image_rgb(warp2.image()), S.image_rgb(full.image())))
+            a, b = S.image_rgb(warp2.image()), S.image_rgb(full.image())
+            psnrs.append(S.psnr(a, b))
+            d = np.abs(a.astype(np.int16) - b.astype(np.int16))
+            maxabs.append(int(d.max()))
+            same.append(float((d.max(axis=-1) == 0).mean()))
     warp2.close()
     # ---- pass C: per-stage times (an event before each stage, direct
     # launches), same frames on fresh state; the ray stage is the roofline's
@@ -164,7 +168,8 @@ def run_gpu(args, name):
         warp_ms=frame_ms, full_ms=full_ms, stage_mean=stage_mean,
         rays_per_frame=rays_per_frame, rays_full_per_frame=rays_full / args.steps,
         mrays_warp=rays_warp / t_warp / 1e6, mrays_full=rays_full / t_full / 1e6,
-        ray_ms_full=float(np.mean(ray_ms_full)) if ray_ms_full else float("nan"), psnr=psnrs, depth=depth, words=len(words),
+        ray_ms_full=float(np.mean(ray_ms_full)) if ray_ms_full else float("nan"), psnr=psnrs, maxabs=maxabs,
+        same=same, depth=depth, words=len(words),
         rx=rx, ry=ry)
     # ---- learned fill (gate + BF16 CNN) with the committed weights
     if args.learned and args.legs == "all":
@@ -495,6 +500,8 @@ def main():
         "mrays_s_warp": round(g["mrays_warp"], 1), "mrays_s_full": round(g["mrays_full"], 1),
         "rays_per_warp_frame": round(g["rays_per_frame"]),
         "psnr_warp_vs_full_db": round(float(np.mean([p for p in g["psnr"] if p != float("inf")] or [99.0])), 2),
+        "maxabs_warp_vs_full": max(g["maxabs"]) if g["maxabs"] else None,
+        "identical_pixels_warp_vs_full": round(float(np.mean(g["same"])), 4) if g["same"] else None,
         "stage_ms": {k: round(v, 4) for k, v in g["stage_mean"].items()},
         "autotuned": {k: g[k] for k in ("tuned_warp", "tuned_full") if k in g},
         "learned_fill": None if "learned_fps" not in g else {

@@ -74,6 +74,7 @@ def test_hits_match_fp64_dda(scene):
     rng = np.random.default_rng(5)
     rx, ry = 2048, 1024                    # lodswitch = 4096 ray units
     checked = agree = 0
+    errs = []
     for cam in range(4):
         yaw = float(rng.uniform(-3, 3))
         pitch = float(rng.uniform(0.25, 0.7))
@@ -96,6 +97,25 @@ def test_hits_match_fp64_dda(scene):
                          <= cell[i] * 2 + 2 + 1e-3 for i in range(3))
             colour_ok = ((word - 0xFF000000) & 0xFF) == (int(col[cell[0] % SIZE, cell[1], cell[2] % SIZE]) & 0xFF)
             agree += bool(on_box and colour_ok)
-    print(f'checked {checked} agree {agree}')
+            if on_box and colour_ok:
+                # FP32 hit point vs the FP64 entry point o + t d of the same voxel
+                # (x and z compared modulo the 2*SIZE wrap)
+                p64 = [o[i] + t * d[i] for i in range(3)]
+                err = []
+                for i in range(3):
+                    e = pt[i] - p64[i]
+                    if i != 1:
+                        e -= SIZE * 2 * round(e / (SIZE * 2))
+                    err.append(e)
+                abs_err = math.sqrt(sum(e * e for e in err))
+                rng_len = t * math.sqrt(sum(v * v for v in d))
+                errs.append((abs_err, abs_err / max(abs(v) for v in p64), abs_err / rng_len))
+    errs = np.array(errs)
+    print(f'checked {checked} agree {agree}; hit point vs FP64: max {errs[:, 0].max():.3g} ray units, '
+          f'max {errs[:, 1].max():.3g} relative to the coordinate, max {errs[:, 2].max():.3g} relative to the '
+          f'ray length; median {np.median(errs[:, 1]):.3g}')
     assert checked > 300
     assert agree >= 0.99 * checked, (agree, checked)
+    # FP32 world coordinates of the hits stay within 1e-5 of the coordinate
+    # magnitude of the FP64 march's entry points
+    assert errs[:, 1].max() < 1e-5, errs[:, 1].max()

@@ -286,10 +286,12 @@ int svo_cnn_gradients(svo_cnn* cnn, const uint16_t* x, const float* target, cons
 
 /* One Adam step (learning rate lr) on n_crops random crop x crop windows
  * (origins drawn from seed) of the frame pair the two contexts currently
- * hold: features from `warp`'s pre-filter planes, targets from `full`'s
- * image, loss on warp-frame hole pixels.  Both contexts must have rendered the
- * same camera at one resolution.  loss_out (optional) makes the call block
- * and returns the step's loss. */
+ * hold: features from the CNN input of `warp`'s last svo_frame_warp_learned
+ * (plane 0 as the gate left it), targets from `full`'s image, loss on the
+ * pixels still holes after that frame's rays.  Both contexts must have
+ * rendered the same camera at one resolution; SVO_ERR_STATE if `warp` has
+ * rendered no learned frame since its last resize.  loss_out (optional)
+ * makes the call block and returns the step's loss. */
 int svo_cnn_train_step(svo_cnn* cnn, svo_ctx* warp, svo_ctx* full, const svo_frame_params* p,
                        int n_crops, int crop, uint64_t seed, float lr, float* loss_out);
 /* The frame fill's inference kernel: one workgroup runs all four layers on
@@ -307,8 +309,12 @@ int svo_cnn_set_fused(svo_cnn* cnn, int mode);
  * neither the hole list nor the refresh tile will trace.  A tile with more
  * than max_cnn_holes of them has them raycast in the same ray launch as the
  * hole and tile rays; a tile with fewer is filled by the CNN (image only;
- * screen words keep fillhole2's result).  cnn may be NULL: the gate alone,
- * fillhole2 colours everywhere. */
+ * screen words keep fillhole2's result).  The CNN's input is plane 0 as the
+ * gate leaves it (copied before any ray), so it runs on a second stream
+ * alongside the ray launch; its colours replace the fill's at the tile
+ * pixels still holes after the rays.  Timing mode 1 runs it after the fill
+ * instead (same results).  cnn may be NULL: the gate alone, fillhole2
+ * colours everywhere. */
 int svo_frame_warp_learned(svo_ctx* ctx, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
 /* Gate outcome of the last svo_frame_warp_learned (blocking): tiles sent to
  * the CNN and residual-hole pixels sent back to raycasting. */
@@ -332,8 +338,8 @@ int svo_batch_frame_warp(svo_batch* batch, const svo_frame_params* params);
 int svo_batch_sync(svo_batch* batch);
 /* svo_frame_warp_learned for every view (same cnn and max_cnn_holes for all):
  * the gate of all views in one launch, their gate pixels in the shared ray
- * launch, then the CNN fill of each view.  Per view identical to
- * svo_frame_warp_learned on that context. */
+ * launch, the CNN of each view on the batch's second stream alongside it.
+ * Per view identical to svo_frame_warp_learned on that context. */
 int svo_batch_frame_warp_learned(svo_batch* batch, svo_cnn* cnn, const svo_frame_params* params,
                                  int max_cnn_holes);
 

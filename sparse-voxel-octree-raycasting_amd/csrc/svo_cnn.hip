@@ -209,8 +209,9 @@ __device__ __forceinline__ void rgb_of(uint32_t word, float& r, float& g, float&
 }
 
 struct FeatArgs {
-    const uint32_t* words;   // plane 2 screen words (pre-filter)
-    const float4* coords;    // plane 2 coords
+    const uint32_t* words;   // screen words of the CNN input (hole channel)
+    const float4* coords;    // coords of the CNN input
+    const uint32_t* mask_words;  // training: the loss covers pixels that are holes here (else `words`)
     const uint32_t* image;   // full-raycast image (targets) or null
     int res_x, res_y, crop;
     const int2* origins;     // explicit crop origins, or null
@@ -247,9 +248,9 @@ __global__ __launch_bounds__(256) void k_features(FeatArgs a)
         if (x >= 0 && y >= 0 && x < a.res_x && y < a.res_y) {
             const size_t o = (size_t)y * a.res_x + x;
             const uint32_t w = a.words[o];
+            m = (a.mask_words ? a.mask_words[o] : w) == HOLE_WORD ? 1.f : 0.f;
             if (w == HOLE_WORD) {
                 f[4] = 1.f;
-                m = 1.f;
             } else {
                 rgb_of(w, f[0], f[1], f[2]);
                 const float4 cc = a.coords[o];
@@ -277,11 +278,33 @@ __global__ __launch_bounds__(256) void k_features(FeatArgs a)
     if (a.target && cnt != 0.f) atomicAdd(a.mask_count, cnt);
 }
 
-// CNN RGB -> image at the hole pixels of the centre 16x16 of each tile crop
-__global__ __launch_bounds__(256) void k_write_fill(const float* __restrict__ out, const uint32_t* __restrict__ tiles,
-                                                    const uint32_t* __restrict__ count, int max_crops,
-                                                    const uint32_t* __restrict__ words, uint32_t* image, int res_x,
-                                                    int res_y, int tiles_x)
+__device__ __forceinline__ uint32_t rgb_word(float r, float g, float b)
+{
+    const uint32_t R = (uint32_t)fminf(fmaxf(r * 255.f + 0.5f, 0.f), 255.f);
+    const uint32_t G = (uint32_t)fminf(fmaxf(g * 255.f + 0.5f, 0.f), 255.f);
+    const uint32_t B = (uint32_t)fminf(fmaxf(b * 255.f + 0.5f, 0.f), 255.f);
+    return B | (G << 8) | (R << 16);
+}
+
+// CNN RGB of the centre 16x16 of each tile crop -> 256 colorized words per tile
+__global__ __launch_bounds__(256) void k_stage_fill(const float* __restrict__ out, const uint32_t* __restrict__ count,
+                                                    int max_crops, uint32_t* __restrict__ staged)
+{
+    const int nb = min((int)*count, max_crops);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nb * 256; i += gridDim.x * 256) {
+        const int c = i / 256, q = i % 256;
+        const float* v = out + ((size_t)c * 576 + (size_t)((q >> 4) + 4) * 24 + (q & 15) + 4) * 16;
+        staged[i] = rgb_word(v[0], v[1], v[2]);
+    }
+}
+
+// staged CNN colours -> image at the tile pixels that are still holes in
+// `words` (plane 2: the frame after its rays, before the filter)
+__global__ __launch_bounds__(256) void k_cnn_apply(const uint32_t* __restrict__ staged,
+                                                   const uint32_t* __restrict__ tiles,
+                                                   const uint32_t* __restrict__ count, int max_crops,
+                                                   const uint32_t* __restrict__ words, uint32_t* image, int res_x,
+                                                   int res_y, int tiles_x)
 {
     const int nb = min((int)*count, max_crops);
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nb * 256; i += gridDim.x * 256) {
@@ -290,12 +313,7 @@ __global__ __launch_bounds__(256) void k_write_fill(const float* __restrict__ ou
         const int x = (t % tiles_x) * 16 + (q & 15), y = (t / tiles_x) * 16 + (q >> 4);
         if (x >= res_x || y >= res_y) continue;
         const size_t o = (size_t)y * res_x + x;
-        if (words[o] != HOLE_WORD) continue;
-        const float* v = out + ((size_t)c * 576 + (size_t)((q >> 4) + 4) * 24 + (q & 15) + 4) * 16;
-        const uint32_t r = (uint32_t)fminf(fmaxf(v[0] * 255.f + 0.5f, 0.f), 255.f);
-        const uint32_t g = (uint32_t)fminf(fmaxf(v[1] * 255.f + 0.5f, 0.f), 255.f);
-        const uint32_t b = (uint32_t)fminf(fmaxf(v[2] * 255.f + 0.5f, 0.f), 255.f);
-        image[o] = b | (g << 8) | (r << 16);
+        if (words[o] == HOLE_WORD) image[o] = staged[i];
     }
 }
 
@@ -316,7 +334,8 @@ typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 2) & 3)) << 3); }
 
 struct TileArgs {
-    // frame mode (x == null): features of the crop at tile*16 - 4 from plane 2
+    // frame mode (x == null): features of the crop at tile*16 - 4 from the
+    // CNN input planes, 256 colorized words per tile to `staged`
     const uint32_t* words;
     const float4* coords;
     int res_x, res_y, tiles_x;
@@ -324,7 +343,7 @@ struct TileArgs {
     const uint32_t* count;
     int max_tiles;
     float m0x, m0y, m0z, r20, r21, r22;
-    uint32_t* image;
+    uint32_t* staged;
     // explicit mode: x [n][24][24][16] bf16 -> out [n][16][16][3] float
     const bf16* x;
     float* out;
@@ -523,16 +542,7 @@ __global__ __launch_bounds__(NW * 64) void k_cnn_tiles(TileArgs a)
                 o[1] = g;
                 o[2] = b;
             } else {
-                const int x = ox + 4 + (q & 15), y = oy + 4 + (q >> 4);
-                if (x < a.res_x && y < a.res_y) {
-                    const size_t o = (size_t)y * a.res_x + x;
-                    if (a.words[o] == HOLE_WORD) {
-                        const uint32_t R = (uint32_t)fminf(fmaxf(r * 255.f + 0.5f, 0.f), 255.f);
-                        const uint32_t G = (uint32_t)fminf(fmaxf(g * 255.f + 0.5f, 0.f), 255.f);
-                        const uint32_t B = (uint32_t)fminf(fmaxf(b * 255.f + 0.5f, 0.f), 255.f);
-                        a.image[o] = B | (G << 8) | (R << 16);
-                    }
-                }
+                a.staged[(size_t)c * 256 + q] = rgb_word(r, g, b);
             }
         }
     }
@@ -910,8 +920,8 @@ static FeatArgs feat_args(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, int
 {
     FeatArgs a;
     memset(&a, 0, sizeof(a));
-    a.words = c->d_screen + 2 * c->N;
-    a.coords = c->d_coords + 2 * c->N;
+    a.words = c->d_cnn_words;
+    a.coords = c->d_cnn_coords;
     a.res_x = c->res_x;
     a.res_y = c->res_y;
     a.crop = crop;
@@ -923,15 +933,18 @@ static FeatArgs feat_args(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, int
 }
 
 // One Adam step on n_crops crop x crop windows of the current frame pair:
-// features from the warp context's pre-filter planes, targets from the full
-// context's image, loss on the warp frame's hole pixels.  Both contexts must
-// have rendered the same camera.  The CNN's stream waits for both contexts.
+// features from the warp context's last learned-frame CNN input (plane 0 as
+// its gate left it), targets from the full context's image, loss on the
+// pixels still holes after the warp frame's rays.  Both contexts must have
+// rendered the same camera.  The CNN's stream waits for both contexts.
 extern "C" int svo_cnn_train_step(svo_cnn* n, svo_ctx* warp, svo_ctx* full, const svo_frame_params* p, int n_crops,
                                   int crop, uint64_t seed, float lr, float* loss_out)
 {
     if (!n || !warp || !full || !p) return svo_fail(SVO_ERR_ARG, "null argument");
     if (warp->res_x != full->res_x || warp->res_y != full->res_y || !warp->d_screen)
         return svo_fail(SVO_ERR_STATE, "contexts must share one resolution");
+    if (!warp->cnn_input)
+        return svo_fail(SVO_ERR_STATE, "the warp context has rendered no learned frame since its last resize");
     if (n_crops <= 0 || n_crops > 256 || crop < 16 || crop % 8 || crop > warp->res_x || crop > warp->res_y)
         return svo_fail(SVO_ERR_ARG, "bad crop batch %d x %d", n_crops, crop);
     HIPCHECK(hipSetDevice(n->device));
@@ -962,6 +975,7 @@ extern "C" int svo_cnn_train_step(svo_cnn* n, svo_ctx* warp, svo_ctx* full, cons
     HIPCHECK(hipMemcpyAsync(n->origins, org.data(), sizeof(int2) * n_crops, hipMemcpyHostToDevice, n->stream));
     HIPCHECK(hipMemsetAsync(n->mask_count, 0, 4, n->stream));
     FeatArgs a = feat_args(n, warp, p, crop);
+    a.mask_words = warp->d_screen + 2 * warp->N;  // holes left after the frame's rays
     a.image = full->d_image;
     a.origins = n->origins;
     a.max_crops = n_crops;
@@ -986,8 +1000,8 @@ extern "C" int svo_cnn_train_step(svo_cnn* n, svo_ctx* warp, svo_ctx* full, cons
     return SVO_OK;
 }
 
-// Inference on the gate's tile list (24x24 crops around each 16x16 tile),
-// queued on the context's stream.
+// Inference on the gate's tile list (24x24 crops around each 16x16 tile of
+// the CNN input planes) into the staged colours, queued on `stream`.
 static TileArgs tile_args(svo_cnn* n)
 {
     TileArgs a;
@@ -1025,8 +1039,8 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
 {
     if (n->fused) {
         TileArgs a = tile_args(n);
-        a.words = c->d_screen + 2 * c->N;
-        a.coords = c->d_coords + 2 * c->N;
+        a.words = c->d_cnn_words;
+        a.coords = c->d_cnn_coords;
         a.res_x = c->res_x;
         a.res_y = c->res_y;
         a.tiles_x = c->n_tiles_x;
@@ -1035,7 +1049,7 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
         a.max_tiles = max_tiles;
         a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
         a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
-        a.image = c->d_image;
+        a.staged = c->d_cnn_out;
         return launch_tiles(a, n->tile_mode, max_tiles, c->device, stream);
     }
     int rc = ensure_ws(n, (size_t)max_tiles * 576);
@@ -1049,12 +1063,21 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
     hipLaunchKernelGGL(k_features, dim3(2048), dim3(256), 0, n->stream, a);
     rc = cnn_forward_dev(n, max_tiles, 24, 24, tile_count);
     if (!rc) {
-        hipLaunchKernelGGL(k_write_fill, dim3(2048), dim3(256), 0, n->stream, n->out, tile_list, tile_count,
-                           max_tiles, c->d_screen + 2 * c->N, c->d_image, c->res_x, c->res_y, c->n_tiles_x);
+        hipLaunchKernelGGL(k_stage_fill, dim3(2048), dim3(256), 0, n->stream, n->out, tile_count, max_tiles,
+                           c->d_cnn_out);
         if (hipGetLastError() != hipSuccess) rc = svo_fail(SVO_ERR_HIP, "CNN fill launch failed");
     }
     n->stream = keep;
     return rc;
+}
+
+int cnn_apply_tiles(svo_ctx* c, const uint32_t* tile_list, const uint32_t* tile_count, int max_tiles,
+                    hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_cnn_apply, dim3(2048), dim3(256), 0, stream, c->d_cnn_out, tile_list, tile_count, max_tiles,
+                       c->d_screen + 2 * c->N, c->d_image, c->res_x, c->res_y, c->n_tiles_x);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
 }
 
 // The fused tile network on explicit 24x24 crops: x [n][24][24][16] bf16

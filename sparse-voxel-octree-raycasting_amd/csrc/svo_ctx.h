@@ -7,10 +7,13 @@
 
 #include "svo_internal.h"
 
-enum Stage { ST_CLEAR, ST_SCATTER, ST_RESOLVE, ST_COUNT, ST_SCAN, ST_WRITE, ST_GATE, ST_RAYS, ST_COPY, ST_FILTER, ST_CNN, ST_N };
-static const char* const kStageNames[ST_N] = {"clear",    "proj_scatter", "proj_resolve", "counthole",
-                                              "sumids",   "writeids",     "gate",         "rays",
-                                              "copy",     "fill_colorize", "cnn"};
+enum Stage {
+    ST_CLEAR, ST_SCATTER, ST_RESOLVE, ST_COUNT, ST_SCAN, ST_WRITE, ST_GATE, ST_RAYS, ST_COPY, ST_FILTER, ST_CNN,
+    ST_CNN_APPLY, ST_N
+};
+static const char* const kStageNames[ST_N] = {"clear",    "proj_scatter", "proj_resolve",  "counthole",
+                                              "sumids",   "writeids",     "gate",          "rays",
+                                              "copy",     "fill_colorize", "cnn",          "cnn_apply"};
 
 struct svo_ctx {
     int device = 0;
@@ -37,6 +40,15 @@ struct svo_ctx {
     uint32_t* d_gate_tiles = nullptr;  // tile indices sent to the CNN
     uint32_t* d_gate_rays = nullptr;   // x | y << 16 of holes sent back to raycasting
     int n_tiles_x = 0, n_tiles_y = 0;  // 16x16 gate tiles (partial ones included)
+    // learned frames: the CNN's input is plane 0 as the gate leaves it (copied
+    // before any ray), so the CNN runs on cnn_stream alongside the ray launch;
+    // its colours wait in d_cnn_out until the fill has run
+    uint32_t* d_cnn_words = nullptr;   // N screen words
+    float4* d_cnn_coords = nullptr;    // N coords
+    uint32_t* d_cnn_out = nullptr;     // 256 colorized words per gate tile
+    bool cnn_input = false;            // d_cnn_* hold the last learned frame's input
+    hipStream_t cnn_stream = nullptr;
+    hipEvent_t cnn_fork = nullptr, cnn_join = nullptr;
     int launch_seq = 0;
     bool plane1_holes = false;
     // tuning
@@ -49,7 +61,7 @@ struct svo_ctx {
     // long_trips trips are raycast first (0 = raster order only)
     int long_trips = 1024;
     uint32_t* d_block_long = nullptr;  // two flag arrays (previous / current frame)
-    uint32_t* d_long_list = nullptr;   // 64 pixel words per long block
+    uint32_t* d_long_list = nullptr;   // pixel words (x | y << 16) of the long blocks
     uint32_t* d_long_count = nullptr;
     int long_parity = 0;
     bool long_valid = false;
@@ -72,6 +84,10 @@ struct svo_ctx {
 };
 
 
-// learned-fill hooks implemented in svo_cnn.hip
+// learned-fill hooks implemented in svo_cnn.hip: the CNN on the gate's tiles
+// (input d_cnn_words/d_cnn_coords, output d_cnn_out), then its colours into
+// the image at the tiles' pixels that are still holes in plane 2
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
                    const uint32_t* tile_count, int max_tiles, hipStream_t stream);
+int cnn_apply_tiles(svo_ctx* c, const uint32_t* tile_list, const uint32_t* tile_count, int max_tiles,
+                    hipStream_t stream);

@@ -135,6 +135,9 @@ struct View {
     const uint32_t* rect_skip;   // per 8x8 block: nonzero = raycast from the list instead, or null
     uint32_t* block_long;        // per 8x8 block: set to 1 by a ray of >= long_trips trips, or null
     uint32_t long_trips;
+    // learned frames: destination of the CNN input copy (plane 0 after the gate)
+    uint32_t* cnn_words;
+    float4* cnn_coords;
     // learned-fill gate of this view: counters, CNN tile list, gate pixel list
     uint32_t* gate;
     uint32_t* gate_tiles;
@@ -667,6 +670,17 @@ __global__ __launch_bounds__(256) void k_copy_plane(View v, const View* views, u
     V.coords[d] = V.coords[i];
 }
 
+// learned frames: the CNN's input, plane 0 as the gate leaves it (the rays
+// that follow write plane 0 while the CNN runs)
+__global__ __launch_bounds__(256) void k_cnn_input(View v, const View* views)
+{
+    VIEW_OF(blockIdx.z);
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= V.N) return;
+    V.cnn_words[i] = V.screen[i];
+    V.cnn_coords[i] = V.coords[i];
+}
+
 __device__ __forceinline__ uint32_t rs(const uint32_t* __restrict__ s, long o, long limit)
 {
     return o < limit ? s[o] : HOLE;
@@ -909,6 +923,13 @@ static int free_frame(svo_ctx* c)
     (void)hipFree(c->d_gate_tiles);
     (void)hipFree(c->d_gate_rays);
     c->d_gate = c->d_gate_tiles = c->d_gate_rays = nullptr;
+    (void)hipFree(c->d_cnn_words);
+    (void)hipFree(c->d_cnn_coords);
+    (void)hipFree(c->d_cnn_out);
+    c->d_cnn_words = nullptr;
+    c->d_cnn_coords = nullptr;
+    c->d_cnn_out = nullptr;
+    c->cnn_input = false;
     (void)hipFree(c->d_block_long);
     (void)hipFree(c->d_long_list);
     (void)hipFree(c->d_long_count);
@@ -940,6 +961,9 @@ extern "C" int svo_create(svo_ctx** out, int device)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->n_cu = prop.multiProcessorCount;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->cnn_stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->cnn_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->cnn_join, hipEventDisableTiming) == hipSuccess &&
               hipMalloc(&c->d_queue, 64) == hipSuccess && hipMemset(c->d_queue, 0, 64) == hipSuccess;
     for (int i = 0; ok && i <= ST_N; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
     if (!ok) {
@@ -955,6 +979,7 @@ extern "C" int svo_destroy(svo_ctx* c)
     if (!c) return SVO_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->cnn_stream) (void)hipStreamSynchronize(c->cnn_stream);
     free_frame(c);
     drop_graph(c);
     (void)hipFree(c->d_view);
@@ -966,7 +991,10 @@ extern "C" int svo_destroy(svo_ctx* c)
     (void)hipFree(c->d_queue);
     for (int i = 0; i <= ST_N; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
-    (void)hipStreamDestroy(c->stream);
+    if (c->cnn_fork) (void)hipEventDestroy(c->cnn_fork);
+    if (c->cnn_join) (void)hipEventDestroy(c->cnn_join);
+    if (c->cnn_stream) (void)hipStreamDestroy(c->cnn_stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SVO_OK;
 }
@@ -1363,12 +1391,31 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
 // raycasting and the rest to the CNN, whose RGB replaces the filtered colour
 // of their hole pixels in the image.  Screen words keep the reference
 // filter's result.  cnn == NULL keeps the filter's colours (gate only).
+// the CNN input planes and the staged CNN colours of a learned frame
+static int ensure_cnn_buffers(svo_ctx* c)
+{
+    if (c->d_cnn_words) return SVO_OK;
+    const size_t tiles = (size_t)c->n_tiles_x * c->n_tiles_y;
+    if (hipMalloc(&c->d_cnn_words, c->N * 4) != hipSuccess || hipMalloc(&c->d_cnn_coords, c->N * 16) != hipSuccess ||
+        hipMalloc(&c->d_cnn_out, tiles * 256 * 4) != hipSuccess) {
+        (void)hipFree(c->d_cnn_words);
+        (void)hipFree(c->d_cnn_coords);
+        (void)hipFree(c->d_cnn_out);
+        c->d_cnn_words = nullptr;
+        c->d_cnn_coords = nullptr;
+        c->d_cnn_out = nullptr;
+        return svo_fail(SVO_ERR_NOMEM, "learned-frame buffers at %dx%d", c->res_x, c->res_y);
+    }
+    return SVO_OK;
+}
+
 extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
 {
     RC(need_octree(c));
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     if (max_cnn_holes < 0 || max_cnn_holes > 256) return svo_fail(SVO_ERR_ARG, "max_cnn_holes outside 0..256");
     HIPCHECK(hipSetDevice(c->device));
+    RC(ensure_cnn_buffers(c));
     frame_begin(c);
     RC(warp_compact(c, p));
     int ax, ay, ex, ey;
@@ -1377,12 +1424,28 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
     v.list2 = c->d_gate_rays;
     v.list2_count = c->d_gate + GATE_RAYS;
+    v.cnn_words = c->d_cnn_words;
+    v.cnn_coords = c->d_cnn_coords;
     const int tiles = c->n_tiles_x * c->n_tiles_y;
     mark(c, ST_GATE);
     hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(1), 0, c->stream, v, (const View*)nullptr);
     hipLaunchKernelGGL(k_gate, dim3((unsigned)((tiles + GATE_TILES - 1) / GATE_TILES)), dim3(256), 0, c->stream, v,
                        (const View*)nullptr, max_cnn_holes);
+    hipLaunchKernelGGL(k_cnn_input, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, v,
+                       (const View*)nullptr);
     HIPCHECK(hipGetLastError());
+    c->cnn_input = true;
+    // The CNN reads only its input planes and writes only d_cnn_out, so it
+    // runs on the second stream alongside the ray launch, which is
+    // tail-bound by a few long rays.  Per-stage timing runs it after the
+    // fill on the frame's stream instead, so its stage time is its own.
+    const bool overlap = cnn && c->timing != 1;
+    if (overlap) {
+        HIPCHECK(hipEventRecord(c->cnn_fork, c->stream));
+        HIPCHECK(hipStreamWaitEvent(c->cnn_stream, c->cnn_fork, 0));
+        RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles, c->cnn_stream));
+        HIPCHECK(hipEventRecord(c->cnn_join, c->cnn_stream));
+    }
     // hole list, gate rays and the refresh tile in one ray launch
     mark(c, ST_RAYS);
     RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
@@ -1393,7 +1456,12 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     RC(launch_fill(L, v, nullptr, 1, 1));
     if (cnn) {
         mark(c, ST_CNN);
-        RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles, c->stream));
+        if (overlap)
+            HIPCHECK(hipStreamWaitEvent(c->stream, c->cnn_join, 0));
+        else
+            RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles, c->stream));
+        mark(c, ST_CNN_APPLY);
+        RC(cnn_apply_tiles(c, c->d_gate_tiles, c->d_gate, tiles, c->stream));
     }
     return frame_end(c);
 }
@@ -1550,6 +1618,8 @@ struct svo_batch {
     uint32_t root = 0;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t cnn_stream = nullptr;  // the views' CNN fills, alongside the ray launch
+    hipEvent_t cnn_fork = nullptr, cnn_join = nullptr;
     View* d_views = nullptr;     // 2 slots x n
     View* h_views = nullptr;     // pinned staging, 2 slots x n
     hipEvent_t slot_done[2] = {};
@@ -1566,6 +1636,7 @@ extern "C" int svo_batch_destroy(svo_batch* b)
     if (!b) return SVO_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->cnn_stream) (void)hipStreamSynchronize(b->cnn_stream);
     (void)hipFree(b->d_views);
     (void)hipHostFree(b->h_views);
     (void)hipFree(b->d_first);
@@ -1574,6 +1645,9 @@ extern "C" int svo_batch_destroy(svo_batch* b)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : b->pre)
         if (e) (void)hipEventDestroy(e);
+    if (b->cnn_fork) (void)hipEventDestroy(b->cnn_fork);
+    if (b->cnn_join) (void)hipEventDestroy(b->cnn_join);
+    if (b->cnn_stream) (void)hipStreamDestroy(b->cnn_stream);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return SVO_OK;
@@ -1607,6 +1681,9 @@ extern "C" int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_vi
         b->maxS = std::max(b->maxS, (uint32_t)c->S);
     }
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&b->cnn_stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->cnn_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->cnn_join, hipEventDisableTiming) == hipSuccess &&
               hipMalloc(&b->d_views, sizeof(View) * 2 * n_views) == hipSuccess &&
               hipHostMalloc(&b->h_views, sizeof(View) * 2 * n_views) == hipSuccess &&
               hipMalloc(&b->d_first, 4 * (n_views + 1)) == hipSuccess && hipMalloc(&b->d_queue, 64) == hipSuccess &&
@@ -1655,6 +1732,8 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
     const int n = (int)b->views.size();
     svo_ctx* c0 = b->views[0];
     HIPCHECK(hipSetDevice(b->device));
+    if (learned)
+        for (auto* c : b->views) RC(ensure_cnn_buffers(c));
     // order after the views' own queued work
     for (int i = 0; i < n; i++) {
         HIPCHECK(hipEventRecord(b->pre[i], b->views[i]->stream));
@@ -1681,6 +1760,8 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
         if (learned) {
             v.list2 = c->d_gate_rays;
             v.list2_count = c->d_gate + GATE_RAYS;
+            v.cnn_words = c->d_cnn_words;
+            v.cnn_coords = c->d_cnn_coords;
         }
         hv[i] = v;
     }
@@ -1697,7 +1778,20 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
         hipLaunchKernelGGL(k_gate_reset, dim3(1, (unsigned)n), dim3(1), 0, b->stream, dummy, (const View*)dv);
         hipLaunchKernelGGL(k_gate, dim3((max_gate_tiles + GATE_TILES - 1) / GATE_TILES, (unsigned)n), dim3(256), 0,
                            b->stream, dummy, (const View*)dv, max_cnn_holes);
+        hipLaunchKernelGGL(k_cnn_input, dim3((b->maxN + 255) / 256, 1, (unsigned)n), dim3(256), 0, b->stream, dummy,
+                           (const View*)dv);
         HIPCHECK(hipGetLastError());
+        for (auto* c : b->views) c->cnn_input = true;
+        if (cnn) {  // every view's CNN on the batch's second stream, alongside the ray launch
+            HIPCHECK(hipEventRecord(b->cnn_fork, b->stream));
+            HIPCHECK(hipStreamWaitEvent(b->cnn_stream, b->cnn_fork, 0));
+            for (int i = 0; i < n; i++) {
+                svo_ctx* c = b->views[i];
+                RC(cnn_fill_tiles(cnn, c, &params[i], c->d_gate_tiles, c->d_gate, c->n_tiles_x * c->n_tiles_y,
+                                  b->cnn_stream));
+            }
+            HIPCHECK(hipEventRecord(b->cnn_join, b->cnn_stream));
+        }
     }
     hipLaunchKernelGGL(k_view_first, dim3(1), dim3(1), 0, b->stream, (const View*)dv, n, b->d_first,
                        (uint32_t*)nullptr);
@@ -1705,12 +1799,13 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
     RC(launch_rays_v(c0, b->stream, dummy, dv, b->d_first, n, b->d_queue, b->seq));
     RC(launch_copy(L, dummy, 2));
     RC(launch_fill(L, dummy, nullptr, 1, 1));
-    if (learned && cnn)
+    if (learned && cnn) {
+        HIPCHECK(hipStreamWaitEvent(b->stream, b->cnn_join, 0));
         for (int i = 0; i < n; i++) {
             svo_ctx* c = b->views[i];
-            RC(cnn_fill_tiles(cnn, c, &params[i], c->d_gate_tiles, c->d_gate, c->n_tiles_x * c->n_tiles_y,
-                              b->stream));
+            RC(cnn_apply_tiles(c, c->d_gate_tiles, c->d_gate, c->n_tiles_x * c->n_tiles_y, b->stream));
         }
+    }
     HIPCHECK(hipEventRecord(b->slot_done[slot], b->stream));
     for (auto* c : b->views) HIPCHECK(hipStreamWaitEvent(c->stream, b->slot_done[slot], 0));
     b->frame_no++;

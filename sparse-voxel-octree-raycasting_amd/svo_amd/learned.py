@@ -1,7 +1,9 @@
 """Training and evaluation of the learned hole filler on frame pairs the
-pipeline generates itself: a warp context renders the warp+fill frame, a
-second context renders the full raycast of the same camera (the judge of the
-error), and the CNN takes one Adam step on random crops of each pair.
+pipeline generates itself: a warp context renders the gated warp frame
+without the CNN (its CNN input is plane 0 as the gate leaves it, and the
+holes left after its rays are the pixels the CNN fills), a second context
+renders the full raycast of the same camera (the judge of the error), and
+the CNN takes one Adam step on random crops of each pair.
 """
 from __future__ import annotations
 
@@ -27,7 +29,7 @@ def _contexts(name):
     return ctxs
 
 
-def train(cnn, plan, n_crops=16, crop=64, lr=1e-3, lr_end=1e-4, log=None, log_every=200):
+def train(cnn, plan, n_crops=16, crop=64, lr=1e-3, lr_end=1e-4, log=None, log_every=200, max_cnn_holes=64):
     """plan: list of (workload name, path seed offset, n_frames).  Frames are
     replayed along each seeded path; one step per frame pair.  Returns the
     per-step losses (every log_every steps, others None)."""
@@ -41,7 +43,7 @@ def train(cnn, plan, n_crops=16, crop=64, lr=1e-3, lr_end=1e-4, log=None, log_ev
         for f in range(n_frames):
             r = f % len(rows)
             p = W.frame_params(name, rows, r, frame_index=f)
-            warp.frame_warp(p)
+            warp.frame_warp_learned(None, p, max_cnn_holes)
             full.frame_full(p)
             a = step / max(1, total - 1)
             cur = lr * (lr_end / lr) ** a

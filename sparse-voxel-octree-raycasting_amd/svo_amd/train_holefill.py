@@ -20,6 +20,8 @@ def main():
     ap.add_argument("--crop", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--train-max-holes", type=int, default=256,
+                    help="gate threshold of the training frames (more CNN tiles -> more loss pixels)")
     ap.add_argument("--out", default=L.WEIGHTS)
     a = ap.parse_args()
     plan = []
@@ -33,10 +35,11 @@ def main():
     cnn = Cnn(0, seed=a.seed)
     t = time.time()
     losses = L.train(cnn, plan, a.crops, a.crop, a.lr, a.lr / 10, log=lambda m: print(m, flush=True),
-                     log_every=500)
+                     log_every=500, max_cnn_holes=a.train_max_holes)
     dt = time.time() - t
     got = [x for x in losses if x is not None]
-    L.save(cnn, a.out, pairs=a.pairs, crops=a.crops, crop=a.crop, seconds=dt, final_loss=got[-1])
+    L.save(cnn, a.out, pairs=a.pairs, crops=a.crops, crop=a.crop, seconds=dt, final_loss=got[-1],
+           train_max_holes=a.train_max_holes)
     print(f"trained {a.pairs} pairs in {dt:.1f}s, final loss {got[-1]:.5f} -> {a.out}", flush=True)
     for name in ("flythrough_256", "flythrough_2048"):
         allp, holes = L.evaluate(cnn, name, n_frames=40)

@@ -90,3 +90,58 @@ def test_fused_tile_fill_matches_per_layer_fill(mode):
         a.close()
         b.close()
         cnn.close()
+
+
+def test_cnn_alongside_rays_equals_sequential():
+    """The CNN on the second stream alongside the ray launch and the CNN run
+    after the fill on the frame's stream (per-stage timing) give identical
+    screens and images."""
+    cnn = S.Cnn(0)
+    L.load(cnn, L.WEIGHTS)
+    words, root, depth, _ = W.octree(NAME)
+    rx, ry = W.WORKLOADS[NAME]["res"]
+    rows = W.path(NAME, 12)
+    a, b = S.Context(0), S.Context(0)
+    try:
+        for c in (a, b):
+            c.upload_octree(words, root, depth)
+            c.resize(rx, ry)
+        b.enable_timing(True, mode=1)
+        n_cnn = 0
+        for f in range(12):
+            p = W.frame_params(NAME, rows, f)
+            a.frame_warp_learned(cnn, p, 64)
+            b.frame_warp_learned(cnn, p, 64)
+            a.sync()
+            b.sync()
+            assert_equal_u32(a.screen(), b.screen(), f"frame {f} screen")
+            assert_equal_u32(a.image(), b.image(), f"frame {f} image")
+            n_cnn += a.gate_counts()[0]
+        assert n_cnn > 0
+    finally:
+        a.close()
+        b.close()
+        cnn.close()
+
+
+def test_train_step_needs_a_learned_frame():
+    cnn = S.Cnn(0)
+    words, root, depth, _ = W.octree(NAME)
+    rx, ry = W.WORKLOADS[NAME]["res"]
+    p = W.frame_params(NAME, W.path(NAME, 2), 0)
+    warp, full = S.Context(0), S.Context(0)
+    try:
+        for c in (warp, full):
+            c.upload_octree(words, root, depth)
+            c.resize(rx, ry)
+        warp.frame_warp(p)
+        full.frame_full(p)
+        with pytest.raises(S.SvoError):
+            cnn.train_step(warp, full, p, 4, 32)
+        warp.frame_warp_learned(None, p, 64)
+        loss = cnn.train_step(warp, full, p, 4, 32, want_loss=True)
+        assert np.isfinite(loss)
+    finally:
+        warp.close()
+        full.close()
+        cnn.close()

@@ -1386,11 +1386,6 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     return frame_end(c);
 }
 
-// Warp frame with the learned fill: after the hole and tile rays, a per-tile
-// gate sends tiles with more than max_cnn_holes hole pixels back to
-// raycasting and the rest to the CNN, whose RGB replaces the filtered colour
-// of their hole pixels in the image.  Screen words keep the reference
-// filter's result.  cnn == NULL keeps the filter's colours (gate only).
 // the CNN input planes and the staged CNN colours of a learned frame
 static int ensure_cnn_buffers(svo_ctx* c)
 {
@@ -1409,6 +1404,12 @@ static int ensure_cnn_buffers(svo_ctx* c)
     return SVO_OK;
 }
 
+// Warp frame with the learned fill: after hole compaction a per-tile gate
+// sends tiles with more than max_cnn_holes residual holes back to
+// raycasting (in the frame's ray launch) and the rest to the CNN, whose RGB
+// replaces the filtered colour of their hole pixels in the image.  Screen
+// words keep the reference filter's result.  cnn == NULL keeps the filter's
+// colours (gate only).
 extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
 {
     RC(need_octree(c));

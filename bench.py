@@ -349,14 +349,14 @@ def run_batch(args):
         t = time.perf_counter() - t0
         return t, ctxs[0].ray_count(reset=True)
 
-    # view by view with the library's default knobs, then the batch with its
-    # own tuned knobs
-    t_seq, rays_seq = seq_run()
     if args.tune:
         from svo_amd import autotune as A
         best, ms, _ = A.tune_batch(batch, frames[:min(n_frames, 16)], warmup=6)
         out["tuned"] = dict(best, ms=round(ms, 4))
         log(f"[bench] batch autotune: {best} -> {ms:.4f} ms/batch frame")
+        for c in ctxs[1:]:  # the view-by-view run uses the same knobs (faster there than the defaults)
+            A.apply(c, best)
+    t_seq, rays_seq = seq_run()
     t_bat, rays_bat = batch_run()
     k = n_frames - warm
     # the same frames with the learned fill on (gate + fused CNN per view)

@@ -464,6 +464,10 @@ uint32_t orc_trace_pixel(const uint32_t* oct, size_t n_words, uint32_t root,
     uint32_t col = fetch_color(&o, nodeid, nodeid_before, stack[rek + 1], &local_root,
                                rek, node_test);
     xyz[0] = px / 16.0f; xyz[1] = py / 16.0f; xyz[2] = pz / 16.0f;
+    if (st) {
+        st->last_rek = rek; st->last_lod = lod; st->last_hit = node_test != 0;
+        st->last_ix = ix; st->last_iy = iy; st->last_iz = iz; st->last_sign = sign_xyz;
+    }
     return 0xff000000u + col;
 }
 

@@ -70,6 +70,11 @@ void orc_free(void* p);
 typedef struct {
     uint64_t rays, iters, descends, steps, pops, fetch_words;
     uint64_t desc_by_rek[16];   /* descends by level (rekursion) */
+    /* the last traced ray: its level (rekursion) and lod at loop exit,
+     * whether it exited by a hit (node_test set), its integer position in
+     * the octant-folded frame and the fold (sign_xyz); not accumulated */
+    int64_t last_rek, last_lod, last_hit;
+    int64_t last_ix, last_iy, last_iz, last_sign;
 } orc_stats;
 
 /* One primary ray for pixel (idx,idy); returns the screen word

@@ -2,7 +2,11 @@
 :32-62, fetchColor :64-112) against an independent FP64 voxel-grid DDA
 (Amanatides-Woo) on a small dense scene: for rays that hit within the LOD-1
 range, the hit point lies on the first voxel the FP64 march enters and the
-returned colour byte is that voxel's leaf byte."""
+returned colour byte is that voxel's leaf byte.  Rays that stop at a coarser
+level (LOD) return the colour the octree defines for the cube they stop in:
+its creator voxel's colour (the first voxel of the stream inside it,
+octree.h:46-48, :75-81) from level 3 up, the first occupied child's leaf byte
+at level 2 (kernel.cl:100-109)."""
 import math
 
 import numpy as np
@@ -33,7 +37,7 @@ def scene():
     xyz = np.stack([xs, ys, zs], 1)[order].astype(np.uint16)
     rgba = col[xs, ys, zs][order]
     words, root = S.build_octree(xyz, rgba, D)
-    return occ, col, words, root
+    return occ, col, words, root, xyz, rgba
 
 
 def ray_dir32(p, x, y, rx, ry):
@@ -70,7 +74,7 @@ def march(occ, o, d, max_dist):
 
 
 def test_hits_match_fp64_dda(scene):
-    occ, col, words, root = scene
+    occ, col, words, root, _, _ = scene
     rng = np.random.default_rng(5)
     rx, ry = 2048, 1024                    # lodswitch = 4096 ray units
     checked = agree = 0
@@ -119,3 +123,65 @@ def test_hits_match_fp64_dda(scene):
     # FP32 world coordinates of the hits stay within 1e-5 of the coordinate
     # magnitude of the FP64 march's entry points
     assert errs[:, 1].max() < 1e-5, errs[:, 1].max()
+
+
+def test_lod_hits_return_node_colours(scene):
+    occ, col, words, root, xyz, rgba = scene
+    # creator voxel of every occupied cube of edge 2^lv voxels: the first voxel
+    # of the stream inside it
+    creator = {}
+    for lv in range(2, D + 1):
+        e = 1 << lv
+        keys = (xyz[:, 0].astype(np.int64) // e) * (SIZE * SIZE) + (xyz[:, 1].astype(np.int64) // e) * SIZE + \
+            xyz[:, 2] // e
+        u, first = np.unique(keys, return_index=True)
+        creator[lv] = dict(zip(u.tolist(), first.tolist()))
+    rng = np.random.default_rng(9)
+    rx, ry = 64, 32                        # lodswitch = 128 ray units: most hits stop at a coarse level
+    by_level = {}
+    for cam in range(4):
+        yaw = float(rng.uniform(-3, 3))
+        pitch = float(rng.uniform(0.05, 0.5))
+        pos = (float(rng.uniform(0, 32)), float(rng.uniform(17, 24)), float(rng.uniform(0, 32)))
+        p, _ = O.camera(yaw, pitch, 0.0, pos, D, 0)
+        for y in range(ry):
+            for x in range(rx):
+                d = ray_dir32(p, x, y, rx, ry)
+                if min(abs(v) for v in d) < 2**-20:
+                    continue
+                word, pt, st = O.trace_pixel(words, root, D, rx, ry, p, x, y)
+                if not st.last_hit or st.last_rek < 2:
+                    continue
+                rek = int(st.last_rek)
+                lv = rek - 1                   # the cube the ray stopped in: edge 2^lv voxels
+                ncell = SIZE >> lv
+                # its index from the ray's integer position in the folded frame
+                # (folding p -> 2*SIZE - p maps cell k to ncell-1-k)
+                c = []
+                for i, ip in enumerate((st.last_ix, st.last_iy, st.last_iz)):
+                    k = int(ip) >> rek
+                    c.append(ncell - 1 - k if (int(st.last_sign) >> i) & 1 else k)
+                # the hit point lies on that cube (x and z modulo the 2*SIZE-unit wrap)
+                for i in range(3):
+                    lo, hi = c[i] * (1 << rek), (c[i] + 1) * (1 << rek)
+                    v = pt[i] * 16.0
+                    if i != 1:
+                        v -= 2 * SIZE * math.floor((v - 0.5 * (lo + hi)) / (2 * SIZE) + 0.5)
+                    assert lo - 0.01 <= v <= hi + 0.01, (x, y, i, v, lo, hi)
+                cx, cy, cz = c[0] % ncell, c[1], c[2] % ncell
+                got = (word - 0xFF000000) & 0xFFFFFFFF
+                if lv == 1:
+                    exp = next(int(col[2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + (j >> 2)]) & 0xFF
+                               for j in range(8) if occ[2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + (j >> 2)])
+                    ok = (got & 0xFF) == exp
+                else:
+                    i = creator[lv][cx * SIZE * SIZE + cy * SIZE + cz]
+                    # level 3 returns a byte of a packed byte-node word; above, the colour word
+                    ok = (got & 0xFF) == (int(rgba[i]) & 0xFF) if lv == 2 else (got & 0xFFFFFF) == (int(rgba[i]) & 0xFFFFFF)
+                t = by_level.setdefault(rek, [0, 0])
+                t[0] += 1
+                t[1] += bool(ok)
+    print("level: (rays, agreeing)", by_level)
+    assert sum(v[0] for v in by_level.values()) > 3000
+    assert all(k in by_level for k in (2, 3, 4, 5)), by_level
+    assert all(v[0] == v[1] for v in by_level.values()), by_level

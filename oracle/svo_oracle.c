@@ -6,6 +6,17 @@
  *
  * Each function cites the reference lines it restates.  Semantics notes are
  * in svo_oracle.h.
+ *
+ * How each part is pinned (tests/): the octree builder word for word against
+ * the reference's own octree.h (compiled from the checkout) and committed
+ * fixtures; bitcount and colorize exhaustively; the tile schedule and the
+ * hole compaction by closed forms and an independent formulation; fillhole2
+ * per rule; the projection by its zero-motion invariant and an independent
+ * key-min formulation; the camera by matrix identities.  orc_trace_pixel is
+ * PARTIALLY PINNED: full-depth hits against an FP64 voxel march, and LOD hits
+ * against the octree's node colours; the XOR pop level, the moment the LOD
+ * increments and the miss colour follow the cited lines with no independent
+ * check (the reference kernels need an OpenCL runtime this image lacks).
  */
 #include "svo_oracle.h"
 

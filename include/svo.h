@@ -120,6 +120,14 @@ int svo_raycast_rect(svo_ctx* ctx, const svo_frame_params* p, int add_x,
  * pixel of the next row or of plane 1, as in the reference; all picks are made
  * before any ray writes.  Uses the learned-fill gate lists as scratch. */
 int svo_raycast_quads(svo_ctx* ctx, const svo_frame_params* p, int add_x, int add_y);
+/* raycast_fillhole (kernel.cl:342-401; its call in raycast.h:205-220 is
+ * disabled): on plane 0, a pixel at a depth step (its nearest neighbour among
+ * the 2x2 up-left block is nearer by more than 2.5%) whose motion differs
+ * from that neighbour's becomes a hole, to be traced by the next hole pass.
+ * motion_x / motion_y: N ints each (the reference's mem_x / mem_y, which no
+ * reference kernel writes, so its own result is undefined); NULL = zero
+ * motion, under which no pixel changes.  Copied before the call returns. */
+int svo_fillhole_punch(svo_ctx* ctx, const int32_t* motion_x, const int32_t* motion_y);
 /* ocl_memcpy plane 0 -> dst_plane of screen and coords (raycast.h:394-405) */
 int svo_copy_cache(svo_ctx* ctx, int dst_plane);
 /* raycast_fillhole2 (kernel.cl:404-470), defined on a pre-filter snapshot of

@@ -78,6 +78,7 @@ _write = _sig("orc_writeids", None, _S)
 _holes = _sig("orc_raycast_holes", None, _S, _P)
 _rect = _sig("orc_raycast_rect", None, _S, _P, C.c_int, C.c_int, C.c_int, C.c_int)
 _quads = _sig("orc_raycast_quads", None, _S, _P, C.c_int, C.c_int)
+_punch = _sig("orc_fillhole_punch", None, _S, C.POINTER(C.c_int32), C.POINTER(C.c_int32))
 _copy = _sig("orc_copy_plane", None, _S, C.c_int)
 _fill_snap = _sig("orc_fillhole2_snapshot", None, _S, _u32p)
 _fill_inpl = _sig("orc_fillhole2_inplace", None, _S)
@@ -234,6 +235,21 @@ class Frame:
 
     def raycast_quads(self, p, ax, ay):
         _quads(C.byref(self.s), C.byref(p), ax, ay)
+
+    def fillhole_punch(self, motion_x=None, motion_y=None):
+        """raycast_fillhole on plane 0; motion arrays of N int32 or None (zero)."""
+        args = []
+        keep = []
+        for m in (motion_x, motion_y):
+            if m is None:
+                args.append(None)
+            else:
+                a = np.ascontiguousarray(m, dtype=np.int32).reshape(-1)
+                if a.size != self.N:
+                    raise ValueError("motion buffer must hold one int per pixel")
+                keep.append(a)
+                args.append(a.ctypes.data_as(C.POINTER(C.c_int32)))
+        _punch(C.byref(self.s), *args)
 
     def copy_plane(self, dst):
         _copy(C.byref(self.s), dst)

@@ -795,6 +795,41 @@ static uint32_t fill_pixel(const uint32_t* src, long ofs, int res_x, long limit)
     return col;
 }
 
+/* kernel.cl:342-401 raycast_fillhole.  Per pixel 3 <= x < res_x-4,
+ * 3 <= y < res_y-4 of plane 0 that is not a hole: z1 = the smallest camera
+ * depth (coordinate w) among the pixel and its neighbours (i, j) in
+ * {-1, 0}^2, the first strictly smaller in i-outer, j-inner order
+ * (:376-381); if none is smaller the pixel stays (the reference reads the
+ * motion buffers at an uninitialised offset first, :384-388, then returns);
+ * else it becomes a hole when |z1 - z0| > min(z0, z1) * Z_CONTINUITY * 4
+ * (Z_CONTINUITY = 0.00625, single precision) and the pixel's motion differs
+ * from that neighbour's (:390-392).  Each pixel reads only its own screen
+ * word, so the in-place update has no order dependence. */
+void orc_fillhole_punch(orc_state* s, const int32_t* mx, const int32_t* my)
+{
+    const int res_x = s->res_x, res_y = s->res_y;
+    for (int y = 3; y < res_y - 4; y++)
+        for (int x = 3; x < res_x - 4; x++) {
+            const long of = (long)y * res_x + x;
+            if (s->screen[of] == SCREEN_HOLE) continue;
+            const float z0 = s->coords[of * 4 + 3];
+            float z1 = z0;
+            int xi = 0, yj = 0;
+            for (int i = -1; i < 1; i++)
+                for (int j = -1; j < 1; j++) {
+                    const float z = s->coords[(of + (long)j * res_x + i) * 4 + 3];
+                    if (z < z1) { xi = i; yj = j; z1 = z; }
+                }
+            if (z0 <= z1) continue;
+            const long oij = of + (long)yj * res_x + xi;
+            const int dx0 = mx ? mx[of] : 0, dy0 = my ? my[of] : 0;
+            const int dx = mx ? mx[oij] : 0, dy = my ? my[oij] : 0;
+            const float zmin = z0 < z1 ? z0 : z1;
+            if (fabsf(z1 - z0) > zmin * 0.00625f * 4.0f && (abs(dx - dx0) > 0 || abs(dy - dy0) > 0))
+                s->screen[of] = SCREEN_HOLE;
+        }
+}
+
 /* Filter defined on a pre-filter snapshot of plane 0 (every hole reads
  * unfiltered neighbours). */
 void orc_fillhole2_snapshot(orc_state* s, const uint32_t* snap)

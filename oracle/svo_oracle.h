@@ -20,15 +20,17 @@
  * Parity pins (what checks each function; see tests/):
  *   orc_build_octree    word-for-word vs the reference octree.h compiled on
  *                       the host (oracle/_ref) + hand-derived 1/8-voxel arrays
- *   orc_trace_ray       dense-grid FP64 first-hit DDA on small scenes
- *                       (hit voxel + colour byte), hand-built trees for the
- *                       LOD colour words; trajectory quirks (XOR pop, LOD on
- *                       pop, miss colour) follow kernel.cl:114-214 and are
+ *   orc_trace_pixel     dense-grid FP64 first-hit DDA on small scenes
+ *                       (hit voxel + colour byte); LOD hits against the
+ *                       octree's node colours (creator voxel, first leaf);
+ *                       trajectory quirks (XOR pop, when the LOD increments,
+ *                       miss colour) follow kernel.cl:114-214 and are
  *                       PARTIALLY PINNED (no reference run of kernel.cl is
  *                       possible on this host: no CPU OpenCL runtime)
  *   orc_proj            zero-motion invariant + literal sequential semantics
  *   orc_counthole/...   closed forms + brute force
  *   orc_fillhole2_*     closed-form per-pixel rules
+ *   orc_fillhole_punch  closed-form per-pixel rules
  *   orc_colorize        exhaustive over the low byte
  */
 #ifndef SVO_ORACLE_H
@@ -114,6 +116,11 @@ void orc_fillhole2_inplace(orc_state* s);                         /* serial 16x1
 void orc_colorize(orc_state* s);                                  /* kernel.cl:944-974 */
 
 void orc_raycast_quads(orc_state* s, const orc_params* p, int add_x, int add_y); /* kernel.cl:696-843 */
+/* kernel.cl:342-401 raycast_fillhole (disabled in the reference): plane-0
+ * pixels at a depth discontinuity whose motion differs from the nearer
+ * neighbour's become holes.  motion_x/y: N ints each (the reference's
+ * mem_x/mem_y, which no kernel writes), or NULL for zero motion. */
+void orc_fillhole_punch(orc_state* s, const int32_t* motion_x, const int32_t* motion_y);
 void orc_tile_schedule(int res_x, int res_y, int frame, int* add_x, int* add_y,
                        int* ext_x, int* ext_y);                   /* raycast.h:363-366 */
 

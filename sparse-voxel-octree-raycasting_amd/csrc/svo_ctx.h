@@ -34,6 +34,7 @@ struct svo_ctx {
     uint32_t* d_image = nullptr;
     unsigned long long* d_keys = nullptr;
     uint32_t* d_snap = nullptr;
+    int32_t* d_motion = nullptr;  // svo_fillhole_punch: motion x (N) then y (N)
     uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [2..3] the graph's, [8..9] ray count (u64)
     // learned-fill gate: [0] CNN tile count, [16] gate ray count
     uint32_t* d_gate = nullptr;

@@ -681,6 +681,43 @@ __global__ __launch_bounds__(256) void k_cnn_input(View v, const View* views)
     V.cnn_coords[i] = V.coords[i];
 }
 
+// kernel.cl:342-401 raycast_fillhole (disabled in the reference, raycast.h:
+// 205-220): a plane-0 pixel 3 <= x < res_x-4, 3 <= y < res_y-4 becomes a hole
+// when its nearest neighbour among (i, j) in {-1,0}^2 (camera depth =
+// coordinate w; the first strictly nearer in i-outer, j-inner order) is
+// nearer by more than min(z0, z1) * Z_CONTINUITY * 4 (Z_CONTINUITY = 0.00625)
+// and moves differently (motion buffers mx/my, or none = zero motion).  A
+// pixel without a nearer neighbour stays (the reference first reads the
+// motion at an uninitialised offset there, then returns).  Each pixel reads
+// only its own screen word, so the in-place update is order-independent.
+__global__ __launch_bounds__(256) void k_fillhole_punch(View v, const int32_t* __restrict__ mx,
+                                                        const int32_t* __restrict__ my)
+{
+    const View& V = v;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= V.N) return;
+    const int x = (int)(i % (uint32_t)V.res_x), y = (int)(i / (uint32_t)V.res_x);
+    if (x < 3 || y < 3 || x >= V.res_x - 4 || y >= V.res_y - 4) return;
+    if (V.screen[i] == HOLE) return;
+    const float z0 = V.coords[i].w;
+    float z1 = z0;
+    int xi = 0, yj = 0;
+    for (int a = -1; a < 1; a++)
+        for (int b = -1; b < 1; b++) {
+            const float z = V.coords[(long)i + (long)b * V.res_x + a].w;
+            if (z < z1) {
+                xi = a;
+                yj = b;
+                z1 = z;
+            }
+        }
+    if (z0 <= z1) return;
+    const long oij = (long)i + (long)yj * V.res_x + xi;
+    const int dx0 = mx ? mx[i] : 0, dy0 = my ? my[i] : 0;
+    const int dx = mx ? mx[oij] : 0, dy = my ? my[oij] : 0;
+    if (fabsf(z1 - z0) > fminf(z0, z1) * 0.00625f * 4.0f && (dx != dx0 || dy != dy0)) V.screen[i] = HOLE;
+}
+
 __device__ __forceinline__ uint32_t rs(const uint32_t* __restrict__ s, long o, long limit)
 {
     return o < limit ? s[o] : HOLE;
@@ -923,6 +960,8 @@ static int free_frame(svo_ctx* c)
     (void)hipFree(c->d_gate_tiles);
     (void)hipFree(c->d_gate_rays);
     c->d_gate = c->d_gate_tiles = c->d_gate_rays = nullptr;
+    (void)hipFree(c->d_motion);
+    c->d_motion = nullptr;
     (void)hipFree(c->d_cnn_words);
     (void)hipFree(c->d_cnn_coords);
     (void)hipFree(c->d_cnn_out);
@@ -1583,6 +1622,26 @@ extern "C" int svo_raycast_quads(svo_ctx* c, const svo_frame_params* p, int add_
     set_rays(v, c->d_gate_rays, c->d_gate + GATE_RAYS, 0, 0, 0, 0, nullptr);
     v.edge_ok = 1;
     return launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq);
+}
+
+extern "C" int svo_fillhole_punch(svo_ctx* c, const int32_t* motion_x, const int32_t* motion_y)
+{
+    RC(need_frame(c));
+    HIPCHECK(hipSetDevice(c->device));
+    if ((motion_x || motion_y) && !c->d_motion && hipMalloc(&c->d_motion, c->N * 8) != hipSuccess) {
+        c->d_motion = nullptr;
+        return svo_fail(SVO_ERR_NOMEM, "motion buffers at %dx%d", c->res_x, c->res_y);
+    }
+    if (motion_x || motion_y) {  // the caller's arrays are read before returning
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (motion_x) HIPCHECK(hipMemcpy(c->d_motion, motion_x, c->N * 4, hipMemcpyHostToDevice));
+        if (motion_y) HIPCHECK(hipMemcpy(c->d_motion + c->N, motion_y, c->N * 4, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(k_fillhole_punch, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream,
+                       make_view(c, nullptr), motion_x ? c->d_motion : nullptr,
+                       motion_y ? c->d_motion + c->N : nullptr);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
 }
 
 extern "C" int svo_copy_cache(svo_ctx* c, int dst_plane)

@@ -95,6 +95,7 @@ _f = {
     "set_ray_spread": _sig("svo_set_ray_spread", C.c_int, _ctxp, C.c_int, C.c_int),
     "set_ray_lanes": _sig("svo_set_ray_lanes", C.c_int, _ctxp, C.c_int),
     "set_long_first": _sig("svo_set_long_first", C.c_int, _ctxp, C.c_int),
+    "fillhole_punch": _sig("svo_fillhole_punch", C.c_int, _ctxp, _vp, _vp),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
     "tile_schedule": _sig("svo_tile_schedule", C.c_int, C.c_int, C.c_int, C.c_int,
@@ -155,7 +156,7 @@ EXPORTED = [
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
-    "svo_set_ray_lanes", "svo_set_long_first", "svo_raycast_quads",
+    "svo_set_ray_lanes", "svo_set_long_first", "svo_raycast_quads", "svo_fillhole_punch",
 ]
 
 
@@ -387,6 +388,21 @@ class Context:
     def raycast_quads(self, p, add_x, add_y):
         """raycast_fine: one ray per 2x2 quad of the (res/4)-rounded grid."""
         _check(_f["raycast_quads"](self._h, C.byref(p), int(add_x), int(add_y)))
+
+    def fillhole_punch(self, motion_x=None, motion_y=None):
+        """raycast_fillhole on plane 0; motion arrays of N int32 each or None
+        (zero motion: no pixel changes)."""
+        ptrs, keep = [], []
+        for m in (motion_x, motion_y):
+            if m is None:
+                ptrs.append(None)
+                continue
+            a = np.ascontiguousarray(m, dtype=np.int32).reshape(-1)
+            if a.size != self.N:
+                raise ValueError("motion buffer must hold one int per pixel")
+            keep.append(a)
+            ptrs.append(a.ctypes.data_as(_vp))
+        _check(_f["fillhole_punch"](self._h, *ptrs))
 
     def copy_cache(self, dst_plane):
         _check(_f["copy_cache"](self._h, int(dst_plane)))

@@ -194,3 +194,37 @@ def test_quad_raycast_vs_oracle(ctx, scene1, res, add, frame):
     assert_equal_u32(ctx.screen(), o.screen, "screen planes")
     assert_equal_f32_bits(ctx.coords()[..., :3], o.coords[..., :3], "coords")
     o.close()
+
+
+@pytest.mark.parametrize("res", [(64, 48), (203, 133)])
+def test_fillhole_punch_vs_oracle(ctx, scene1, res):
+    """raycast_fillhole (kernel.cl:342-401) on a warped plane 0 with random
+    motion buffers (about half the pixels move): bit-exact with the oracle,
+    and zero motion changes nothing."""
+    words, root = scene1
+    rx, ry = res
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=8)
+    pr = path(CONFIG1, 4)
+    for f in range(3):
+        p = params(CONFIG1, pr, f)
+        ctx.frame_warp(p)
+        o.frame_warp(O.params_from(p))
+    ctx.sync()
+    _compare_all(ctx, o, f"{res} before punch")
+    before = ctx.screen(0).copy()
+    ctx.fillhole_punch(None, None)
+    ctx.sync()
+    assert_equal_u32(ctx.screen(0), before, "zero motion")
+    rng = np.random.default_rng(rx)
+    mx = (rng.integers(0, 2, (ry, rx)) * rng.integers(-3, 4, (ry, rx))).astype(np.int32)
+    my = (rng.integers(0, 2, (ry, rx)) * rng.integers(-3, 4, (ry, rx))).astype(np.int32)
+    ctx.fillhole_punch(mx, my)
+    o.fillhole_punch(mx, my)
+    ctx.sync()
+    assert_equal_u32(ctx.screen(), o.screen, f"{res} punched screen")
+    punched = int(((ctx.screen(0) == 0xFFFFFF00) & (before != 0xFFFFFF00)).sum())
+    print(f"{res}: {punched} pixels punched", flush=True)
+    assert punched > 0
+    o.close()

@@ -294,3 +294,75 @@ def test_quad_raycast_picks_and_flat_writes(res, add, frame):
         assert int(after[ofs]) == w
         assert np.array_equal(crd[ofs, :3].view(np.uint32), np.array(pos, np.float32).view(np.uint32))
     f.close()
+
+
+# ---- raycast_fillhole (kernel.cl:342-401), the disabled hole-punching pass:
+# each case is derived from the kernel's rule by hand
+def _punch_case(z0, nbr, motion_p, nbr_motion, hole=False, px=(8, 8), res=(16, 16)):
+    """Pixel px with depth z0; nbr: {(i, j): depth} for its (i, j) in {-1,0}^2
+    neighbours (others at z0); motions as (dx, dy).  Returns whether the pixel
+    became a hole."""
+    rx, ry = res
+    words, root = S.build_octree(np.array([[1, 1, 1]], np.uint16), np.array([0x11223344], np.uint32), 8)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=1)
+    o.screen[0][:] = 0xFF000000 + 7
+    o.coords[0][..., 3] = z0
+    mx = np.zeros((ry, rx), np.int32)
+    my = np.zeros((ry, rx), np.int32)
+    x, y = px
+    for (i, j), z in nbr.items():
+        o.coords[0][y + j, x + i, 3] = z
+        mx[y + j, x + i], my[y + j, x + i] = nbr_motion.get((i, j), (0, 0))
+    mx[y, x], my[y, x] = motion_p
+    if hole:
+        o.screen[0][y, x] = 0xFFFFFF00
+    o.fillhole_punch(mx, my)
+    after = o.screen[0].copy()
+    o.close()
+    return after[y, x] == 0xFFFFFF00
+
+
+def test_punch_depth_step_with_different_motion():
+    assert _punch_case(10.0, {(0, -1): 5.0}, (0, 0), {(0, -1): (1, 0)})
+    assert _punch_case(10.0, {(-1, 0): 5.0}, (0, 0), {(-1, 0): (0, -2)})
+
+
+def test_punch_needs_different_motion():
+    assert not _punch_case(10.0, {(0, -1): 5.0}, (3, 4), {(0, -1): (3, 4)})
+
+
+def test_punch_needs_a_depth_step():
+    # |z1 - z0| must exceed min(z0, z1) * 0.00625 * 4 = 0.2475 for z1 = 9.9
+    assert not _punch_case(10.0, {(-1, -1): 9.9}, (0, 0), {(-1, -1): (5, 5)})
+    assert _punch_case(10.0, {(-1, -1): 9.7}, (0, 0), {(-1, -1): (5, 5)})
+    # a farther neighbour is never the reference
+    assert not _punch_case(10.0, {(-1, -1): 20.0}, (0, 0), {(-1, -1): (5, 5)})
+
+
+def test_punch_picks_the_first_nearest_neighbour():
+    # i outer, j inner, strictly smaller: (-1,-1) comes before (0,-1) at equal depth
+    assert _punch_case(10.0, {(-1, -1): 5.0, (0, -1): 5.0}, (0, 0), {(-1, -1): (1, 0), (0, -1): (0, 0)})
+    assert not _punch_case(10.0, {(-1, -1): 5.0, (0, -1): 5.0}, (0, 0), {(-1, -1): (0, 0), (0, -1): (1, 0)})
+    # the nearest wins over an earlier, less near one
+    assert not _punch_case(10.0, {(-1, -1): 6.0, (-1, 0): 5.0}, (0, 0), {(-1, -1): (1, 0), (-1, 0): (0, 0)})
+
+
+def test_punch_skips_holes_and_borders():
+    assert _punch_case(10.0, {(0, -1): 5.0}, (0, 0), {(0, -1): (1, 0)}, hole=True)  # stays a hole
+    # 3 <= x < res_x - 4: x = 2 and x = 12 (res 16) are never touched
+    assert not _punch_case(10.0, {(0, -1): 5.0}, (0, 0), {(0, -1): (1, 0)}, px=(2, 8))
+    assert not _punch_case(10.0, {(0, -1): 5.0}, (0, 0), {(0, -1): (1, 0)}, px=(12, 8))
+    assert _punch_case(10.0, {(0, -1): 5.0}, (0, 0), {(0, -1): (1, 0)}, px=(11, 11))
+
+
+def test_punch_without_motion_is_a_no_op():
+    rx, ry = 40, 24
+    words, root = S.build_octree(np.array([[1, 1, 1]], np.uint16), np.array([0x11223344], np.uint32), 8)
+    o = O.Frame(rx, ry, words, root, 8, nthreads=1)
+    rng = np.random.default_rng(3)
+    o.screen[0][:] = rng.integers(0, 2**32, (ry, rx), dtype=np.uint64).astype(np.uint32)
+    o.coords[0][..., 3] = rng.uniform(1, 100, (ry, rx)).astype(np.float32)
+    before = o.screen[0].copy()
+    o.fillhole_punch(None, None)
+    assert np.array_equal(o.screen[0], before)
+    o.close()

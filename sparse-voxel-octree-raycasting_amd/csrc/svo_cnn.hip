@@ -191,10 +191,11 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ pred, co
 }
 
 // Input features of crop c (origin from `origins`, or tile list -> tile*16-4):
-// RGB of the colorized pre-filter word (plane 2), camera depth z/(z+64) and
-// the hole mask; zero outside the screen and at holes.  With `image`, also
-// the target RGB (full raycast) and the loss mask (holes inside the screen),
-// counting mask pixels into *mask_count.
+// RGB of the colorized word of the CNN input planes (plane 0 as the gate of a
+// learned frame leaves it), camera depth z/(z+64) and the hole mask; zero
+// outside the screen and at holes.  With `image`, also the target RGB (full
+// raycast) and the loss mask (pixels that are holes in `mask_words`, else in
+// the input, inside the screen), counting mask pixels into *mask_count.
 __device__ __forceinline__ void rgb_of(uint32_t word, float& r, float& g, float& b)
 {
     const int a = (int)word, c = a & 3;

@@ -612,7 +612,11 @@ __global__ __launch_bounds__(64) void k_writeids(View v, const View* views)
 }
 
 // exclusive scan of ids[0..S) into ids[S..2S), total -> ids[0]; one block per view
-__global__ __launch_bounds__(1024) void k_sumids(View v, const View* views)
+// One workgroup per view; each thread scans 8 consecutive block counts per
+// round (loads issued together), then the workgroup scans the thread totals.
+// With `first` (a single view in the frame graph), thread 0 also writes the
+// view's ray range [0, rays) and clears the next ray launch's queue counter.
+__global__ __launch_bounds__(1024) void k_sumids(View v, const View* views, uint32_t* first, uint32_t* zero)
 {
     VIEW_OF(blockIdx.z);
     uint32_t* ids = V.ids;
@@ -620,31 +624,45 @@ __global__ __launch_bounds__(1024) void k_sumids(View v, const View* views)
     __shared__ uint32_t wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < S; base += 1024) {
-        const uint32_t i = base + (uint32_t)tid;
-        const uint32_t x = (i < S) ? ids[i] : 0u;
-        uint32_t incl = x;
+    for (uint32_t base = 0; base < S; base += 8192) {
+        const uint32_t i0 = base + (uint32_t)tid * 8u;
+        uint32_t x[8], pre[8];
+        for (int k = 0; k < 8; k++) x[k] = (i0 + (uint32_t)k < S) ? ids[i0 + k] : 0u;
+        uint32_t tot = 0;
+        for (int k = 0; k < 8; k++) {
+            pre[k] = tot;  // exclusive within the thread
+            tot += x[k];
+        }
+        uint32_t incl = tot;
         for (int d = 1; d < 64; d <<= 1) {
-            uint32_t t = __shfl_up(incl, d);
+            const uint32_t t = __shfl_up(incl, d);
             if (lane >= d) incl += t;
         }
         if (lane == 63) wsum[w] = incl;
         __syncthreads();
         if (w == 0) {
-            uint32_t s = (lane < 16) ? wsum[lane] : 0u;
+            uint32_t q = (lane < 16) ? wsum[lane] : 0u;
             for (int d = 1; d < 16; d <<= 1) {
-                uint32_t t = __shfl_up(s, d);
-                if (lane >= d) s += t;
+                const uint32_t t = __shfl_up(q, d);
+                if (lane >= d) q += t;
             }
-            if (lane < 16) wsum[lane] = s;  // inclusive wave totals
+            if (lane < 16) wsum[lane] = q;  // inclusive wave totals
         }
         __syncthreads();
-        const uint32_t wpre = (w > 0) ? wsum[w - 1] : 0u;
-        if (i < S) ids[S + i] = carry + wpre + incl - x;
+        const uint32_t tpre = carry + ((w > 0) ? wsum[w - 1] : 0u) + incl - tot;
+        for (int k = 0; k < 8; k++)
+            if (i0 + (uint32_t)k < S) ids[S + i0 + k] = tpre + pre[k];
         carry += wsum[15];
         __syncthreads();
     }
-    if (tid == 0) ids[0] = carry;
+    if (tid == 0) {
+        ids[0] = carry;
+        if (first) {
+            first[0] = 0u;
+            first[1] = view_rays(V);
+        }
+        if (zero) *zero = 0u;
+    }
 }
 
 // first ray of each batched view (after their compaction), one thread
@@ -1237,9 +1255,9 @@ static int launch_count(const Launch& L, const View& v)
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }
-static int launch_sum(const Launch& L, const View& v)
+static int launch_sum(const Launch& L, const View& v, uint32_t* first = nullptr, uint32_t* zero = nullptr)
 {
-    hipLaunchKernelGGL(k_sumids, dim3(1, 1, (unsigned)L.n), dim3(1024), 0, L.stream, v, L.views);
+    hipLaunchKernelGGL(k_sumids, dim3(1, 1, (unsigned)L.n), dim3(1024), 0, L.stream, v, L.views, first, zero);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }
@@ -1346,7 +1364,7 @@ static int warp_front(svo_ctx* c, const svo_frame_params* p)
 
 // Capture the warp frame's launches (frames >= 2: no clear) into one graph.
 // Every kernel reads the frame's view from c->d_view; the ray launch uses its
-// own queue counters (d_queue[2..3]), reset by k_view_first inside the graph.
+// own queue counters (d_queue[2..3]), reset by k_sumids inside the graph.
 static int build_graph(svo_ctx* c)
 {
     if (!c->d_view) {
@@ -1365,12 +1383,8 @@ static int build_graph(svo_ctx* c)
     int seq = 0;
     int rc = launch_scatter_resolve(L, ph, 2, 1);
     if (!rc) rc = launch_count(L, ph);
-    if (!rc) rc = launch_sum(L, ph);
+    if (!rc) rc = launch_sum(L, ph, c->d_first, c->d_queue + 2);  // also the ray range and queue counter
     if (!rc) rc = launch_write(L, ph);
-    if (!rc) {
-        hipLaunchKernelGGL(k_view_first, dim3(1), dim3(1), 0, c->stream, dv, 1, c->d_first, c->d_queue + 2);
-        if (hipGetLastError() != hipSuccess) rc = svo_fail(SVO_ERR_HIP, "graph capture: k_view_first");
-    }
     if (!rc) rc = launch_rays_v(c, c->stream, ph, dv, c->d_first, 1, c->d_queue + 2, seq);
     if (!rc) rc = launch_copy(L, ph, 2);
     if (!rc) rc = launch_fill(L, ph, nullptr, 1, 1);

@@ -228,3 +228,48 @@ def test_fillhole_punch_vs_oracle(ctx, scene1, res):
     print(f"{res}: {punched} pixels punched", flush=True)
     assert punched > 0
     o.close()
+
+
+def _np_compaction_vec(scr, rx, ry):
+    """counthole / sumids / writeids by array operations (kernel.cl:234-340):
+    4 ids per all-hole 2x2 quad, blocks of 16x16 in raster order, quads in
+    row-major order inside a block."""
+    bx, by = rx // 16, ry // 16
+    s = scr[:by * 16, :bx * 16] == 0xFFFFFF00
+    q = s[0::2, 0::2] & s[0::2, 1::2] & s[1::2, 0::2] & s[1::2, 1::2]
+    qb = q.reshape(by, 8, bx, 8).transpose(0, 2, 1, 3).reshape(by * bx, 8, 8)
+    counts = (4 * qb.reshape(by * bx, 64).sum(1)).astype(np.uint32)
+    offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))[:-1]]).astype(np.uint32)
+    b, j, i = np.nonzero(qb)
+    v = ((b % bx) * 16 + 2 * i) | (((b // bx) * 16 + 2 * j) << 16)
+    ids = np.stack([v, v + 1, v + 1 + (1 << 16), v + (1 << 16)], 1).reshape(-1).astype(np.uint32)
+    return counts, offs, int(counts.sum(dtype=np.int64)), ids
+
+
+@pytest.mark.parametrize("res", [(203, 133), (2048, 1088), (8192, 8192)])
+def test_compaction_stages_vs_array_formulation(ctx, scene1, res):
+    """counthole, sumids and writeids on a random hole mask: block counts,
+    exclusive offsets (several scan rounds of 8192 blocks at the larger sizes),
+    total and the id list equal an independent array formulation."""
+    words, root = scene1
+    rx, ry = res
+    ctx.upload_octree(words, root, 8)
+    ctx.resize(rx, ry)
+    rng = np.random.default_rng(rx + ry)
+    scr = np.where(rng.random((ry, rx)) < 0.9, 0xFFFFFF00, rng.integers(0, 2**31, (ry, rx))).astype(np.uint32)
+    ctx.write(S.BUF_SCREEN, 0, scr)
+    print(f"compaction {rx}x{ry}", flush=True)
+    ctx.counthole()
+    ctx.sync()
+    Sn = (rx // 16) * (ry // 16)
+    counts, offs, H, ids = _np_compaction_vec(scr, rx, ry)
+    got = ctx.ids()
+    assert np.array_equal(got[:Sn], counts)
+    ctx.sumids()
+    ctx.writeids()
+    ctx.sync()
+    got = ctx.ids()
+    assert int(got[0]) == H
+    assert np.array_equal(got[Sn + 1:2 * Sn], offs[1:])  # ids[S] = 0 as well; ids[0] holds the total
+    assert int(got[Sn]) == 0
+    assert np.array_equal(got[2 * Sn:2 * Sn + H], ids)

@@ -261,6 +261,12 @@ int svo_set_ray_spread(svo_ctx* ctx, int group, int only_if_fits);
  * then places the rays of a wave 1/lanes of the list apart.  Results do not
  * depend on it. */
 int svo_set_ray_lanes(svo_ctx* ctx, int lanes);
+/* Shape of the ray blocks in which rectangle rays (refresh tile, full
+ * frame) are handed out, one block per wave: width 8 (8x8, default), 16
+ * (16x4), 32 (32x2) or 64 (64x1) pixels.  The reference's raycast_fine_2
+ * runs 16x16 work-groups (raycast.h:366); here the shape only orders the
+ * hand-out, so results do not depend on it. */
+int svo_set_ray_block(svo_ctx* ctx, int width);
 /* Full frames (svo_frame_full): the 8x8 screen blocks in which the previous
  * full frame had a ray of at least long_trips traversal trips are raycast
  * first (as a pixel list ahead of the raster-order blocks), so the long

@@ -58,6 +58,7 @@ struct svo_ctx {
     int spread_shift = 0;    // ray hand-out: log2 of the spread group (6 = list order)
     int spread_fit = 1;      // spread only launches whose rays all fit in the resident lanes
     int lanes_shift = 6;     // log2 of the lanes per wave that take rays
+    int blk_shift = 3;       // rectangle rays in blocks of 2^blk_shift x 2^(6-blk_shift) (8x8)
     // full frames: 8x8 blocks whose previous full frame had a ray of at least
     // long_trips trips are raycast first (0 = raster order only)
     int long_trips = 1024;

@@ -128,8 +128,9 @@ struct View {
     float cx0, cx1, cx2, cy0, cy1, cy2, cz0, cz1, cz2;  // ray columns
     float fovx, fovy;
     int plane_lo, n_planes;                // proj source planes
-    int add_x, add_y, ext_x, ext_y, tiles_x;  // refresh rectangle
+    int add_x, add_y, ext_x, ext_y, tiles_x;  // refresh rectangle, tiles_x blocks per row
     uint32_t n_rect;
+    int blk_shift;  // its rays are handed out in blocks of 2^blk_shift x 2^(6-blk_shift) pixels
     uint32_t* ray_image;                   // colorize ray words straight into this, or null
     int edge_ok;  // list pixels may be (res_x, y) or (x, res_y) (quad rays)
     const uint32_t* rect_skip;   // per 8x8 block: nonzero = raycast from the list instead, or null
@@ -230,14 +231,15 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
         if (w == QUAD_NONE) return false;
         x = (int)(w & 0xffffu);
         y = (int)(w >> 16);
-        r.blk = V.block_long ? (uint32_t)((y >> 3) * V.tiles_x + (x >> 3)) : NO_BLOCK;
+        r.blk = V.block_long ? (uint32_t)((y >> (6 - V.blk_shift)) * V.tiles_x + (x >> V.blk_shift)) : NO_BLOCK;
     } else {
         k -= H + H2;
         const uint32_t t = k >> 6, w = k & 63u;
         if (V.rect_skip && V.rect_skip[t]) return false;
         r.blk = t;
         int tx = (int)(t % (uint32_t)V.tiles_x), ty = (int)(t / (uint32_t)V.tiles_x);
-        int lx = tx * 8 + (int)(w & 7u), ly = ty * 8 + (int)(w >> 3);
+        const int bs = V.blk_shift;
+        int lx = (tx << bs) + (int)(w & ((1u << bs) - 1u)), ly = (ty << (6 - bs)) + (int)(w >> bs);
         if (lx >= V.ext_x || ly >= V.ext_y) return false;
         x = V.add_x + lx;
         y = V.add_y + ly;
@@ -818,7 +820,7 @@ __global__ __launch_bounds__(256) void k_quad_select(View v, int add_x, int add_
     list[g] = (uint32_t)idx | ((uint32_t)idy << 16);
 }
 
-// Long blocks of a full frame go first.  An 8x8 block whose previous full
+// Long blocks of a full frame go first.  A ray block (8x8 by default) whose previous full
 // frame had a ray of at least long_trips trips (flag in `prev`) has its
 // on-screen pixels written to the front list (count in pixels, at most the
 // screen) and the rectangle pass skips it; the flags for this frame (`cur`)
@@ -830,6 +832,7 @@ __global__ __launch_bounds__(256) void k_quad_select(View v, int add_x, int add_
 __global__ __launch_bounds__(256) void k_long_blocks(const uint32_t* __restrict__ prev, uint32_t* __restrict__ cur,
                                                      uint32_t* __restrict__ list, uint32_t* __restrict__ count,
                                                      uint32_t* __restrict__ next_count, uint32_t n, uint32_t tiles_x,
+                                                     uint32_t bshift,
                                                      uint32_t res_x, uint32_t res_y)
 {
     __shared__ uint32_t s_wave[4], s_base;
@@ -838,8 +841,9 @@ __global__ __launch_bounds__(256) void k_long_blocks(const uint32_t* __restrict_
     if (i == 0) *next_count = 0u;
     const bool hv = i < n && prev && prev[i] != 0u;
     if (i < n) cur[i] = 0u;
-    const uint32_t x0 = (i % tiles_x) * 8u, y0 = (i / tiles_x) * 8u;
-    const uint32_t w = hv ? min(8u, res_x - x0) : 0u, h = hv ? min(8u, res_y - y0) : 0u;
+    const uint32_t bw = 1u << bshift, bh = 64u >> bshift;
+    const uint32_t x0 = (i % tiles_x) * bw, y0 = (i / tiles_x) * bh;
+    const uint32_t w = hv ? min(bw, res_x - x0) : 0u, h = hv ? min(bh, res_y - y0) : 0u;
     const uint32_t mine = w * h;
     uint32_t incl = mine;  // inclusive scan of pixel counts over the wave
     for (int d = 1; d < 64; d <<= 1) {
@@ -1085,6 +1089,15 @@ extern "C" int svo_upload_octree(svo_ctx* c, const uint32_t* words, size_t n_wor
     return SVO_OK;
 }
 
+// the most ray blocks a full screen has over the block shapes 8x8 .. 64x1
+static size_t max_ray_blocks(int res_x, int res_y)
+{
+    size_t m = 0;
+    for (int s = 3; s <= 6; s++)
+        m = std::max(m, (size_t)((res_x + (1 << s) - 1) >> s) * (size_t)((res_y + (64 >> s) - 1) / (64 >> s)));
+    return m;
+}
+
 extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
 {
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
@@ -1102,7 +1115,7 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
         hipMalloc(&c->d_gate, 128) != hipSuccess ||
         hipMalloc(&c->d_gate_tiles, ((size_t)(res_x + 15) / 16) * ((res_y + 15) / 16) * 4) != hipSuccess ||
         hipMalloc(&c->d_gate_rays, N * 4) != hipSuccess ||
-        hipMalloc(&c->d_block_long, ((size_t)(res_x + 7) / 8) * ((res_y + 7) / 8) * 8) != hipSuccess ||
+        hipMalloc(&c->d_block_long, max_ray_blocks(res_x, res_y) * 8) != hipSuccess ||
         hipMalloc(&c->d_long_list, N * 4) != hipSuccess ||
         hipMalloc(&c->d_long_count, 512) != hipSuccess) {
         free_frame(c);
@@ -1175,6 +1188,7 @@ static View make_view(svo_ctx* c, const svo_frame_params* p)
     v.N = (uint32_t)c->N;
     v.S = (uint32_t)c->S;
     v.tiles_x = 1;
+    v.blk_shift = c->blk_shift;
     v.gate = c->d_gate;
     v.gate_tiles = c->d_gate_tiles;
     v.gate_rays = c->d_gate_rays;
@@ -1203,8 +1217,9 @@ static void set_rays(View& v, const uint32_t* list, const uint32_t* list_count, 
     v.add_y = add_y;
     v.ext_x = std::max(0, ext_x);
     v.ext_y = std::max(0, ext_y);
-    v.tiles_x = std::max(1, (v.ext_x + 7) / 8);
-    v.n_rect = (uint32_t)(((v.ext_x + 7) / 8) * ((v.ext_y + 7) / 8) * 64);
+    const int bw = 1 << v.blk_shift, bh = 64 >> v.blk_shift;
+    v.tiles_x = std::max(1, (v.ext_x + bw - 1) / bw);
+    v.n_rect = (uint32_t)(((v.ext_x + bw - 1) / bw) * ((v.ext_y + bh - 1) / bh) * 64);
     v.ray_image = ray_image;
 }
 
@@ -1531,13 +1546,14 @@ extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
     mark(c, ST_RAYS);
     if (c->long_trips) {
         // the first full frame after a resize, upload or setting change only collects flags
-        const uint32_t nb = (uint32_t)(((c->res_x + 7) / 8) * ((c->res_y + 7) / 8));
+        const uint32_t nb = v.n_rect >> 6;  // the screen's ray blocks
         uint32_t* cur = c->d_block_long + (size_t)c->long_parity * nb;
         const uint32_t* prev = c->long_valid ? c->d_block_long + (size_t)(c->long_parity ^ 1) * nb : nullptr;
         uint32_t* count = c->d_long_count + 64 * c->long_parity;  // zero: cleared by the previous full frame
         uint32_t* next_count = c->d_long_count + 64 * (c->long_parity ^ 1);
         hipLaunchKernelGGL(k_long_blocks, dim3((nb + 255) / 256), dim3(256), 0, c->stream, prev, cur, c->d_long_list,
-                           count, next_count, nb, (uint32_t)v.tiles_x, (uint32_t)c->res_x, (uint32_t)c->res_y);
+                           count, next_count, nb, (uint32_t)v.tiles_x, (uint32_t)v.blk_shift, (uint32_t)c->res_x,
+                           (uint32_t)c->res_y);
         HIPCHECK(hipGetLastError());
         v.list = c->d_long_list;
         v.list_count = count;
@@ -1549,6 +1565,19 @@ extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
     }
     RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
     return frame_end(c);
+}
+
+extern "C" int svo_set_ray_block(svo_ctx* c, int width)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    int shift = 3;
+    while (shift <= 6 && (1 << shift) != width) shift++;
+    if (shift > 6) return svo_fail(SVO_ERR_ARG, "ray block width %d is not 8, 16, 32 or 64", width);
+    if (c->blk_shift != shift) {
+        c->blk_shift = shift;
+        c->long_valid = false;  // the long-block flags index blocks of the old shape
+    }
+    return SVO_OK;
 }
 
 extern "C" int svo_set_long_first(svo_ctx* c, int long_trips)

@@ -95,6 +95,7 @@ _f = {
     "set_ray_spread": _sig("svo_set_ray_spread", C.c_int, _ctxp, C.c_int, C.c_int),
     "set_ray_lanes": _sig("svo_set_ray_lanes", C.c_int, _ctxp, C.c_int),
     "set_long_first": _sig("svo_set_long_first", C.c_int, _ctxp, C.c_int),
+    "set_ray_block": _sig("svo_set_ray_block", C.c_int, _ctxp, C.c_int),
     "fillhole_punch": _sig("svo_fillhole_punch", C.c_int, _ctxp, _vp, _vp),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
@@ -156,7 +157,7 @@ EXPORTED = [
     "svo_frame_time", "svo_set_graph", "svo_rle4_load", "svo_rle4_save", "svo_image_save_ppm",
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
-    "svo_set_ray_lanes", "svo_set_long_first", "svo_raycast_quads", "svo_fillhole_punch",
+    "svo_set_ray_lanes", "svo_set_long_first", "svo_set_ray_block", "svo_raycast_quads", "svo_fillhole_punch",
 ]
 
 
@@ -434,6 +435,11 @@ class Context:
     def set_ray_lanes(self, lanes=64):
         """Lanes per wave that take rays (64 = all)."""
         _check(_f["set_ray_lanes"](self._h, int(lanes)))
+
+    def set_ray_block(self, width=8):
+        """Rectangle rays are handed out in blocks of width x 64/width pixels
+        (8, 16, 32 or 64).  Results are the same for every shape."""
+        _check(_f["set_ray_block"](self._h, int(width)))
 
     def set_long_first(self, long_trips=1024):
         """Full frames raycast first the 8x8 blocks whose previous full frame

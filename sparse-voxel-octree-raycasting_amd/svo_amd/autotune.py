@@ -5,10 +5,10 @@ spread across waves, 64 = list order; lanes per wave, svo_set_ray_lanes, is
 applied when a setting names it but is not searched by default: fewer lanes
 helped short windows and hurt long ones) and, through svo_set_tuning, the work-queue refill threshold (idle
 lanes per refill), persistent workgroups per CU, traversal steps between
-refill checks, and the LDS node-cache depth (octree words of the
-breadth-first top levels kept in LDS); full frames also search the trip
-count that puts a screen block on the long-rays-first list
-(svo_set_long_first).  Every candidate replays the same frame window from the same starting
+refill checks, the LDS node-cache depth (octree words of the breadth-first
+top levels kept in LDS) and the shape of the blocks in which rectangle rays
+are handed out (svo_set_ray_block); full frames also search the trip count
+that puts a screen block on the long-rays-first list (svo_set_long_first).  Every candidate replays the same frame window from the same starting
 state (the window starts with the two clearing frames), so candidates see
 identical work; the score is the mean GPU time per frame over the timed
 frames (frames/s follows the mean), measured with the library's per-frame
@@ -27,6 +27,7 @@ DEFAULT_SPACE = {
     "wg_per_cu": [4, 8, 16],
     "iters": [8, 16, 64],
     "lds_cache_words": [0, 96, 736, 2048, 4096],
+    "ray_block": [16, 8, 32, 64],  # the second entry starts the descent: 8x8
 }
 
 
@@ -82,6 +83,8 @@ def apply(ctx, cfg):
         ctx.set_ray_lanes(int(cfg["lanes"]))
     if "long_trips" in cfg:
         ctx.set_long_first(int(cfg["long_trips"]))
+    if "ray_block" in cfg:
+        ctx.set_ray_block(int(cfg["ray_block"]))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):

@@ -111,13 +111,15 @@ def test_warp_graph_vs_direct_launches(scene1):
         d.close()
 
 
-@pytest.mark.parametrize("spread,refill,wg,iters,cache,lanes", [(64, 16, 4, 64, 0, 64), (1, 48, 8, 8, 736, 32),
-                                                                (4, 1, 1, 1, 4096, 16), (16, 64, 32, 4096, 96, 8),
-                                                                (1, 8, 2, 16, 0, 1)])
-def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, cache, lanes):
+@pytest.mark.parametrize("spread,refill,wg,iters,cache,lanes,block", [(64, 16, 4, 64, 0, 64, 16),
+                                                                      (1, 48, 8, 8, 736, 32, 64),
+                                                                      (4, 1, 1, 1, 4096, 16, 32),
+                                                                      (16, 64, 32, 4096, 96, 8, 8),
+                                                                      (1, 8, 2, 16, 0, 1, 64)])
+def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, cache, lanes, block):
     """Ray hand-out order, refill threshold, workgroups per CU, trips per
-    refill check, the LDS node cache and lanes per wave change only the
-    schedule."""
+    refill check, the LDS node cache, lanes per wave and the ray block shape
+    change only the schedule."""
     words, root = scene1
     rx, ry = CONFIG1["res"]
     pr = path(CONFIG1, 6)
@@ -129,9 +131,10 @@ def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, c
         c.set_tuning(refill, wg, iters, cache)
         c.set_ray_spread(spread, only_if_fits=(spread != 16))
         c.set_ray_lanes(lanes)
+        c.set_ray_block(block)
         for f in range(6):
             p = params(CONFIG1, pr, f)
-            print(f"knobs {spread, refill, wg, iters, cache, lanes} frame {f}", flush=True)
+            print(f"knobs {spread, refill, wg, iters, cache, lanes, block} frame {f}", flush=True)
             c.frame_warp(p)
             o.frame_warp(O.params_from(p))
             c.sync()
@@ -147,8 +150,8 @@ def test_launch_knobs_do_not_change_results(scene1, spread, refill, wg, iters, c
         o.close()
 
 
-@pytest.mark.parametrize("long_trips", [1, 16, 512])
-def test_full_frame_long_first_bitexact(scene1, long_trips):
+@pytest.mark.parametrize("long_trips,block", [(1, 8), (16, 8), (512, 8), (16, 32), (1, 64)])
+def test_full_frame_long_first_bitexact(scene1, long_trips, block):
     """Full frames that raycast the previous frame's long-ray blocks first
     (1: every block, so the front list is the whole screen): identical to the
     oracle, with the setting switched off for one frame, across a warp frame
@@ -158,13 +161,14 @@ def test_full_frame_long_first_bitexact(scene1, long_trips):
     c = S.Context(0)
     try:
         c.upload_octree(words, root, 8)
+        c.set_ray_block(block)
         for rx, ry in ((CONFIG1["res"][0], CONFIG1["res"][1]), (203, 133)):
             c.resize(rx, ry)
             o = O.Frame(rx, ry, words, root, 8, nthreads=8)
             for f in range(6):
                 p = params(CONFIG1, pr, f)
                 c.set_long_first(0 if f == 4 else long_trips)
-                print(f"long first {long_trips} {rx}x{ry} frame {f}", flush=True)
+                print(f"long first {long_trips} block {block} {rx}x{ry} frame {f}", flush=True)
                 if f == 2 and rx % 8 == 0:
                     c.frame_warp(p)  # warp state in between (sizes the warp tests cover)
                 c.frame_full(p)

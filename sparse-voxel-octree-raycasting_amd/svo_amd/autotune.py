@@ -24,7 +24,7 @@ import time
 DEFAULT_SPACE = {
     "spread": [1, 64],
     "refill": [8, 16, 32, 48],
-    "wg_per_cu": [4, 8, 16],
+    "wg_per_cu": [2, 4, 3, 8, 16],  # the second entry starts the descent
     "iters": [8, 16, 64],
     "lds_cache_words": [0, 96, 736, 2048, 4096],
     "ray_block": [16, 8, 32, 64],  # the second entry starts the descent: 8x8

@@ -180,9 +180,11 @@ def run_gpu(args, name):
             lw = S.Context(0)
             lw.upload_octree(words, root, depth)
             lw.resize(rx, ry)
-            if "tuned_warp" in res:
-                t = res["tuned_warp"]
-                A.apply(lw, t)
+            if args.tune:  # learned frames trace ~2x the rays of warp frames: their own knobs
+                tp = [W.frame_params(name, rows, f) for f in range(min(total, 70))]
+                best, ms, _ = A.tune(lw, tp, warmup=10, mode="learned", cnn=cnn, max_cnn_holes=args.max_cnn_holes)
+                res["tuned_learned"] = dict(best, ms=round(ms, 4))
+                log(f"[bench] autotune learned: {best} -> {ms:.4f} ms/frame")
             for f in range(args.warmup):
                 lw.frame_warp_learned(cnn, params[f], args.max_cnn_holes)
             lw.sync()
@@ -195,6 +197,8 @@ def run_gpu(args, name):
             # per-stage times of the same frames on fresh state: the CNN stage
             # is the fused tile kernel; its work is the gate's CNN tile count
             lw = fresh()
+            if "tuned_learned" in res:
+                A.apply(lw, res["tuned_learned"])
             lw.enable_timing(mode=1)
             cnn_ms, tiles, gate_rays, lstage = [], [], [], {}
             for f in range(args.warmup, min(total, args.warmup + 40)):
@@ -503,7 +507,7 @@ def main():
         "maxabs_warp_vs_full": max(g["maxabs"]) if g["maxabs"] else None,
         "identical_pixels_warp_vs_full": round(float(np.mean(g["same"])), 4) if g["same"] else None,
         "stage_ms": {k: round(v, 4) for k, v in g["stage_mean"].items()},
-        "autotuned": {k: g[k] for k in ("tuned_warp", "tuned_full") if k in g},
+        "autotuned": {k: g[k] for k in ("tuned_warp", "tuned_full", "tuned_learned") if k in g},
         "learned_fill": None if "learned_fps" not in g else {
             "fps": round(g["learned_fps"], 2), "speedup_vs_full": round(g["learned_fps"] / g["full_fps"], 3),
             "psnr_vs_full_db": g["psnr_modes"], "psnr_hole_pixels_db": g["psnr_hole_pixels"],

@@ -31,15 +31,26 @@ DEFAULT_SPACE = {
 }
 
 
-def _frame_time(ctx, params, warmup, mode):
+def _frame_fn(ctx, mode, cnn, max_cnn_holes):
+    if mode == "warp":
+        return ctx.frame_warp
+    if mode == "full":
+        return ctx.frame_full
+    if mode == "learned":
+        return lambda p: ctx.frame_warp_learned(cnn, p, max_cnn_holes)
+    raise ValueError(f"unknown frame mode {mode!r}")
+
+
+def _frame_time(ctx, params, warmup, mode, cnn=None, max_cnn_holes=64):
+    frame = _frame_fn(ctx, mode, cnn, max_cnn_holes)
     ctx.enable_timing(False)
     for p in params[:warmup]:
-        (ctx.frame_warp if mode == "warp" else ctx.frame_full)(p)
+        frame(p)
     ctx.sync()
     ctx.enable_timing(mode=2)  # event pair per frame, graph launches included
     ts = []
     for p in params[warmup:]:
-        (ctx.frame_warp if mode == "warp" else ctx.frame_full)(p)
+        frame(p)
         ctx.sync()
         ts.append(ctx.frame_time())
     ctx.enable_timing(False)
@@ -87,10 +98,11 @@ def apply(ctx, cfg):
         ctx.set_ray_block(int(cfg["ray_block"]))
 
 
-def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None):
+def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None, cnn=None, max_cnn_holes=64):
     """Pick the fastest knob setting for this context's scene/resolution.
 
-    params: FrameParams list (params[0] must have frame 0).  With
+    params: FrameParams list (params[0] must have frame 0).  mode: "warp",
+    "full" or "learned" (svo_frame_warp_learned with `cnn`).  With
     coordinate=True the knobs are tuned one at a time (coordinate descent,
     two sweeps); otherwise the full grid is measured.  Applies and returns
     (best_setting, best_ms, trials)."""
@@ -101,7 +113,7 @@ def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=No
 
     def measure(cfg):
         apply(ctx, cfg)
-        ms = _frame_time(ctx, params, warmup, mode)
+        ms = _frame_time(ctx, params, warmup, mode, cnn, max_cnn_holes)
         trials.append((dict(cfg), ms))
         if log:
             log(f"[autotune] {cfg} -> {ms:.4f} ms")

@@ -65,3 +65,36 @@ def test_tuned_context_stays_bitexact(mode):
     finally:
         c.close()
         o.close()
+
+
+@pytest.mark.gpu
+def test_tuned_learned_frames_equal_default_knobs():
+    """Learned-frame tuning (gate + CNN frames) picks knobs that change only
+    the schedule: the tuned context's frames equal a default context's."""
+    from svo_amd import learned as L
+    words, root = scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
+    rx, ry = CONFIG1["res"]
+    pr = path(CONFIG1, 10)
+    ps = [params(CONFIG1, pr, f) for f in range(10)]
+    space = {"refill": [8, 32], "wg_per_cu": [2, 4], "iters": [16, 64], "lds_cache_words": [0, 736]}
+    cnn = S.Cnn(0)
+    L.load(cnn, L.WEIGHTS)
+    a, b = S.Context(0), S.Context(0)
+    try:
+        for c in (a, b):
+            c.upload_octree(words, root, 8)
+            c.resize(rx, ry)
+        best, ms, trials = A.tune(a, ps, warmup=4, mode="learned", space=space, cnn=cnn, max_cnn_holes=64)
+        print("tuned learned", best, ms, flush=True)
+        assert all(best[k] in space[k] for k in space) and ms > 0
+        for f in range(6):
+            a.frame_warp_learned(cnn, ps[f], 64)
+            b.frame_warp_learned(cnn, ps[f], 64)
+            a.sync()
+            b.sync()
+            assert_equal_u32(a.screen(), b.screen(), f"frame {f} screen")
+            assert_equal_u32(a.image(), b.image(), f"frame {f} image")
+    finally:
+        a.close()
+        b.close()
+        cnn.close()

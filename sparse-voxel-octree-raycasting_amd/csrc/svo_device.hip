@@ -295,8 +295,11 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
 // instruction per 4 cycles, so the trip is written for a short instruction
 // path: selects instead of exec-mask branches, the pop test as one compare
 // (floor(log2 m) >= rek  <=>  m >= 2^rek, m > 0), and the child fetch of
-// fetchChildOffset (kernel.cl:32-62) as a single word read that is issued
-// before the DDA step of the other lanes and consumed after it.
+// fetchChildOffset (kernel.cl:32-62) as a single word read.  The compiler
+// merges the two descend regions, so the read completes before the DDA step
+// of the other lanes runs; a version that kept them apart (the wait after
+// the step) was slower, the extra branch region costing every lone ray more
+// than the overlap saved divergent waves.
 template <bool kCache>
 __device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, OctBuf ob, uint32_t* stk, const uint32_t* cache,
                                          int tid)

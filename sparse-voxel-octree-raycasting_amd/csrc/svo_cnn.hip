@@ -37,6 +37,16 @@ __device__ __forceinline__ bf16 f2bf(float f)
     return (bf16)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
+// relu(acc + bias) of four channels as four bf16 (channel 0 in the low half)
+__device__ __forceinline__ uint2 relu_bf16x4(f4v acc, f4v bias)
+{
+    uint32_t h[4];
+    for (int i = 0; i < 4; i++) {
+        const float v = acc[i] + bias[i];
+        h[i] = (uint32_t)f2bf(v > 0.f ? v : 0.f);
+    }
+    return make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+}
 
 constexpr uint32_t HOLE_WORD = 0xffffff00u;
 constexpr int TILE = 8;          // output tile edge per wave
@@ -353,6 +363,11 @@ struct TileArgs {
     const float* b[4];
 };
 
+// Both tile layers compute the transposed product D[channel][pixel] (the
+// weights are the MFMA's A operand, the pixels its B operand), so a lane
+// ends with four consecutive output channels of one pixel and stores them as
+// one 8-byte write.
+
 // layers 2-4: in = swizzled 32-channel pixels of a win x win region, Wl =
 // swizzled [9][COUT][32] weights
 template <int COUT, bool LAST>
@@ -367,6 +382,10 @@ __device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out,
     bf8v bfr[9][NT];
     for (int tap = 0; tap < 9; tap++)
         for (int nn = 0; nn < NT; nn++) bfr[tap][nn] = *(const bf8v*)(Wl + swz(tap * COUT + nn * 16 + row, kg));
+    // this lane's output channels: nn * 16 + kg * 4 + i
+    f4v bv[NT];
+    for (int nn = 0; nn < NT; nn++)
+        for (int i = 0; i < 4; i++) bv[nn][i] = bias[nn * 16 + kg * 4 + i];
     for (int m0 = wave * 2; m0 * 16 < npix; m0 += 2 * nw) {
         f4v acc[2][NT];
         int q0[2];
@@ -380,24 +399,22 @@ __device__ __forceinline__ void tile_layer32(const bf16* in, int win, bf16* out,
             for (int m = 0; m < 2; m++) {
                 const bf8v afr = *(const bf8v*)(in + swz(q0[m] + dq, kg));
                 for (int nn = 0; nn < NT; nn++)
-                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[tap][nn], acc[m][nn], 0, 0, 0);
+                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[tap][nn], afr, acc[m][nn], 0, 0, 0);
             }
         }
-        for (int m = 0; m < 2; m++)
+        for (int m = 0; m < 2; m++) {
+            const int q = (m0 + m) * 16 + row;
+            if (q >= npix) continue;
             for (int nn = 0; nn < NT; nn++) {
-                const int co = nn * 16 + row;
-                const float bv = bias[co];
-                for (int i = 0; i < 4; i++) {
-                    const int q = (m0 + m) * 16 + kg * 4 + i;
-                    if (q >= npix) continue;
-                    const float v = acc[m][nn][i] + bv;
-                    if (LAST) {
-                        if (co < 3) outf[q * 3 + co] = v;
-                    } else {
-                        out[swz(q, co >> 3) + (co & 7)] = f2bf(v > 0.f ? v : 0.f);
-                    }
+                const int co0 = nn * 16 + kg * 4;
+                if (LAST) {
+                    if (co0 == 0)
+                        for (int i = 0; i < 3; i++) outf[q * 3 + i] = acc[m][nn][i] + bv[nn][i];
+                } else {
+                    *(uint2*)(out + swz(q, co0 >> 3) + (co0 & 7)) = relu_bf16x4(acc[m][nn], bv[nn]);
                 }
             }
+        }
     }
 }
 
@@ -427,6 +444,9 @@ __device__ __forceinline__ void tile_layer1(const bf16* P, bf16* Q, const bf16* 
     s4v b1[9][2];
     for (int tap = 0; tap < 9; tap++)
         for (int nn = 0; nn < 2; nn++) b1[tap][nn] = *(const s4v*)(Wl + (tap * 32 + nn * 16 + row) * 16 + kg * 4);
+    f4v bv[2];
+    for (int nn = 0; nn < 2; nn++)
+        for (int i = 0; i < 4; i++) bv[nn][i] = bias[nn * 16 + kg * 4 + i];
     for (int m0 = wave * 2; m0 * 16 < 484; m0 += 2 * nw) {
         f4v acc[2][2];
         int q0[2];
@@ -440,20 +460,17 @@ __device__ __forceinline__ void tile_layer1(const bf16* P, bf16* Q, const bf16* 
             for (int m = 0; m < 2; m++) {
                 const s4v afr = *(const s4v*)(P + (q0[m] + dq) * 16 + kg * 4);
                 for (int nn = 0; nn < 2; nn++)
-                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, b1[tap][nn], acc[m][nn], 0, 0, 0);
+                    acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b1[tap][nn], afr, acc[m][nn], 0, 0, 0);
             }
         }
-        for (int m = 0; m < 2; m++)
+        for (int m = 0; m < 2; m++) {
+            const int q = (m0 + m) * 16 + row;
+            if (q >= 484) continue;
             for (int nn = 0; nn < 2; nn++) {
-                const int co = nn * 16 + row;
-                const float bv = bias[co];
-                for (int i = 0; i < 4; i++) {
-                    const int q = (m0 + m) * 16 + kg * 4 + i;
-                    if (q >= 484) continue;
-                    const float v = acc[m][nn][i] + bv;
-                    Q[swz(q, co >> 3) + (co & 7)] = f2bf(v > 0.f ? v : 0.f);
-                }
+                const int co0 = nn * 16 + kg * 4;
+                *(uint2*)(Q + swz(q, co0 >> 3) + (co0 & 7)) = relu_bf16x4(acc[m][nn], bv[nn]);
             }
+        }
     }
 }
 

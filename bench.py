@@ -124,6 +124,20 @@ def run_gpu(args, name):
         c.sync()
         return c
 
+    # ---- pass A2: the same frames with the colorized image copied to host
+    # memory after each (svo_read, blocking): the rate a caller that needs the
+    # pixels on the host sees, PCIe copy included
+    if args.legs == "all":
+        wr = fresh()
+        t0 = time.perf_counter()
+        for f in range(args.warmup, total):
+            wr.frame_warp(params[f])
+            wr.image()
+        t_readback = time.perf_counter() - t0
+        wr.close()
+        res["warp_fps_image_to_host"] = args.steps / t_readback
+        log(f"[bench] warp + image to host: {args.steps} frames in {t_readback * 1e3:.1f} ms")
+
     warp2 = fresh()
     warp2.enable_timing(mode=2)
     full.enable_timing(mode=2)
@@ -498,6 +512,7 @@ def main():
                                "warp+fill frames along the seeded flythrough",
                    "res": [wl["rx"], wl["ry"]], "octree_words": wl["words"]},
         "warp_p50_ms": round(pct(g["warp_ms"], 50), 4), "warp_p99_ms": round(pct(g["warp_ms"], 99), 4),
+        "warp_fps_image_to_host": round(g["warp_fps_image_to_host"], 2) if "warp_fps_image_to_host" in g else None,
         "full_fps": round(g["full_fps"], 2),
         "full_p50_ms": round(pct(g["full_ms"], 50), 4), "full_p99_ms": round(pct(g["full_ms"], 99), 4),
         "speedup_warp_vs_full": round(g["warp_fps"] / g["full_fps"], 3),

@@ -463,8 +463,8 @@ def main():
                     help="warp: only the warp-frame legs (for profiling the warp kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and is not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); sharding across GPUs is a "
+              "design, not built.", file=sys.stderr)
         sys.exit(2)
     if args.warmup < 2:
         raise SystemExit("--warmup must be >= 2 (frames 0-1 are full clears)")

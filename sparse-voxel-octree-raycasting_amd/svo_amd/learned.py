@@ -2,7 +2,7 @@
 pipeline generates itself: a warp context renders the gated warp frame
 without the CNN (its CNN input is plane 0 as the gate leaves it, and the
 holes left after its rays are the pixels the CNN fills), a second context
-renders the full raycast of the same camera (the judge of the error), and
+renders the full raycast of the same camera (the target image), and
 the CNN takes one Adam step on random crops of each pair.
 """
 from __future__ import annotations

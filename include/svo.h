@@ -195,6 +195,10 @@ int svo_rle4_save(const char* path, int sx, int sy, int sz, const int32_t* vol);
 /* Write a colorized image (SVO_BUF_IMAGE words, b + g<<8 + r<<16, the PBO
  * layout of raycast.h:425-455) as a binary PPM, rows top to bottom. */
 int svo_image_save_ppm(const char* path, const uint32_t* image, int w, int h);
+/* The same image as an 8-bit RGB PNG (uncompressed deflate blocks).  Both
+ * writers replace the reference's display path (the PBO upload,
+ * raycast.h:425-472, shown through GL), which has no file output. */
+int svo_image_save_png(const char* path, const uint32_t* image, int w, int h);
 
 /* Seeded synthetic scenes (voxel streams for svo_build_octree).
  *   kind 1: value-noise heightfield shell (2 octaves)

@@ -1008,6 +1008,93 @@ extern "C" int svo_camera_path(uint64_t seed, int depth, int n_frames, float max
 // The colorized words are the PBO layout b + g<<8 + r<<16 that raycast.h
 // uploads with glTexSubImage2D(GL_BGRA) (raycast.h:425-455); a binary PPM
 // stores r, g, b bytes per pixel, rows top to bottom.
+// PNG (RFC 2083 / ISO 15948): 8-bit RGB, filter 0 on every row, the zlib
+// stream (RFC 1950) made of stored deflate blocks (RFC 1951 BTYPE 00), so no
+// compressor is needed.  CRC-32 is the reflected polynomial 0xEDB88320 over
+// chunk type + data; Adler-32 is taken modulo 65521 over the raw rows.
+namespace {
+uint32_t crc32_update(uint32_t c, const unsigned char* p, size_t n)
+{
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; k++) v = (v & 1u) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
+            table[i] = v;
+        }
+        ready = true;
+    }
+    for (size_t i = 0; i < n; i++) c = table[(c ^ p[i]) & 255u] ^ (c >> 8);
+    return c;
+}
+void put_be32(std::vector<unsigned char>& o, uint32_t v)
+{
+    for (int s = 24; s >= 0; s -= 8) o.push_back((unsigned char)(v >> s));
+}
+void put_chunk(std::vector<unsigned char>& o, const char* type, const std::vector<unsigned char>& data)
+{
+    put_be32(o, (uint32_t)data.size());
+    const size_t at = o.size();
+    o.insert(o.end(), type, type + 4);
+    o.insert(o.end(), data.begin(), data.end());
+    put_be32(o, crc32_update(0xffffffffu, o.data() + at, 4 + data.size()) ^ 0xffffffffu);
+}
+}  // namespace
+
+extern "C" int svo_image_save_png(const char* path, const uint32_t* image, int w, int h)
+{
+    if (!path || !image || w < 1 || h < 1 || w > (1 << 20) || h > (1 << 20))
+        return svo_fail(SVO_ERR_ARG, "bad image arguments");
+    const size_t row = (size_t)w * 3 + 1;
+    std::vector<unsigned char> raw(row * h);
+    for (int y = 0; y < h; y++) {
+        unsigned char* r = raw.data() + row * y;
+        r[0] = 0;  // filter type None
+        for (int x = 0; x < w; x++) {
+            const uint32_t v = image[(size_t)y * w + x];
+            r[1 + 3 * x] = (unsigned char)(v >> 16);
+            r[2 + 3 * x] = (unsigned char)(v >> 8);
+            r[3 + 3 * x] = (unsigned char)v;
+        }
+    }
+    std::vector<unsigned char> z;
+    z.reserve(raw.size() + raw.size() / 65535 * 5 + 16);
+    z.push_back(0x78);  // CMF: deflate, 32 KiB window
+    z.push_back(0x01);  // FLG: (0x78 * 256 + 0x01) % 31 == 0, no dictionary
+    uint32_t a = 1, b = 0;
+    for (size_t ofs = 0; ofs < raw.size() || ofs == 0;) {
+        const size_t len = std::min<size_t>(65535, raw.size() - ofs);
+        const bool last = ofs + len == raw.size();
+        z.push_back(last ? 1 : 0);  // BFINAL, BTYPE 00
+        z.push_back((unsigned char)len);
+        z.push_back((unsigned char)(len >> 8));
+        z.push_back((unsigned char)~len);
+        z.push_back((unsigned char)(~len >> 8));
+        z.insert(z.end(), raw.begin() + ofs, raw.begin() + ofs + len);
+        for (size_t i = ofs; i < ofs + len; i++) {
+            a = (a + raw[i]) % 65521u;
+            b = (b + a) % 65521u;
+        }
+        ofs += len;
+        if (last) break;
+    }
+    put_be32(z, (b << 16) | a);
+    std::vector<unsigned char> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    std::vector<unsigned char> ihdr;
+    put_be32(ihdr, (uint32_t)w);
+    put_be32(ihdr, (uint32_t)h);
+    ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8 bits, RGB, deflate, filter set 0, no interlace
+    put_chunk(out, "IHDR", ihdr);
+    put_chunk(out, "IDAT", z);
+    put_chunk(out, "IEND", {});
+    FILE* f = fopen(path, "wb");
+    if (!f) return svo_fail(SVO_ERR_IO, "cannot create %s", path);
+    bool ok = fwrite(out.data(), 1, out.size(), f) == out.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SVO_OK : svo_fail(SVO_ERR_IO, "write to %s failed", path);
+}
+
 extern "C" int svo_image_save_ppm(const char* path, const uint32_t* image, int w, int h)
 {
     if (!path || !image || w < 1 || h < 1) return svo_fail(SVO_ERR_ARG, "bad image arguments");

@@ -138,6 +138,7 @@ _f = {
                       C.POINTER(_u16p), C.POINTER(_u32p), C.POINTER(C.c_size_t)),
     "rle4_save": _sig("svo_rle4_save", C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)),
     "image_save_ppm": _sig("svo_image_save_ppm", C.c_int, C.c_char_p, _u32p, C.c_int, C.c_int),
+    "image_save_png": _sig("svo_image_save_png", C.c_int, C.c_char_p, _u32p, C.c_int, C.c_int),
     "camera_path": _sig("svo_camera_path", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_float,
                         C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)),
 }
@@ -158,6 +159,7 @@ EXPORTED = [
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
     "svo_set_ray_lanes", "svo_set_long_first", "svo_set_ray_block", "svo_raycast_quads", "svo_fillhole_punch",
+    "svo_image_save_png",
 ]
 
 
@@ -278,6 +280,13 @@ def save_ppm(path, image):
     img = np.ascontiguousarray(image, dtype=np.uint32)
     h, w = img.shape
     _check(_f["image_save_ppm"](str(path).encode(), img.ctypes.data_as(_u32p), int(w), int(h)))
+
+
+def save_png(path, image):
+    """Write a colorized image (uint32 [h, w], b + g<<8 + r<<16) as 8-bit RGB PNG."""
+    img = np.ascontiguousarray(image, dtype=np.uint32)
+    h, w = img.shape
+    _check(_f["image_save_png"](str(path).encode(), img.ctypes.data_as(_u32p), int(w), int(h)))
 
 
 def rle4_save(path, vol):

@@ -54,7 +54,17 @@ def build(verbose=True):
     tmp = out + ".tmp"
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-pthread"])
     os.replace(tmp, out)
+    build_example()
     return out
+
+
+def build_example():
+    """The C host-loop example, linked against the in-tree libsvo.so."""
+    src = os.path.join(HERE, "examples", "render_flythrough.c")
+    exe = os.path.join(HERE, "examples", "render_flythrough")
+    run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-pedantic", "-I", os.path.join(os.path.dirname(HERE), "include"),
+         src, "-L", OUT_DIR, "-lsvo", "-Wl,-rpath,$ORIGIN/../svo_amd", "-o", exe])
+    return exe
 
 
 if __name__ == "__main__":

@@ -42,3 +42,30 @@ def assert_equal_u32(a, b, what):
 
 def assert_equal_f32_bits(a, b, what):
     assert_equal_u32(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32), what)
+
+
+def read_png(raw):
+    """Decode an 8-bit RGB, filter-0 PNG with the standard library: every
+    chunk CRC checked with binascii.crc32, the IDAT stream inflated by zlib
+    (which checks its Adler-32)."""
+    import binascii
+    import struct
+    import zlib
+    assert raw[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, []
+    while pos < len(raw):
+        n, = struct.unpack(">I", raw[pos:pos + 4])
+        typ, data = raw[pos + 4:pos + 8], raw[pos + 8:pos + 8 + n]
+        crc, = struct.unpack(">I", raw[pos + 8 + n:pos + 12 + n])
+        assert crc == binascii.crc32(typ + data) & 0xffffffff, typ
+        chunks.append((typ, data))
+        pos += 12 + n
+    assert [c[0] for c in chunks] == [b"IHDR", b"IDAT", b"IEND"]
+    w, h, depth, ctype, comp, filt, inter = struct.unpack(">IIBBBBB", chunks[0][1])
+    assert (depth, ctype, comp, filt, inter) == (8, 2, 0, 0, 0)
+    rows = zlib.decompress(chunks[1][1])
+    assert len(rows) == h * (3 * w + 1)
+    px = np.frombuffer(rows, np.uint8).reshape(h, 3 * w + 1)
+    assert (px[:, 0] == 0).all()
+    return px[:, 1:].reshape(h, w, 3)
+

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import svo_amd as S
+from helpers import read_png
 
 
 def _file(maps):
@@ -150,32 +151,6 @@ def test_ppm_writer(tmp_path):
     assert raw[len(b"P6\n2 2\n255\n"):] == bytes([0x11, 0x22, 0x33, 0xFF, 0x00, 0x80, 0, 0, 0, 0xAB, 0xCD, 0xEF])
 
 
-def _read_png(raw):
-    """Decode an 8-bit RGB, filter-0 PNG with the standard library: every
-    chunk CRC checked with binascii.crc32, the IDAT stream inflated by zlib
-    (which checks its Adler-32)."""
-    import binascii
-    import struct
-    import zlib
-    assert raw[:8] == b"\x89PNG\r\n\x1a\n"
-    pos, chunks = 8, []
-    while pos < len(raw):
-        n, = struct.unpack(">I", raw[pos:pos + 4])
-        typ, data = raw[pos + 4:pos + 8], raw[pos + 8:pos + 8 + n]
-        crc, = struct.unpack(">I", raw[pos + 8 + n:pos + 12 + n])
-        assert crc == binascii.crc32(typ + data) & 0xffffffff, typ
-        chunks.append((typ, data))
-        pos += 12 + n
-    assert [c[0] for c in chunks] == [b"IHDR", b"IDAT", b"IEND"]
-    w, h, depth, ctype, comp, filt, inter = struct.unpack(">IIBBBBB", chunks[0][1])
-    assert (depth, ctype, comp, filt, inter) == (8, 2, 0, 0, 0)
-    rows = zlib.decompress(chunks[1][1])
-    assert len(rows) == h * (3 * w + 1)
-    px = np.frombuffer(rows, np.uint8).reshape(h, 3 * w + 1)
-    assert (px[:, 0] == 0).all()
-    return px[:, 1:].reshape(h, w, 3)
-
-
 @pytest.mark.parametrize("w,h", [(2, 2), (1, 1), (403, 211)])
 def test_png_writer(tmp_path, w, h):
     """403x211 RGB rows span several 65,535-byte stored deflate blocks."""
@@ -185,7 +160,7 @@ def test_png_writer(tmp_path, w, h):
         img = np.array([[0x00112233, 0x00FF0080], [0x00000000, 0x00ABCDEF]], np.uint32)
     p = tmp_path / "a.png"
     S.save_png(p, img)
-    rgb = _read_png(p.read_bytes())
+    rgb = read_png(p.read_bytes())
     want = np.stack([(img >> 16) & 255, (img >> 8) & 255, img & 255], axis=-1).astype(np.uint8)
     assert np.array_equal(rgb, want)
     q = tmp_path / "a.ppm"

@@ -280,6 +280,14 @@ int svo_set_ray_block(svo_ctx* ctx, int width);
  * 16x16-workgroup grid over its rectangle (raycast.h:361-366), so no
  * counterpart.  Default 1024; results do not depend on it. */
 int svo_set_long_first(svo_ctx* ctx, int long_trips);
+/* Learned frames (svo_frame_warp_learned): the number of workgroups the
+ * fused CNN tile kernel runs with while it shares the GPU with the ray
+ * launch (0 = two per CU, its grid when it runs alone).  Fewer workgroups
+ * take longer but leave the ray launch's SIMDs freer; the default is half
+ * the CUs.  Batched learned frames and timing mode 1 use the full grid.
+ * No counterpart in the reference (it has no CNN); results do not depend
+ * on it. */
+int svo_set_cnn_overlap(svo_ctx* ctx, int workgroups);
 
 /* ---- learned hole filler (no counterpart in the reference) ------------
  * A 4-layer 3x3 CNN (5 -> 32 -> 32 -> 32 -> 3 channels, ReLU) run with BF16

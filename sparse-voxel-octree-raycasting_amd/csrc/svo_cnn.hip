@@ -1042,18 +1042,22 @@ static unsigned tile_grid(int max_tiles, int device, int per_cu)
 
 // tile kernel variant: 0 = 4 waves, weights staged per layer (two workgroups
 // per CU); 1 = 8 waves, weights resident (one workgroup per CU)
-static int launch_tiles(const TileArgs& a, int mode, int max_tiles, int device, hipStream_t stream)
+// (grid > 0: that many workgroups, at most one per tile; the workgroups loop
+// over the tile list, so any count covers every tile)
+static int launch_tiles(const TileArgs& a, int mode, int max_tiles, int device, hipStream_t stream, int grid)
 {
+    const unsigned g1 = grid > 0 ? (unsigned)std::max(1, std::min(max_tiles, grid)) : tile_grid(max_tiles, device, 1);
+    const unsigned g2 = grid > 0 ? (unsigned)std::max(1, std::min(max_tiles, grid)) : tile_grid(max_tiles, device, 2);
     if (mode == 1)
-        hipLaunchKernelGGL((k_cnn_tiles<8, true>), dim3(tile_grid(max_tiles, device, 1)), dim3(512), 0, stream, a);
+        hipLaunchKernelGGL((k_cnn_tiles<8, true>), dim3(g1), dim3(512), 0, stream, a);
     else
-        hipLaunchKernelGGL((k_cnn_tiles<4, false>), dim3(tile_grid(max_tiles, device, 2)), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((k_cnn_tiles<4, false>), dim3(g2), dim3(256), 0, stream, a);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }
 
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
-                   const uint32_t* tile_count, int max_tiles, hipStream_t stream)
+                   const uint32_t* tile_count, int max_tiles, hipStream_t stream, int grid)
 {
     if (n->fused) {
         TileArgs a = tile_args(n);
@@ -1068,7 +1072,7 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
         a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
         a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
         a.staged = c->d_cnn_out;
-        return launch_tiles(a, n->tile_mode, max_tiles, c->device, stream);
+        return launch_tiles(a, n->tile_mode, max_tiles, c->device, stream, grid);
     }
     int rc = ensure_ws(n, (size_t)max_tiles * 576);
     if (rc) return rc;
@@ -1120,7 +1124,7 @@ extern "C" int svo_cnn_forward_tiles(svo_cnn* n, const uint16_t* x, int count, f
     a.x = n->xt;
     a.out = n->ot;
     a.n = count;
-    RCC(launch_tiles(a, n->tile_mode, count, n->device, n->stream));
+    RCC(launch_tiles(a, n->tile_mode, count, n->device, n->stream, 0));
     HIPCHECK(hipMemcpyAsync(out, n->ot, (size_t)count * 256 * 3 * 4, hipMemcpyDeviceToHost, n->stream));
     HIPCHECK(hipStreamSynchronize(n->stream));
     return SVO_OK;

@@ -62,6 +62,9 @@ struct svo_ctx {
     // full frames: 8x8 blocks whose previous full frame had a ray of at least
     // long_trips trips are raycast first (0 = raster order only)
     int long_trips = 1024;
+    // learned frames: workgroups of the CNN tile kernel while it runs
+    // alongside the ray launch (0 = two per CU); set to half the CUs at creation
+    int cnn_overlap_wgs = 0;
     uint32_t* d_block_long = nullptr;  // two flag arrays (previous / current frame)
     uint32_t* d_long_list = nullptr;   // pixel words (x | y << 16) of the long blocks
     uint32_t* d_long_count = nullptr;
@@ -89,7 +92,8 @@ struct svo_ctx {
 // learned-fill hooks implemented in svo_cnn.hip: the CNN on the gate's tiles
 // (input d_cnn_words/d_cnn_coords, output d_cnn_out), then its colours into
 // the image at the tiles' pixels that are still holes in plane 2
+// (grid: workgroups of the fused tile kernel, 0 = two per CU)
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
-                   const uint32_t* tile_count, int max_tiles, hipStream_t stream);
+                   const uint32_t* tile_count, int max_tiles, hipStream_t stream, int grid = 0);
 int cnn_apply_tiles(svo_ctx* c, const uint32_t* tile_list, const uint32_t* tile_count, int max_tiles,
                     hipStream_t stream);

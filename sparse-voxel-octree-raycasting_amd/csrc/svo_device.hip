@@ -1024,6 +1024,7 @@ extern "C" int svo_create(svo_ctx** out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->n_cu = prop.multiProcessorCount;
+    c->cnn_overlap_wgs = std::max(1, c->n_cu / 2);
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&c->cnn_stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&c->cnn_fork, hipEventDisableTiming) == hipSuccess &&
@@ -1515,7 +1516,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     if (overlap) {
         HIPCHECK(hipEventRecord(c->cnn_fork, c->stream));
         HIPCHECK(hipStreamWaitEvent(c->cnn_stream, c->cnn_fork, 0));
-        RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles, c->cnn_stream));
+        RC(cnn_fill_tiles(cnn, c, p, c->d_gate_tiles, c->d_gate, tiles, c->cnn_stream, c->cnn_overlap_wgs));
         HIPCHECK(hipEventRecord(c->cnn_join, c->cnn_stream));
     }
     // hole list, gate rays and the refresh tile in one ray launch
@@ -1592,6 +1593,15 @@ extern "C" int svo_set_long_first(svo_ctx* c, int long_trips)
         c->long_trips = long_trips;
         c->long_valid = false;  // flags are not kept while off and mean another threshold
     }
+    return SVO_OK;
+}
+
+extern "C" int svo_set_cnn_overlap(svo_ctx* c, int workgroups)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (workgroups < 0 || workgroups > 65536)
+        return svo_fail(SVO_ERR_ARG, "CNN workgroups %d outside 0..65536", workgroups);
+    c->cnn_overlap_wgs = workgroups;
     return SVO_OK;
 }
 

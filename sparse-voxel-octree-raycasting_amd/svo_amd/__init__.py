@@ -96,6 +96,7 @@ _f = {
     "set_ray_lanes": _sig("svo_set_ray_lanes", C.c_int, _ctxp, C.c_int),
     "set_long_first": _sig("svo_set_long_first", C.c_int, _ctxp, C.c_int),
     "set_ray_block": _sig("svo_set_ray_block", C.c_int, _ctxp, C.c_int),
+    "set_cnn_overlap": _sig("svo_set_cnn_overlap", C.c_int, _ctxp, C.c_int),
     "fillhole_punch": _sig("svo_fillhole_punch", C.c_int, _ctxp, _vp, _vp),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
@@ -159,7 +160,7 @@ EXPORTED = [
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
     "svo_set_ray_lanes", "svo_set_long_first", "svo_set_ray_block", "svo_raycast_quads", "svo_fillhole_punch",
-    "svo_image_save_png",
+    "svo_image_save_png", "svo_set_cnn_overlap",
 ]
 
 
@@ -455,6 +456,12 @@ class Context:
         had a ray of at least long_trips trips (0 = raster order only).
         Results are the same either way."""
         _check(_f["set_long_first"](self._h, int(long_trips)))
+
+    def set_cnn_overlap(self, workgroups):
+        """Workgroups of the CNN tile kernel while it runs alongside the
+        ray launch in learned frames (0 = two per CU).  Results are the same
+        either way."""
+        _check(_f["set_cnn_overlap"](self._h, int(workgroups)))
 
     def ray_count(self, reset=False):
         """Rays traced by the ray kernels since the last reset."""

@@ -8,7 +8,9 @@ lanes per refill), persistent workgroups per CU, traversal steps between
 refill checks, the LDS node-cache depth (octree words of the breadth-first
 top levels kept in LDS) and the shape of the blocks in which rectangle rays
 are handed out (svo_set_ray_block); full frames also search the trip count
-that puts a screen block on the long-rays-first list (svo_set_long_first).  Every candidate replays the same frame window from the same starting
+that puts a screen block on the long-rays-first list (svo_set_long_first),
+and learned frames the CNN's workgroups alongside the ray launch
+(svo_set_cnn_overlap).  Every candidate replays the same frame window from the same starting
 state (the window starts with the two clearing frames), so candidates see
 identical work; the score is the mean GPU time per frame over the timed
 frames (frames/s follows the mean), measured with the library's per-frame
@@ -83,6 +85,8 @@ def _search(space, coordinate, measure):
 
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
+# learned frames also search the CNN's workgroups alongside the ray launch
+LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512])
 
 
 def apply(ctx, cfg):
@@ -96,6 +100,8 @@ def apply(ctx, cfg):
         ctx.set_long_first(int(cfg["long_trips"]))
     if "ray_block" in cfg:
         ctx.set_ray_block(int(cfg["ray_block"]))
+    if "cnn_wgs" in cfg:
+        ctx.set_cnn_overlap(int(cfg["cnn_wgs"]))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None, cnn=None, max_cnn_holes=64):
@@ -107,7 +113,7 @@ def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=No
     two sweeps); otherwise the full grid is measured.  Applies and returns
     (best_setting, best_ms, trials)."""
     if space is None:
-        space = FULL_SPACE if mode == "full" else DEFAULT_SPACE
+        space = {"full": FULL_SPACE, "learned": LEARNED_SPACE}.get(mode, DEFAULT_SPACE)
     space = dict(space)
     trials = []
 

@@ -33,6 +33,12 @@ def test_full_space_adds_long_first():
     assert all(k in A.FULL_SPACE for k in A.DEFAULT_SPACE)
 
 
+def test_learned_space_adds_cnn_workgroups():
+    assert "cnn_wgs" in A.LEARNED_SPACE and "cnn_wgs" not in A.DEFAULT_SPACE
+    assert all(k in A.LEARNED_SPACE for k in A.DEFAULT_SPACE)
+    assert A.LEARNED_SPACE["cnn_wgs"][1] == 128  # the descent starts at half the CUs
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["warp", "full"])
 def test_tuned_context_stays_bitexact(mode):

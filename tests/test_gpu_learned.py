@@ -92,10 +92,12 @@ def test_fused_tile_fill_matches_per_layer_fill(mode):
         cnn.close()
 
 
-def test_cnn_alongside_rays_equals_sequential():
-    """The CNN on the second stream alongside the ray launch and the CNN run
-    after the fill on the frame's stream (per-stage timing) give identical
-    screens and images."""
+@pytest.mark.parametrize("overlap_wgs", [None, 0, 1, 7])
+def test_cnn_alongside_rays_equals_sequential(overlap_wgs):
+    """The CNN on the second stream alongside the ray launch (with the
+    default, the full or a small workgroup count) and the CNN run after the
+    fill on the frame's stream (per-stage timing) give identical screens and
+    images."""
     cnn = S.Cnn(0)
     L.load(cnn, L.WEIGHTS)
     words, root, depth, _ = W.octree(NAME)
@@ -106,6 +108,8 @@ def test_cnn_alongside_rays_equals_sequential():
         for c in (a, b):
             c.upload_octree(words, root, depth)
             c.resize(rx, ry)
+        if overlap_wgs is not None:
+            a.set_cnn_overlap(overlap_wgs)
         b.enable_timing(True, mode=1)
         n_cnn = 0
         for f in range(12):

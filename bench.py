@@ -137,6 +137,28 @@ def run_gpu(args, name):
         wr.close()
         res["warp_fps_image_to_host"] = args.steps / t_readback
         log(f"[bench] warp + image to host: {args.steps} frames in {t_readback * 1e3:.1f} ms")
+        # the same with svo_read_image_async into two pinned host buffers: each
+        # copy overlaps the next frame (the library double-buffers the image)
+        pins = [S.PinnedImage(rx, ry) for _ in range(2)]
+        outs = [b.array for b in pins]
+        wa = fresh()
+        t0 = time.perf_counter()
+        for f in range(args.warmup, total):
+            wa.frame_warp(params[f])
+            wa.read_image_async(outs[f & 1])
+        wa.sync()
+        t_async = time.perf_counter() - t0
+        last = wa.image()
+        ok = bool(np.array_equal(outs[(total - 1) & 1], last))
+        wa.close()
+        del outs
+        for b in pins:
+            b.close()
+        res["warp_fps_image_to_host_async"] = args.steps / t_async
+        log(f"[bench] warp + async image to pinned host memory: {args.steps} frames in {t_async * 1e3:.1f} ms "
+            f"(last copy equals the device image: {ok})")
+        if not ok:
+            raise SystemExit("async image copy differs from the device image")
 
     warp2 = fresh()
     warp2.enable_timing(mode=2)
@@ -513,6 +535,8 @@ def main():
                    "res": [wl["rx"], wl["ry"]], "octree_words": wl["words"]},
         "warp_p50_ms": round(pct(g["warp_ms"], 50), 4), "warp_p99_ms": round(pct(g["warp_ms"], 99), 4),
         "warp_fps_image_to_host": round(g["warp_fps_image_to_host"], 2) if "warp_fps_image_to_host" in g else None,
+        "warp_fps_image_to_host_async": (round(g["warp_fps_image_to_host_async"], 2)
+                                         if "warp_fps_image_to_host_async" in g else None),
         "full_fps": round(g["full_fps"], 2),
         "full_p50_ms": round(pct(g["full_ms"], 50), 4), "full_p99_ms": round(pct(g["full_ms"], 99), 4),
         "speedup_warp_vs_full": round(g["warp_fps"] / g["full_fps"], 3),

@@ -139,6 +139,20 @@ int svo_colorize(svo_ctx* ctx);
 /* ---- transfers ------------------------------------------------------- */
 /* ocl_copy_to_host (ocl.h:215-221): blocking read of `bytes` at byte_ofs. */
 int svo_read(svo_ctx* ctx, int buf, size_t byte_ofs, size_t bytes, void* dst);
+/* Non-blocking copy of the last frame's colorized image (SVO_BUF_IMAGE,
+ * N uint32) into dst, on a copy stream of its own: the next frame renders
+ * into a second image buffer meanwhile, so the copy overlaps it (the
+ * reference uploads the image to a GL pixel buffer instead, raycast.h:
+ * 425-455).  dst should be pinned host memory (hipHostMalloc) for the copy
+ * to run asynchronously; it holds the image once svo_sync returns.  A
+ * buffer is reused two frames later, after its copy has completed.  For a
+ * view of a batch, call svo_batch_sync first. */
+int svo_read_image_async(svo_ctx* ctx, uint32_t* dst);
+/* Pinned (page-locked) host memory for svo_read_image_async destinations,
+ * and its release.  No reference counterpart (ocl_copy_to_host, ocl.h:215,
+ * reads into ordinary memory, blocking). */
+int svo_host_alloc(size_t bytes, void** out);
+int svo_host_free(void* p);
 /* blocking write (host -> device) into a frame buffer */
 int svo_write(svo_ctx* ctx, int buf, size_t byte_ofs, size_t bytes, const void* src);
 /* ocl_end_all_kernels (ocl.h:253-265): wait for all queued work. */

@@ -31,7 +31,15 @@ struct svo_ctx {
     uint32_t* d_screen = nullptr;
     float4* d_coords = nullptr;
     uint32_t* d_ids = nullptr;
-    uint32_t* d_image = nullptr;
+    uint32_t* d_image = nullptr;  // the frame's image: image_buf[0] or image_buf[1]
+    // svo_read_image_async: a second image buffer, so the next frame renders
+    // into one while the other is copied to the host on copy_stream
+    uint32_t* image_buf[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t image_ready = nullptr;
+    hipEvent_t image_copied[2] = {nullptr, nullptr};
+    bool image_copy_pending[2] = {false, false};
+    bool image_flip = false;  // the next frame switches buffers
     unsigned long long* d_keys = nullptr;
     uint32_t* d_snap = nullptr;
     int32_t* d_motion = nullptr;  // svo_fillhole_punch: motion x (N) then y (N)

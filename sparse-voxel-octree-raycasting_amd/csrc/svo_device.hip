@@ -969,10 +969,17 @@ __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max
 
 static int free_frame(svo_ctx* c)
 {
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);  // no copy reads a freed image
+    if (c->d_image && c->d_image != c->image_buf[0] && c->d_image != c->image_buf[1])
+        (void)hipFree(c->d_image);  // a resize that failed before recording it
     (void)hipFree(c->d_screen);
     (void)hipFree(c->d_coords);
     (void)hipFree(c->d_ids);
-    (void)hipFree(c->d_image);
+    (void)hipFree(c->image_buf[0]);
+    (void)hipFree(c->image_buf[1]);
+    c->image_buf[0] = c->image_buf[1] = nullptr;
+    c->image_copy_pending[0] = c->image_copy_pending[1] = false;
+    c->image_flip = false;
     (void)hipFree(c->d_keys);
     (void)hipFree(c->d_snap);
     c->d_screen = nullptr;
@@ -1046,6 +1053,10 @@ extern "C" int svo_destroy(svo_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     if (c->cnn_stream) (void)hipStreamSynchronize(c->cnn_stream);
     free_frame(c);
+    for (auto& e : c->image_copied)
+        if (e) (void)hipEventDestroy(e);
+    if (c->image_ready) (void)hipEventDestroy(c->image_ready);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     drop_graph(c);
     (void)hipFree(c->d_view);
     (void)hipHostFree(c->h_view);
@@ -1129,6 +1140,7 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
     HIPCHECK(hipMemset(c->d_coords, 0, 4 * N * 16));
     HIPCHECK(hipMemset(c->d_ids, 0, ids_words * 4));
     HIPCHECK(hipMemset(c->d_image, 0, N * 4));
+    c->image_buf[0] = c->d_image;
     HIPCHECK(hipMemset(c->d_keys, 0xff, N * 8));
     HIPCHECK(hipMemset(c->d_snap, 0, N * 4));
     HIPCHECK(hipMemset(c->d_gate, 0, 128));
@@ -1178,8 +1190,23 @@ static int frame_end(svo_ctx* c)
     return SVO_OK;
 }
 
+// After svo_read_image_async the next frame renders into the other image
+// buffer; that buffer's own last copy (two frames back) must be complete.
+static void take_image(svo_ctx* c)
+{
+    if (!c->image_flip || !c->image_buf[1]) return;
+    c->image_flip = false;
+    const int s = c->d_image == c->image_buf[1] ? 0 : 1;
+    if (c->image_copy_pending[s]) {
+        (void)hipEventSynchronize(c->image_copied[s]);
+        c->image_copy_pending[s] = false;
+    }
+    c->d_image = c->image_buf[s];
+}
+
 static View make_view(svo_ctx* c, const svo_frame_params* p)
 {
+    take_image(c);
     View v;
     memset(&v, 0, sizeof(v));
     v.screen = c->d_screen;
@@ -1965,6 +1992,49 @@ extern "C" int svo_read(svo_ctx* c, int buf, size_t byte_ofs, size_t bytes, void
     return SVO_OK;
 }
 
+extern "C" int svo_host_alloc(size_t bytes, void** out)
+{
+    if (!out || bytes == 0) return svo_fail(SVO_ERR_ARG, "bad host allocation arguments");
+    *out = nullptr;
+    HIPCHECK(hipHostMalloc(out, bytes, hipHostMallocDefault));
+    return SVO_OK;
+}
+
+extern "C" int svo_host_free(void* p)
+{
+    if (p) HIPCHECK(hipHostFree(p));
+    return SVO_OK;
+}
+
+extern "C" int svo_read_image_async(svo_ctx* c, uint32_t* dst)
+{
+    RC(need_frame(c));
+    if (!dst) return svo_fail(SVO_ERR_ARG, "null destination");
+    HIPCHECK(hipSetDevice(c->device));
+    if (!c->copy_stream) {
+        bool ok = hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&c->image_ready, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&c->image_copied[0], hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&c->image_copied[1], hipEventDisableTiming) == hipSuccess;
+        if (!ok) return svo_fail(SVO_ERR_HIP, "image copy stream set-up failed");
+    }
+    if (!c->image_buf[1]) {
+        if (hipMalloc(&c->image_buf[1], 4 * c->N) != hipSuccess) {
+            c->image_buf[1] = nullptr;
+            return svo_fail(SVO_ERR_NOMEM, "second image buffer");
+        }
+        HIPCHECK(hipMemset(c->image_buf[1], 0, 4 * c->N));
+    }
+    const int s = c->d_image == c->image_buf[1] ? 1 : 0;
+    HIPCHECK(hipEventRecord(c->image_ready, c->stream));
+    HIPCHECK(hipStreamWaitEvent(c->copy_stream, c->image_ready, 0));
+    HIPCHECK(hipMemcpyAsync(dst, c->d_image, 4 * c->N, hipMemcpyDeviceToHost, c->copy_stream));
+    HIPCHECK(hipEventRecord(c->image_copied[s], c->copy_stream));
+    c->image_copy_pending[s] = true;
+    c->image_flip = true;
+    return SVO_OK;
+}
+
 extern "C" int svo_write(svo_ctx* c, int buf, size_t byte_ofs, size_t bytes, const void* src)
 {
     if (!c || (!src && bytes)) return svo_fail(SVO_ERR_ARG, "null argument");
@@ -1983,6 +2053,10 @@ extern "C" int svo_sync(svo_ctx* c)
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->copy_stream) {
+        HIPCHECK(hipStreamSynchronize(c->copy_stream));
+        c->image_copy_pending[0] = c->image_copy_pending[1] = false;
+    }
     if (c->frame_timed) {
         c->frame_timed = false;
         int last = ST_N, first = -1;

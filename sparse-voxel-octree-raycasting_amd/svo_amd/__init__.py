@@ -97,6 +97,9 @@ _f = {
     "set_long_first": _sig("svo_set_long_first", C.c_int, _ctxp, C.c_int),
     "set_ray_block": _sig("svo_set_ray_block", C.c_int, _ctxp, C.c_int),
     "set_cnn_overlap": _sig("svo_set_cnn_overlap", C.c_int, _ctxp, C.c_int),
+    "read_image_async": _sig("svo_read_image_async", C.c_int, _ctxp, _u32p),
+    "host_alloc": _sig("svo_host_alloc", C.c_int, C.c_size_t, C.POINTER(_vp)),
+    "host_free": _sig("svo_host_free", C.c_int, _vp),
     "fillhole_punch": _sig("svo_fillhole_punch", C.c_int, _ctxp, _vp, _vp),
     "camera": _sig("svo_camera", C.c_int, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                    C.c_int, C.c_int, C.POINTER(FrameParams), C.POINTER(C.c_float)),
@@ -160,7 +163,7 @@ EXPORTED = [
     "svo_build_octree_gpu", "svo_cnn_forward_tiles", "svo_cnn_set_fused", "svo_set_ray_spread",
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
     "svo_set_ray_lanes", "svo_set_long_first", "svo_set_ray_block", "svo_raycast_quads", "svo_fillhole_punch",
-    "svo_image_save_png", "svo_set_cnn_overlap",
+    "svo_image_save_png", "svo_set_cnn_overlap", "svo_read_image_async", "svo_host_alloc", "svo_host_free",
 ]
 
 
@@ -281,6 +284,29 @@ def save_ppm(path, image):
     img = np.ascontiguousarray(image, dtype=np.uint32)
     h, w = img.shape
     _check(_f["image_save_ppm"](str(path).encode(), img.ctypes.data_as(_u32p), int(w), int(h)))
+
+
+class PinnedImage:
+    """A page-locked host buffer (svo_host_alloc) viewed as a uint32 image
+    [res_y, res_x], for Context.read_image_async; free with close()."""
+
+    def __init__(self, res_x, res_y):
+        self._p = _vp()
+        _check(_f["host_alloc"](4 * int(res_x) * int(res_y), C.byref(self._p)))
+        buf = (C.c_uint32 * (int(res_x) * int(res_y))).from_address(self._p.value)
+        self.array = np.ctypeslib.as_array(buf).reshape(int(res_y), int(res_x))
+
+    def close(self):
+        if self._p:
+            self.array = None
+            _check(_f["host_free"](self._p))
+            self._p = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def save_png(path, image):
@@ -516,6 +542,14 @@ class Context:
 
     def image(self):
         return self.read(BUF_IMAGE, 0, 4 * self.N).reshape(self.res_y, self.res_x)
+
+    def read_image_async(self, out):
+        """Start copying the last frame's image into `out` (uint32, res_y x
+        res_x, C-contiguous; pinned memory lets the copy overlap the next
+        frame).  `out` holds the image after sync(); keep it alive until then."""
+        if out.dtype != np.uint32 or not out.flags["C_CONTIGUOUS"] or out.size != self.N:
+            raise ValueError("out must be a C-contiguous uint32 array of res_x * res_y elements")
+        _check(_f["read_image_async"](self._h, out.ctypes.data_as(_u32p)))
 
 
 _fp = C.POINTER(C.c_float)

@@ -143,8 +143,8 @@ int svo_read(svo_ctx* ctx, int buf, size_t byte_ofs, size_t bytes, void* dst);
  * N uint32) into dst, on a copy stream of its own: the next frame renders
  * into a second image buffer meanwhile, so the copy overlaps it (the
  * reference uploads the image to a GL pixel buffer instead, raycast.h:
- * 425-455).  dst should be pinned host memory (hipHostMalloc) for the copy
- * to run asynchronously; it holds the image once svo_sync returns.  A
+ * 425-455).  dst should be pinned host memory (svo_host_alloc) for the
+ * copy to run asynchronously; it holds the image once svo_sync returns.  A
  * buffer is reused two frames later, after its copy has completed.  For a
  * view of a batch, call svo_batch_sync first. */
 int svo_read_image_async(svo_ctx* ctx, uint32_t* dst);

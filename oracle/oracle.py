@@ -68,6 +68,9 @@ _build = _sig("orc_build_octree", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, C
 _free = _sig("orc_free", None, C.c_void_p)
 _trace = _sig("orc_trace_pixel", C.c_uint32, _u32p, C.c_size_t, C.c_uint32, C.c_int, C.c_int, C.c_int,
               _P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats))
+_trace_pops = _sig("orc_trace_pixel_pops", C.c_uint32, _u32p, C.c_size_t, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                   _P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int,
+                   C.POINTER(C.c_int))
 _init = _sig("orc_state_init", C.c_int, _S, C.c_int, C.c_int, _u32p, C.c_size_t, C.c_uint32, C.c_int)
 _sfree = _sig("orc_state_free", None, _S)
 _memset = _sig("orc_memset_screen", None, _S, C.c_size_t, C.c_uint32, C.c_size_t)
@@ -160,6 +163,19 @@ def trace_pixel(words, root, depth, res_x, res_y, p, x, y):
     w = _trace(words.ctypes.data_as(_u32p), len(words), root, depth, res_x, res_y, C.byref(p), x, y,
                xyz, C.byref(st))
     return w, tuple(xyz), st
+
+
+def trace_pixel_pops(words, root, depth, res_x, res_y, p, x, y, max_pops=1 << 16):
+    """The same ray as trace_pixel, with its pops: int32 [n, 8] rows of
+    (x, y, z before the step, x, y, z after, level before, level popped to),
+    positions in the octant-folded frame."""
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    xyz = (C.c_float * 3)()
+    log = np.zeros((max_pops, 8), np.int32)
+    n = C.c_int(0)
+    w = _trace_pops(words.ctypes.data_as(_u32p), len(words), root, depth, res_x, res_y, C.byref(p), x, y, xyz,
+                    log.ctypes.data_as(C.POINTER(C.c_int32)), max_pops, C.byref(n))
+    return w, tuple(xyz), log[:n.value].copy()
 
 
 class Frame:

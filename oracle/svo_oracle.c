@@ -379,9 +379,31 @@ static inline float dot3(float ax, float ay, float az, float bx, float by, float
  * takes the same branch (kernel.cl:204), so -1 stands for it. */
 static inline int ilog2_or_neg(uint32_t v) { return v ? 31 - __builtin_clz(v) : -1; }
 
+static uint32_t trace_impl(const uint32_t* oct, size_t n_words, uint32_t root,
+                           int depth, int res_x, int res_y, const orc_params* p,
+                           int idx, int idy, float xyz[3], orc_stats* st,
+                           int32_t* pops, int max_pops, int* n_pops);
+
 uint32_t orc_trace_pixel(const uint32_t* oct, size_t n_words, uint32_t root,
                          int depth, int res_x, int res_y, const orc_params* p,
                          int idx, int idy, float xyz[3], orc_stats* st)
+{
+    return trace_impl(oct, n_words, root, depth, res_x, res_y, p, idx, idy, xyz, st, NULL, 0, NULL);
+}
+
+uint32_t orc_trace_pixel_pops(const uint32_t* oct, size_t n_words, uint32_t root,
+                              int depth, int res_x, int res_y, const orc_params* p,
+                              int idx, int idy, float xyz[3], int32_t* pops, int max_pops,
+                              int* n_pops)
+{
+    *n_pops = 0;
+    return trace_impl(oct, n_words, root, depth, res_x, res_y, p, idx, idy, xyz, NULL, pops, max_pops, n_pops);
+}
+
+static uint32_t trace_impl(const uint32_t* oct, size_t n_words, uint32_t root,
+                           int depth, int res_x, int res_y, const orc_params* p,
+                           int idx, int idy, float xyz[3], orc_stats* st,
+                           int32_t* pops, int max_pops, int* n_pops)
 {
     const int D = depth;
     const int DAND = (1 << D) - 1;
@@ -462,6 +484,11 @@ uint32_t orc_trace_pixel(const uint32_t* oct, size_t n_words, uint32_t root,
         if (distance > 400000.0f) break;
         if (rek_new < rek) continue;
         if (st) st->pops++;
+        if (pops && *n_pops < max_pops) {  /* integer cells before and after, levels */
+            int32_t* e = pops + 8 * (*n_pops)++;
+            e[0] = bx; e[1] = by; e[2] = bz; e[3] = ix; e[4] = iy; e[5] = iz;
+            e[6] = rek; e[7] = rek_new + 1;
+        }
         rek = rek_new + 1;
         nodeid = stack[rek];
         nodeid_before = stack[rek + 1];

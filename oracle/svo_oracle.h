@@ -23,10 +23,13 @@
  *   orc_trace_pixel     dense-grid FP64 first-hit DDA on small scenes
  *                       (hit voxel + colour byte); LOD hits against the
  *                       octree's node colours (creator voxel, first leaf);
- *                       trajectory quirks (XOR pop, when the LOD increments,
- *                       miss colour) follow kernel.cl:114-214 and are
- *                       PARTIALLY PINNED (no reference run of kernel.cl is
- *                       possible on this host: no CPU OpenCL runtime)
+ *                       pop levels against the geometric rule (one above
+ *                       the highest changed position bit) via the pop log;
+ *                       trajectory quirks (the XOR pop on edge and corner
+ *                       crossings, when the LOD increments, miss colour)
+ *                       follow kernel.cl:114-214 and are PARTIALLY PINNED
+ *                       (no reference run of kernel.cl is possible on this
+ *                       host: no CPU OpenCL runtime)
  *   orc_proj            zero-motion invariant + literal sequential semantics
  *   orc_counthole/...   closed forms + brute force
  *   orc_fillhole2_*     closed-form per-pixel rules
@@ -84,6 +87,13 @@ typedef struct {
 uint32_t orc_trace_pixel(const uint32_t* oct, size_t n_words, uint32_t root,
                          int depth, int res_x, int res_y, const orc_params* p,
                          int idx, int idy, float xyz[3], orc_stats* st);
+/* The same ray, logging its pops (kernel.cl:197-208): per pop 8 ints, the
+ * integer position before and after the step (folded frame), the level
+ * before and the level it pops to.  At most max_pops are logged. */
+uint32_t orc_trace_pixel_pops(const uint32_t* oct, size_t n_words, uint32_t root,
+                              int depth, int res_x, int res_y, const orc_params* p,
+                              int idx, int idy, float xyz[3], int32_t* pops, int max_pops,
+                              int* n_pops);
 
 /* ---- frame state (raycast.h:79-85, :169-173, :267-270) ---- */
 typedef struct {

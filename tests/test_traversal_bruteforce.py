@@ -185,3 +185,49 @@ def test_lod_hits_return_node_colours(scene):
     assert sum(v[0] for v in by_level.values()) > 3000
     assert all(k in by_level for k in (2, 3, 4, 5)), by_level
     assert all(v[0] == v[1] for v in by_level.values()), by_level
+
+
+
+def test_pops_go_above_the_highest_changed_bit(scene):
+    """Every pop of the restatement, logged per step: the ray left the node it
+    was in (a position bit at or above its level changed), and it resumes one
+    level above the highest changed bit of the step, the lowest level whose
+    node is certain to hold the new cell.  That level is computed here from
+    the OR of the three axes' changes.  The reference XORs them instead
+    (kernel.cl:197-201); the two differ only when two axes flip the same
+    highest bit in one step (an edge or corner crossing), which is counted.
+    Every other pop pins the log2, the +1 and the 2^D mask of
+    kernel.cl:197-208 against the geometric rule; most steps change two or
+    three axes (the others move inside the cell), so this covers more than
+    single-axis steps."""
+    occ, col, words, root, xyz, rgba = scene
+    dand = SIZE - 1
+    rng = np.random.default_rng(21)
+    rx, ry = 256, 128
+    agree = quirk = multi = 0
+    for cam in range(3):
+        yaw = float(rng.uniform(-3, 3))
+        pitch = float(rng.uniform(0.05, 0.5))
+        pos = (float(rng.uniform(0, 32)), float(rng.uniform(17, 24)), float(rng.uniform(0, 32)))
+        p, _ = O.camera(yaw, pitch, 0.0, pos, D, 0)
+        for y in range(0, ry, 8):
+            for x in range(0, rx, 8):
+                d = ray_dir32(p, x, y, rx, ry)
+                if min(abs(v) for v in d) < 2**-20:
+                    continue
+                _, _, pops = O.trace_pixel_pops(words, root, D, rx, ry, p, x, y)
+                for bx, by, bz, ix, iy, iz, lv_before, lv_after in pops.tolist():
+                    c = [(bx ^ ix) & dand, (by ^ iy) & dand, (bz ^ iz) & dand]
+                    or_lv = (c[0] | c[1] | c[2]).bit_length()
+                    xor_lv = (c[0] ^ c[1] ^ c[2]).bit_length()
+                    assert (c[0] | c[1] | c[2]) >> lv_before, "a pop without leaving the node"
+                    assert lv_after == xor_lv  # the reference's rule, literally
+                    multi += sum(v != 0 for v in c) > 1
+                    if xor_lv == or_lv:
+                        agree += 1
+                        assert lv_after == or_lv and lv_after > lv_before
+                    else:
+                        quirk += 1
+    assert agree > 10000 and multi > agree // 2
+    assert quirk <= agree // 100
+    print(f"pops: {agree} at the geometric level ({multi} with several axes changed), {quirk} XOR-only")

@@ -341,8 +341,10 @@ __global__ __launch_bounds__(256) void k_cnn_apply(const uint32_t* __restrict__ 
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
 // 32-channel pixels: the 16-byte channel group g of pixel p sits in slot
-// g ^ ((p >> 2) & 3), so 16 consecutive pixels cover all LDS banks
-__device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 2) & 3)) << 3); }
+// g ^ ((p >> 1) & 3).  A 128-bit LDS read is served 8 lanes at a time from
+// 128 bytes of banks: 8 consecutive pixels (64-byte stride) of one group
+// then land on 8 distinct 16-byte bank slots (pixel parity x slot)
+__device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 1) & 3)) << 3); }
 
 struct TileArgs {
     // frame mode (x == null): features of the crop at tile*16 - 4 from the

@@ -345,6 +345,11 @@ typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 // 128 bytes of banks: 8 consecutive pixels (64-byte stride) of one group
 // then land on 8 distinct 16-byte bank slots (pixel parity x slot)
 __device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 1) & 3)) << 3); }
+// 16-channel input pixels (layer 1): the 8-byte group k (4 channels) of
+// pixel p sits in slot k ^ ((p >> 2) & 3); a 64-bit read is served 16 lanes
+// at a time from 128 bytes, and 16 consecutive pixels (32-byte stride) of
+// one group then cover 16 distinct 8-byte slots
+__device__ __forceinline__ int swz1(int p, int k) { return p * 16 + ((k ^ ((p >> 2) & 3)) << 2); }
 
 struct TileArgs {
     // frame mode (x == null): features of the crop at tile*16 - 4 from the
@@ -460,7 +465,7 @@ __device__ __forceinline__ void tile_layer1(const bf16* P, bf16* Q, const bf16* 
         for (int tap = 0; tap < 9; tap++) {
             const int dq = (tap / 3) * 24 + tap % 3;
             for (int m = 0; m < 2; m++) {
-                const s4v afr = *(const s4v*)(P + (q0[m] + dq) * 16 + kg * 4);
+                const s4v afr = *(const s4v*)(P + swz1(q0[m] + dq, kg));
                 for (int nn = 0; nn < 2; nn++)
                     acc[m][nn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b1[tap][nn], afr, acc[m][nn], 0, 0, 0);
             }
@@ -529,8 +534,10 @@ __global__ __launch_bounds__(NW * 64) void k_cnn_tiles(TileArgs a)
                 lo.y = (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16);
                 lo.z = (uint32_t)f2bf(f[4]);
             }
-            *(uint4*)(P + q * 16) = lo;
-            *(uint4*)(P + q * 16 + 8) = hi;
+            *(uint2*)(P + swz1(q, 0)) = make_uint2(lo.x, lo.y);
+            *(uint2*)(P + swz1(q, 1)) = make_uint2(lo.z, lo.w);
+            *(uint2*)(P + swz1(q, 2)) = make_uint2(hi.x, hi.y);
+            *(uint2*)(P + swz1(q, 3)) = make_uint2(hi.z, hi.w);
         }
         if (!RESIDENT) stage_weights(Wl, a.w[0], 9 * 32, false, tid, NT);
         __syncthreads();

@@ -70,7 +70,8 @@ _trace = _sig("orc_trace_pixel", C.c_uint32, _u32p, C.c_size_t, C.c_uint32, C.c_
               _P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats))
 _trace_pops = _sig("orc_trace_pixel_pops", C.c_uint32, _u32p, C.c_size_t, C.c_uint32, C.c_int, C.c_int, C.c_int,
                    _P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int,
-                   C.POINTER(C.c_int))
+                   C.POINTER(C.c_int), C.POINTER(Stats))
+set_mutation = _sig("orc_set_mutation", None, C.c_int)
 _init = _sig("orc_state_init", C.c_int, _S, C.c_int, C.c_int, _u32p, C.c_size_t, C.c_uint32, C.c_int)
 _sfree = _sig("orc_state_free", None, _S)
 _memset = _sig("orc_memset_screen", None, _S, C.c_size_t, C.c_uint32, C.c_size_t)
@@ -165,17 +166,28 @@ def trace_pixel(words, root, depth, res_x, res_y, p, x, y):
     return w, tuple(xyz), st
 
 
-def trace_pixel_pops(words, root, depth, res_x, res_y, p, x, y, max_pops=1 << 16):
-    """The same ray as trace_pixel, with its pops: int32 [n, 8] rows of
-    (x, y, z before the step, x, y, z after, level before, level popped to),
-    positions in the octant-folded frame."""
+def trace_steps(words, root, depth, res_x, res_y, p, x, y, max_steps=1 << 17):
+    """The same ray as trace_pixel, with every DDA step: int32 [n, 9] rows of
+    (x, y, z before the step, x, y, z after, level, XOR level + 1, ends by
+    distance), positions in the octant-folded frame; the step pops to the
+    XOR level + 1 when that is >= its level and it does not end the ray.
+    Returns (word, xyz, steps, stats)."""
     words = np.ascontiguousarray(words, dtype=np.uint32)
     xyz = (C.c_float * 3)()
-    log = np.zeros((max_pops, 8), np.int32)
+    log = np.zeros((max_steps, 9), np.int32)
     n = C.c_int(0)
+    st = Stats()
     w = _trace_pops(words.ctypes.data_as(_u32p), len(words), root, depth, res_x, res_y, C.byref(p), x, y, xyz,
-                    log.ctypes.data_as(C.POINTER(C.c_int32)), max_pops, C.byref(n))
-    return w, tuple(xyz), log[:n.value].copy()
+                    log.ctypes.data_as(C.POINTER(C.c_int32)), max_steps, C.byref(n), C.byref(st))
+    return w, tuple(xyz), log[:n.value].copy(), st
+
+
+def trace_pixel_pops(words, root, depth, res_x, res_y, p, x, y, max_pops=1 << 17):
+    """The pops of the ray: int32 [n, 8] rows of (x, y, z before the step,
+    x, y, z after, level before, level popped to)."""
+    w, xyz, steps, _ = trace_steps(words, root, depth, res_x, res_y, p, x, y, max_pops)
+    popped = (steps[:, 7] >= steps[:, 6]) & (steps[:, 8] == 0)
+    return w, xyz, steps[popped, :8].copy()
 
 
 class Frame:

@@ -12,11 +12,13 @@
  * fixtures; bitcount and colorize exhaustively; the tile schedule and the
  * hole compaction by closed forms and an independent formulation; fillhole2
  * per rule; the projection by its zero-motion invariant and an independent
- * key-min formulation; the camera by matrix identities.  orc_trace_pixel is
- * PARTIALLY PINNED: full-depth hits against an FP64 voxel march, and LOD hits
- * against the octree's node colours; the XOR pop level, the moment the LOD
- * increments and the miss colour follow the cited lines with no independent
- * check (the reference kernels need an OpenCL runtime this image lacks).
+ * key-min formulation; the camera by matrix identities.  orc_trace_pixel:
+ * full-depth hits against an FP64 voxel march, LOD hits against the octree's
+ * node colours, and the whole step/pop sequence, the level a LOD hit stops
+ * at, the exit kind (hit, distance, y crossing), the exit node and its colour
+ * against an independent FP64 model of the traversal (largest-empty-cell
+ * skipping with the LOD rule applied at cell exits); planted mistakes in the
+ * LOD rule and the exit node fail those checks (orc_set_mutation).
  */
 #include "svo_oracle.h"
 
@@ -394,11 +396,17 @@ uint32_t orc_trace_pixel(const uint32_t* oct, size_t n_words, uint32_t root,
 uint32_t orc_trace_pixel_pops(const uint32_t* oct, size_t n_words, uint32_t root,
                               int depth, int res_x, int res_y, const orc_params* p,
                               int idx, int idy, float xyz[3], int32_t* pops, int max_pops,
-                              int* n_pops)
+                              int* n_pops, orc_stats* st)
 {
     *n_pops = 0;
-    return trace_impl(oct, n_words, root, depth, res_x, res_y, p, idx, idy, xyz, NULL, pops, max_pops, n_pops);
+    return trace_impl(oct, n_words, root, depth, res_x, res_y, p, idx, idy, xyz, st, pops, max_pops, n_pops);
 }
+
+/* Planted mistakes for the tests that show the traversal checks catch them
+ * (0 = the reference's rules; see svo_oracle.h).  Not thread-safe: set it
+ * only around single-threaded trace calls. */
+static int g_mutation = 0;
+void orc_set_mutation(int m) { g_mutation = m; }
 
 static uint32_t trace_impl(const uint32_t* oct, size_t n_words, uint32_t root,
                            int depth, int res_x, int res_y, const orc_params* p,
@@ -450,6 +458,7 @@ static uint32_t trace_impl(const uint32_t* oct, size_t n_words, uint32_t root,
          * after 2^22 loop trips finishes as a miss at its current node */
         if (++guard > (1u << 22)) break;
         if (st) st->iters++;
+        if (g_mutation == 1 && distance > lodswitch) { lodswitch *= 2; ++lod; }  /* planted: every trip */
         int p0x = ix >> rek, p0y = iy >> rek, p0z = iz >> rek;
         int node_index = ((p0x & 1) | ((p0y & 1) << 1) | ((p0z & 1) << 2)) ^ sign_xyz;
         node_test = nodeid & (1u << node_index);
@@ -459,6 +468,7 @@ static uint32_t trace_impl(const uint32_t* oct, size_t n_words, uint32_t root,
             nodeid = fetch_child(&o, nodeid, nodeid_before, &local_root,
                                  (uint32_t)node_index, node_test, rek);
             nodeid_before = tmp;
+            if (g_mutation == 2 && distance > lodswitch) { lodswitch *= 2; ++lod; }  /* planted: on descends */
             if (rek <= lod) break;
             rek--;
             stack[rek] = nodeid;
@@ -481,24 +491,27 @@ static uint32_t trace_impl(const uint32_t* oct, size_t n_words, uint32_t root,
         int x0rz = bz ^ iz;
         int x0r = x0rx ^ x0ry ^ x0rz;
         int rek_new = ilog2_or_neg((uint32_t)(x0r & DAND));
+        if (pops && *n_pops < max_pops) {
+            /* every step: integer cells before and after, the level, the XOR
+             * level + 1 (0: no bit changed), whether it ends the ray by distance */
+            int32_t* e = pops + 9 * (*n_pops)++;
+            e[0] = bx; e[1] = by; e[2] = bz; e[3] = ix; e[4] = iy; e[5] = iz;
+            e[6] = rek; e[7] = rek_new + 1; e[8] = distance > 400000.0f;
+        }
         if (distance > 400000.0f) break;
         if (rek_new < rek) continue;
         if (st) st->pops++;
-        if (pops && *n_pops < max_pops) {  /* integer cells before and after, levels */
-            int32_t* e = pops + 8 * (*n_pops)++;
-            e[0] = bx; e[1] = by; e[2] = bz; e[3] = ix; e[4] = iy; e[5] = iz;
-            e[6] = rek; e[7] = rek_new + 1;
-        }
         rek = rek_new + 1;
         nodeid = stack[rek];
         nodeid_before = stack[rek + 1];
         if (rek == D) nodeid_before = nodeid = root;
-        if (distance > lodswitch) { lodswitch *= 2; ++lod; }
+        if (g_mutation != 1 && g_mutation != 2 && distance > lodswitch) { lodswitch *= 2; ++lod; }
     } while (!(x0ry & (2 * DAND + 2)));
     if (sign_x) px = (float)SCALE_MAX - px;
     if (sign_y) py = (float)SCALE_MAX - py;
     if (sign_z) pz = (float)SCALE_MAX - pz;
 
+    if (g_mutation == 3 && !node_test) nodeid = nodeid_before;  /* planted: exit colour from the parent */
     uint32_t col = fetch_color(&o, nodeid, nodeid_before, stack[rek + 1], &local_root,
                                rek, node_test);
     xyz[0] = px / 16.0f; xyz[1] = py / 16.0f; xyz[2] = pz / 16.0f;

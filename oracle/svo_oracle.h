@@ -87,13 +87,22 @@ typedef struct {
 uint32_t orc_trace_pixel(const uint32_t* oct, size_t n_words, uint32_t root,
                          int depth, int res_x, int res_y, const orc_params* p,
                          int idx, int idy, float xyz[3], orc_stats* st);
-/* The same ray, logging its pops (kernel.cl:197-208): per pop 8 ints, the
- * integer position before and after the step (folded frame), the level
- * before and the level it pops to.  At most max_pops are logged. */
+/* The same ray, logging every DDA step (kernel.cl:181-208): per step 9 ints,
+ * the integer position before and after the step (folded frame), the level
+ * of the step, the XOR level + 1 (floor(log2((dx^dy^dz) & (2^D-1))) + 1, 0
+ * when no bit changed; the step pops to it when it is >= the level), and 1
+ * if the step ends the ray by distance (no pop then).  At most max_pops
+ * steps are logged.  st (optional) receives the ray's exit state. */
 uint32_t orc_trace_pixel_pops(const uint32_t* oct, size_t n_words, uint32_t root,
                               int depth, int res_x, int res_y, const orc_params* p,
                               int idx, int idy, float xyz[3], int32_t* pops, int max_pops,
-                              int* n_pops);
+                              int* n_pops, orc_stats* st);
+/* Test hook: plant a mistake in orc_trace_pixel* so the tests can show their
+ * checks catch it.  0 = the reference's rules (default); 1 = the LOD switch
+ * evaluated on every loop trip instead of on pops (kernel.cl:209); 2 = on
+ * descends instead; 3 = a ray that exits without a hit returns its parent
+ * node's colour.  Process-global: only for single-threaded trace calls. */
+void orc_set_mutation(int m);
 
 /* ---- frame state (raycast.h:79-85, :169-173, :267-270) ---- */
 typedef struct {

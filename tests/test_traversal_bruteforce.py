@@ -189,22 +189,23 @@ def test_lod_hits_return_node_colours(scene):
 
 
 def test_pops_go_above_the_highest_changed_bit(scene):
-    """Every pop of the restatement, logged per step: the ray left the node it
-    was in (a position bit at or above its level changed), and it resumes one
-    level above the highest changed bit of the step, the lowest level whose
-    node is certain to hold the new cell.  That level is computed here from
-    the OR of the three axes' changes.  The reference XORs them instead
-    (kernel.cl:197-201); the two differ only when two axes flip the same
-    highest bit in one step (an edge or corner crossing), which is counted.
-    Every other pop pins the log2, the +1 and the 2^D mask of
-    kernel.cl:197-208 against the geometric rule; most steps change two or
-    three axes (the others move inside the cell), so this covers more than
-    single-axis steps."""
+    """Every DDA step of the restatement, logged: a step pops when the XOR of
+    the three axes' position changes (kernel.cl:197-201) has its highest bit
+    at or above the step's level, and resumes one level above that bit.  The
+    geometric rule takes the OR of the changes instead: the lowest level
+    whose node is certain to hold the new cell.  The two differ only when
+    two axes flip the same highest bit in one step (an edge or corner
+    crossing).  Every step where they agree pins the log2, the +1 and the
+    2^D mask of kernel.cl:197-208 against the geometric rule (most steps
+    change two or three axes, so this covers more than single-axis steps);
+    the steps where they differ are counted, including the ones where the
+    XOR cancels a bit the OR keeps so that the reference skips a pop the
+    geometry calls for."""
     occ, col, words, root, xyz, rgba = scene
     dand = SIZE - 1
     rng = np.random.default_rng(21)
     rx, ry = 256, 128
-    agree = quirk = multi = 0
+    agree = quirk = skipped = multi = pops = 0
     for cam in range(3):
         yaw = float(rng.uniform(-3, 3))
         pitch = float(rng.uniform(0.05, 0.5))
@@ -215,19 +216,25 @@ def test_pops_go_above_the_highest_changed_bit(scene):
                 d = ray_dir32(p, x, y, rx, ry)
                 if min(abs(v) for v in d) < 2**-20:
                     continue
-                _, _, pops = O.trace_pixel_pops(words, root, D, rx, ry, p, x, y)
-                for bx, by, bz, ix, iy, iz, lv_before, lv_after in pops.tolist():
+                _, _, steps, _ = O.trace_steps(words, root, D, rx, ry, p, x, y)
+                for bx, by, bz, ix, iy, iz, lv, xor1, far in steps.tolist():
                     c = [(bx ^ ix) & dand, (by ^ iy) & dand, (bz ^ iz) & dand]
                     or_lv = (c[0] | c[1] | c[2]).bit_length()
                     xor_lv = (c[0] ^ c[1] ^ c[2]).bit_length()
-                    assert (c[0] | c[1] | c[2]) >> lv_before, "a pop without leaving the node"
-                    assert lv_after == xor_lv  # the reference's rule, literally
-                    multi += sum(v != 0 for v in c) > 1
+                    assert xor1 == xor_lv  # the reference's rule, literally
+                    if far:
+                        continue
+                    popped = xor_lv - 1 >= lv
+                    pops += popped
+                    multi += popped and sum(v != 0 for v in c) > 1
                     if xor_lv == or_lv:
-                        agree += 1
-                        assert lv_after == or_lv and lv_after > lv_before
+                        agree += popped
+                        if popped:
+                            assert (c[0] | c[1] | c[2]) >> lv, "a pop without leaving the node"
                     else:
                         quirk += 1
+                        skipped += (or_lv - 1 >= lv) and not popped
     assert agree > 10000 and multi > agree // 2
     assert quirk <= agree // 100
-    print(f"pops: {agree} at the geometric level ({multi} with several axes changed), {quirk} XOR-only")
+    print(f"pops: {pops}, {agree} at the geometric level ({multi} with several axes changed); "
+          f"{quirk} steps where the XOR level differs from the OR level, {skipped} of them skip a pop")

@@ -37,6 +37,9 @@
 namespace {
 
 constexpr int RAY_BLOCK = 256;
+#ifndef SVO_TRIP_UNROLL
+#define SVO_TRIP_UNROLL 2  // trips per wave-uniform exit check
+#endif
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t bc8(uint32_t v) { return (uint32_t)__popc(v & 0xffu); }
@@ -287,22 +290,27 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
     return true;
 }
 
-// LDS: node stack, row `rek` (1..D-1) x RAY_BLOCK lanes, then the node cache
+// LDS: node stack, rows 0..D+1 x RAY_BLOCK lanes, then the node cache.
+// Row `rek` (1..D-1) holds the ray's node at depth D-rek; rows D and D+1 hold
+// the root.  So STK(rek) is always the node being tested and STK(rek+1) its
+// parent, and a pop reads both rows without a range check: the reference's
+// rows D and D+1 (kernel.cl:635-637) belong to no node of the ray and are
+// read only where they hold the root (rek == D) or where the value is not
+// consulted (a normal depth-1 node's parent, rek == D-1, depths >= 8).
 #define STK(rek) stk[(rek) * RAY_BLOCK + tid]
+constexpr int STK_ROWS_EXTRA = 2;  // rows D and D+1
 
 // One trip of the CAST_RAY do-while loop (kernel.cl:152-211).  Returns true
 // when the loop exits.  A lone wave in the tail of a launch issues one
-// instruction per 4 cycles, so the trip is written for a short instruction
-// path: selects instead of exec-mask branches, the pop test as one compare
-// (floor(log2 m) >= rek  <=>  m >= 2^rek, m > 0), and the child fetch of
-// fetchChildOffset (kernel.cl:32-62) as a single word read.  The compiler
-// merges the two descend regions, so the read completes before the DDA step
-// of the other lanes runs; a version that kept them apart (the wait after
-// the step) was slower, the extra branch region costing every lone ray more
-// than the overlap saved divergent waves.
+// instruction per 4 cycles, so the trip is written for a short path with few
+// exec-mask regions: one divergent branch (descend or step), selects inside
+// each body, the pop as unconditional stack reads at the new level (no pop
+// re-reads the current node and its parent, which the stack also holds), the
+// pop test as one compare (floor(log2 m) >= rek  <=>  m >= 2^rek, m > 0), and
+// the child fetch of fetchChildOffset (kernel.cl:32-62) as one word read.
 template <bool kCache>
-__device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, OctBuf ob, uint32_t* stk, const uint32_t* cache,
-                                         int tid)
+__device__ __forceinline__ uint32_t step_ray(Ray& r, const RayArgs& a, OctBuf ob, uint32_t* stk,
+                                             const uint32_t* cache, int tid)
 {
     const int D = a.depth;
     const int rek = r.rek;
@@ -311,30 +319,38 @@ __device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, OctBuf ob, ui
     const uint32_t node = r.node;
     const uint32_t test = node & (1u << ni);
     r.test = test;
-
-    // ---- descend, part 1: address and read (kernel.cl:42-61).  At rek == 1
-    // the fetched word is never consumed (the hit exits and fetchColor reads
-    // only node_before and the stack); the read there stays inside the block.
-    uint32_t w = 0, nadd = 0;
-    bool synth = false;
+    uint32_t done;  // 0/1 as a value (not a lane mask): no mask merges at the join
     if (test) {
+        // ---- descend (kernel.cl:42-61, :169-178).  Block nodes address their
+        // children from local_root, set on entry from a normal parent; at
+        // rek == 2 the depth-(D-1) ref is synthesised from the byte node.  At
+        // rek == 1 the fetched word is never consumed (the hit exits and
+        // fetchColor reads only node_before and the stack).
         const uint32_t n0 = node >> 9;
         const bool blk = (node & 256u) != 0;
         const bool enter = blk && !(r.before & 256u);
-        r.lroot = enter ? (n0 << 6) : r.lroot;
-        nadd = (uint32_t)__popc(node & ((test << 1) - 1u));
-        synth = blk && rek == 2;
-        const uint32_t base = (enter ? 0u : n0) + r.lroot;
-        const uint32_t idx = blk ? base + (synth ? (nadd >> 2) : nadd) : n0 + (uint32_t)ni;
+        const uint32_t lroot = enter ? (n0 << 6) : r.lroot;
+        const uint32_t nadd = (uint32_t)__popc(node & ((test << 1) - 1u));
+        const bool synth = blk && rek == 2;
+        const uint32_t off = synth ? (nadd >> 2) : nadd;
+        const uint32_t idx = (enter ? 0u : n0) + (blk ? lroot + off : (uint32_t)ni);
+        uint32_t w;
         if (kCache && idx < a.cache_words)
             w = cache[idx];
         else
             w = oct_load(ob, idx);
-    }
-
-    bool done = false;
-    if (!test) {
-        // DDA step to the nearest cell face (kernel.cl:181-194)
+        const uint32_t child = synth ? ((w >> ((nadd & 3u) << 3)) & 255u) + 256u + (nadd << 9) : w;
+        r.lroot = lroot;
+        r.before = node;
+        r.node = child;
+        done = (uint32_t)(rek <= r.lod);
+        // the stack write is unconditional: when the ray stops here (rek-1 is
+        // then 0..lod-1) the row is never read again (finish reads row 2 only
+        // for a ray at rek 1, and row 0 never)
+        STK(rek - 1) = child;
+        r.rek = done ? rek : rek - 1;
+    } else {
+        // ---- DDA step to the nearest cell face (kernel.cl:181-194)
         const float mx = (float)((p0x + 1) << rek) - r.px;
         const float my = (float)((p0y + 1) << rek) - r.py;
         const float mz = (float)((p0z + 1) << rek) - r.pz;
@@ -352,7 +368,7 @@ __device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, OctBuf ob, ui
         r.pz += gz * m;
         const int nx = (int)r.px, ny = (int)r.py, nz = (int)r.pz;
         r.dist += dist;
-        // pop level (kernel.cl:197-209)
+        // pop level (kernel.cl:197-209): the XOR of the three axes' changes
         const int DAND = (1 << D) - 1;
         const int x0ry = r.iy ^ ny;
         const uint32_t xm = (uint32_t)((r.ix ^ nx) ^ x0ry ^ (r.iz ^ nz)) & (uint32_t)DAND;
@@ -360,32 +376,16 @@ __device__ __forceinline__ bool step_ray(Ray& r, const RayArgs& a, OctBuf ob, ui
         r.iy = ny;
         r.iz = nz;
         const bool far = r.dist > 400000.0f;
-        if (!far && xm >= (1u << rek)) {
-            const int nr = 32 - __clz((int)xm);  // rek_new + 1
-            r.rek = nr;
-            const uint32_t sn = STK(min(nr, D - 1));
-            const uint32_t sb = STK(min(nr + 1, D - 1));
-            // rows D and D+1 belong to no node of this ray; the reference
-            // reads a neighbour's slots there, values overwritten by the root
-            // at rek == D and consulted only for block nodes at rek == D-1
-            // (depth-1 nodes are normal nodes for D >= 8)
-            r.node = nr >= D ? a.root : sn;
-            r.before = nr + 1 >= D ? a.root : sb;
-            const bool l = r.dist > r.lodsw;
-            r.lodsw = l ? r.lodsw * 2.0f : r.lodsw;
-            r.lod += l ? 1 : 0;
-        }
-        done = far || (x0ry & (2 * DAND + 2)) != 0;
-    } else {
-        // ---- descend, part 2 (kernel.cl:58, :169-178)
-        const uint32_t sb = (w >> ((nadd & 3u) << 3)) & 255u;
-        r.node = synth ? sb + 256u + (nadd << 9) : w;
-        r.before = node;
-        done = rek <= r.lod;
-        if (!done) {
-            r.rek = rek - 1;
-            STK(rek - 1) = r.node;
-        }
+        const bool pop = !far && xm >= (1u << rek);
+        const int nr = pop ? 32 - __clz((int)xm) : rek;  // rek_new + 1, or stay
+        r.rek = nr;
+        r.node = STK(nr);
+        r.before = STK(nr + 1);
+        // LOD switch, evaluated on pops only (kernel.cl:209)
+        const bool l = pop && r.dist > r.lodsw;
+        r.lodsw = l ? r.lodsw * 2.0f : r.lodsw;
+        r.lod += l ? 1 : 0;
+        done = (uint32_t)far | ((uint32_t)x0ry >> (D + 1) & 1u);
     }
     return done;
 }
@@ -425,9 +425,11 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
 {
     extern __shared__ uint32_t smem[];
     uint32_t* stk = smem;
-    uint32_t* cache = smem + a.depth * RAY_BLOCK;
+    uint32_t* cache = smem + (a.depth + STK_ROWS_EXTRA) * RAY_BLOCK;
     const int tid = threadIdx.x;
     for (uint32_t i = (uint32_t)tid; i < a.cache_words; i += RAY_BLOCK) cache[i] = a.oct[i];
+    STK(a.depth) = a.root;  // no ray writes rows D and D+1
+    STK(a.depth + 1) = a.root;
     __syncthreads();
     const OctBuf ob = oct_buffer(a.oct, a.oct_bytes);
     const int lane = __lane_id();
@@ -491,24 +493,26 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
             if (exhausted) break;
             continue;
         }
-        if (active) {
-            // at most SVO_MAX_RAY_ITERS trips per ray; a ray still running
-            // then finishes at its current node
-            const uint32_t lim = min((uint32_t)a.iters, SVO_MAX_RAY_ITERS - r.guard);
-            uint32_t it = 0;
-            bool done = false;
-            for (; it < lim; ++it) {
-                if (step_ray<kCache>(r, a, ob, stk, cache, tid)) {
-                    done = true;
-                    ++it;
-                    break;
+        // A batch of up to `iters` trips.  Lanes whose ray ends stop (run =
+        // false) and the batch ends early once no lane runs; the loop exit is
+        // wave-uniform, so a trip carries no per-lane loop bookkeeping.  At
+        // most SVO_MAX_RAY_ITERS trips per ray: a ray still running then
+        // finishes at its current node.
+        uint32_t run = active ? 1u : 0u;
+        for (int it = 0; it < a.iters; it += SVO_TRIP_UNROLL) {
+#pragma unroll
+            for (int u = 0; u < SVO_TRIP_UNROLL; ++u) {
+                if (run) {
+                    const uint32_t d = step_ray<kCache>(r, a, ob, stk, cache, tid);
+                    r.guard++;
+                    run = (d | (uint32_t)(r.guard >= SVO_MAX_RAY_ITERS)) ^ 1u;
                 }
             }
-            r.guard += it;
-            if (done || r.guard >= SVO_MAX_RAY_ITERS) {
-                finish_ray(r, a, ob, stk, cache, tid);
-                active = false;
-            }
+            if (__ballot(run != 0u) == 0ull) break;
+        }
+        if (active && !run) {
+            finish_ray(r, a, ob, stk, cache, tid);
+            active = false;
         }
     }
     if (lane == 0 && traced) atomicAdd(a.ray_count, (unsigned long long)traced);
@@ -1190,8 +1194,10 @@ static int frame_end(svo_ctx* c)
     return SVO_OK;
 }
 
-// After svo_read_image_async the next frame renders into the other image
-// buffer; that buffer's own last copy (two frames back) must be complete.
+// After svo_read_image_async the next call that rewrites the whole image (a
+// frame, a batch frame, svo_colorize) renders into the other image buffer;
+// that buffer's own last copy (two frames back) must be complete.  Calls
+// that write part of the image or none keep the current buffer.
 static void take_image(svo_ctx* c)
 {
     if (!c->image_flip || !c->image_buf[1]) return;
@@ -1206,7 +1212,6 @@ static void take_image(svo_ctx* c)
 
 static View make_view(svo_ctx* c, const svo_frame_params* p)
 {
-    take_image(c);
     View v;
     memset(&v, 0, sizeof(v));
     v.screen = c->d_screen;
@@ -1355,7 +1360,7 @@ static int launch_rays_v(svo_ctx* c, hipStream_t stream, const View& v, const Vi
     a.cache_last = a.cache_words ? a.cache_words - 1 : 0;
     a.oct_bytes = (uint32_t)std::min<size_t>((c->n_words + kOctPad) * 4, 0xfffffffcull);
     const unsigned grid = (unsigned)(c->n_cu * c->wg_per_cu);
-    const size_t lds = ((size_t)c->depth * RAY_BLOCK + std::max(a.cache_words, 1u)) * 4;
+    const size_t lds = ((size_t)(c->depth + STK_ROWS_EXTRA) * RAY_BLOCK + std::max(a.cache_words, 1u)) * 4;
     if (a.cache_words)
         hipLaunchKernelGGL(k_rays<true>, dim3(grid), dim3(RAY_BLOCK), lds, stream, a);
     else
@@ -1470,6 +1475,7 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     RC(need_octree(c));
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     HIPCHECK(hipSetDevice(c->device));
+    take_image(c);
     frame_begin(c);
     if (c->use_graph && c->timing != 1 && p->frame >= 2) {
         RC(frame_warp_graph(c, p));
@@ -1516,6 +1522,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     if (max_cnn_holes < 0 || max_cnn_holes > 256) return svo_fail(SVO_ERR_ARG, "max_cnn_holes outside 0..256");
     HIPCHECK(hipSetDevice(c->device));
     RC(ensure_cnn_buffers(c));
+    take_image(c);
     frame_begin(c);
     RC(warp_compact(c, p));
     int ax, ay, ex, ey;
@@ -1571,6 +1578,7 @@ extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)
     RC(need_octree(c));
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     HIPCHECK(hipSetDevice(c->device));
+    take_image(c);
     frame_begin(c);
     View v = make_view(c, p);
     set_rays(v, nullptr, nullptr, 0, 0, c->res_x, c->res_y, c->d_image);
@@ -1746,6 +1754,7 @@ extern "C" int svo_colorize(svo_ctx* c)
 {
     RC(need_frame(c));
     HIPCHECK(hipSetDevice(c->device));
+    take_image(c);
     return launch_fill(single(c), make_view(c, nullptr), c->d_screen, 0, 1);
 }
 
@@ -1894,6 +1903,7 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
                                0, b->stream, c->d_screen, HOLE, 4 * c->N);
             c->plane1_holes = true;
         }
+        take_image(c);
         View v = make_view(c, p);
         v.plane_lo = c->plane1_holes ? 2 : 1;
         v.n_planes = 3 - v.plane_lo;

@@ -28,6 +28,8 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 CNN_TILE_FLOPS = 2 * (484 * 9 * 5 * 32 + 400 * 9 * 32 * 32 + 324 * 9 * 32 * 32 + 256 * 9 * 32 * 3)
 CNN_TILE_MFMA_FLOPS = 32 * 2 * 9 * 8192 + 26 * 2 * 9 * 16384 + 22 * 2 * 9 * 16384 + 16 * 9 * 16384
 METRIC = "frames/s & p99 frame ms at 1920x1024 (warp+fill vs full raycast); Mrays/s"
+TUNE_SEED_OFF = 500  # the autotuner's held-out camera paths
+CPU_SHARE = 16  # host CPUs per GPU on the measurement boxes
 
 
 def log(*a):
@@ -74,9 +76,12 @@ def run_gpu(args, name):
         c.upload_octree(words, root, depth)
         c.resize(rx, ry)
     res = {"octree_build": build}
+    # the autotuner replays frames of a held-out path (same scene and motion
+    # bounds, another seed), so the knobs are not picked on the timed frames
+    trows = W.path(name, 70, seed_off=TUNE_SEED_OFF)
+    tp = [W.frame_params(name, trows, f) for f in range(70)]
     if args.tune:
         from svo_amd import autotune as A
-        tp = [W.frame_params(name, rows, f) for f in range(min(total, 70))]
         for c, mode in ((warp, "warp"), (full, "full")):
             best, ms, _ = A.tune(c, tp, warmup=10, mode=mode)
             res[f"tuned_{mode}"] = dict(best, ms=round(ms, 4))
@@ -217,7 +222,6 @@ def run_gpu(args, name):
             lw.upload_octree(words, root, depth)
             lw.resize(rx, ry)
             if args.tune:  # learned frames trace ~2x the rays of warp frames: their own knobs
-                tp = [W.frame_params(name, rows, f) for f in range(min(total, 70))]
                 best, ms, _ = A.tune(lw, tp, warmup=10, mode="learned", cnn=cnn, max_cnn_holes=args.max_cnn_holes)
                 res["tuned_learned"] = dict(best, ms=round(ms, 4))
                 log(f"[bench] autotune learned: {best} -> {ms:.4f} ms/frame")
@@ -263,9 +267,60 @@ def run_gpu(args, name):
     return res
 
 
-def run_stress(args, name="cave_4096", n_timed=60):
-    """Divergence stress: the depth-12 cave with a fast camera (up to 32
-    voxels and 5 deg yaw per frame).  Scene generated and octree built on the
+GATE_DEPTHS = [0.0, 4096.0, 1024.0, 256.0, 64.0]
+
+
+def run_gate_curve(args, name, tuned=None):
+    """Learned frames (gate + fused CNN, committed weights) along the gate's
+    depth rule: for each depth limit (0 = off; world units) the back-to-back
+    frames/s over the timed frames and the PSNR against full raycasting on a
+    held-out path, with the share of pixels sent back to raycasting."""
+    import svo_amd as S
+    from svo_amd import autotune as A
+    from svo_amd import learned as L
+    from svo_amd import workloads as W
+
+    if not os.path.exists(L.WEIGHTS):
+        return None
+    w = W.WORKLOADS[name]
+    words, root, depth, _ = W.octree(name)
+    rx, ry = w["res"]
+    rows = W.path(name, args.warmup + args.steps)
+    params = [W.frame_params(name, rows, f) for f in range(args.warmup + args.steps)]
+    cnn = S.Cnn(0)
+    L.load(cnn, L.WEIGHTS)
+    out = []
+    for d in GATE_DEPTHS:
+        c = S.Context(0)
+        c.upload_octree(words, root, depth)
+        c.resize(rx, ry)
+        if tuned:
+            A.apply(c, tuned)
+        c.set_gate_depth(d)
+        for f in range(args.warmup):
+            c.frame_warp_learned(cnn, params[f], args.max_cnn_holes)
+        c.sync()
+        t0 = time.perf_counter()
+        for f in range(args.warmup, args.warmup + args.steps):
+            c.frame_warp_learned(cnn, params[f], args.max_cnn_holes)
+        c.sync()
+        fps = args.steps / (time.perf_counter() - t0)
+        gate_rays = c.gate_counts()[1]
+        c.close()
+        allp, holes = L.evaluate(cnn, name, seed_off=1000, n_frames=24, skip=8, max_cnn_holes=args.max_cnn_holes,
+                                 gate_depth=d)
+        row = {"gate_depth": d, "fps": round(fps, 2), "psnr_learned_db": round(allp["learned"], 2),
+               "psnr_gate_db": round(allp["gate"], 2), "psnr_plain_db": round(allp["plain"], 2),
+               "gate_rays_last_frame": int(gate_rays)}
+        log(f"[bench] gate curve {name}: {row}")
+        out.append(row)
+    cnn.close()
+    return out
+
+
+def run_stress(args, name="cave_4096", n_timed=190):
+    """Divergence stress: the depth-12 cave over the whole 4096^3 volume with
+    a fast camera (up to 32 voxels and 5 deg yaw per frame).  Scene generated and octree built on the
     GPU; warp frames (throughput and per-frame events, hole fraction read per
     frame) and full frames."""
     import svo_amd as S
@@ -314,18 +369,19 @@ def run_stress(args, name="cave_4096", n_timed=60):
         holes.append(w2.hole_count() / (rx * ry))
     full.enable_timing(mode=2)
     fm = []
-    for f in range(10, min(total, 30)):
+    for f in range(10, total):
         full.frame_full(params[f])
         full.sync()
         fm.append(full.frame_time())
     for c in (warp, full, w2):
         c.close()
-    return {"workload": f"{name}: {rx}x{ry}, SVO depth {depth}, 3-D noise cave shell, up to {w['vox']:.0f} voxels "
-                        f"and {w['yaw']:.0f} deg yaw per frame",
+    return {"workload": f"{name}: {rx}x{ry}, SVO depth {depth}, 3-D noise cave shell over the whole volume, up to "
+                        f"{w['vox']:.0f} voxels and {w['yaw']:.0f} deg yaw per frame",
             "voxels": int(nvox), "octree_words": int(len(words)), "scene_gpu_s": round(t_gen, 2),
             "build_gpu_s": round(t_build, 2), "warp_fps": round(n_timed / t_warp, 2),
             "warp_p50_ms": round(pct(ms, 50), 4), "warp_p99_ms": round(pct(ms, 99), 4),
             "full_p50_ms": round(pct(fm, 50), 4), "full_fps_from_p50": round(1000.0 / pct(fm, 50), 2),
+            "frames": total, "timed_frames": n_timed,
             "hole_list_fraction_mean": round(float(np.mean(holes)), 4),
             "hole_list_fraction_max": round(float(np.max(holes)), 4),
             "rays_per_warp_frame": round(rays / n_timed)}
@@ -389,9 +445,12 @@ def run_batch(args):
         t = time.perf_counter() - t0
         return t, ctxs[0].ray_count(reset=True)
 
-    if args.tune:
+    if args.tune:  # on held-out view paths
         from svo_amd import autotune as A
-        best, ms, _ = A.tune_batch(batch, frames[:min(n_frames, 16)], warmup=6)
+        tpaths = [W.view_path(v, seed_off=TUNE_SEED_OFF) for v in views]
+        tframes = [[W.frame_params(v["scene"], tpaths[i], f) for i, v in enumerate(views)]
+                   for f in range(min(n_frames, 16))]
+        best, ms, _ = A.tune_batch(batch, tframes, warmup=6)
         out["tuned"] = dict(best, ms=round(ms, 4))
         log(f"[bench] batch autotune: {best} -> {ms:.4f} ms/batch frame")
         for c in ctxs[1:]:  # the view-by-view run uses the same knobs (faster there than the defaults)
@@ -428,6 +487,47 @@ def run_batch(args):
     return out
 
 
+def host_cpu():
+    """The host the CPU baseline runs on: logical CPUs (nproc), the CPUs this
+    process may run on, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return dict(nproc=os.cpu_count(), allowed=allowed, model=model)
+
+
+def fetch_words_per_ray(name, first, last, threads):
+    """F-bar of the roofline: mean octree words a ray of the warp frames
+    first..last-1 reads (the oracle replays the path from frame 0 and counts
+    them on exactly those frames, the ones the GPU timing covers)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from svo_amd import workloads as W
+
+    w = W.WORKLOADS[name]
+    words, root, depth, _ = W.octree(name)
+    rows = W.path(name)
+    fr = O.Frame(*w["res"], words, root, depth, nthreads=threads)
+    for f in range(first):
+        fr.frame_warp(O.params_from(W.frame_params(name, rows, f)))
+    st0 = fr.stats
+    for f in range(first, last):
+        fr.frame_warp(O.params_from(W.frame_params(name, rows, f)))
+    st1 = fr.stats
+    fr.close()
+    return (st1["fetch_words"] - st0["fetch_words"]) / max(1, st1["rays"] - st0["rays"])
+
+
 def run_cpu_baseline(name, n_warp_frames, n_full_frames, threads):
     """Oracle (the CPU restatement, kind "port") on a bounded sample of the
     same workload: frames 0-1 (full raycasts through the hole path) untimed,
@@ -443,25 +543,18 @@ def run_cpu_baseline(name, n_warp_frames, n_full_frames, threads):
     fr = O.Frame(rx, ry, words, root, depth, nthreads=threads)
     for f in range(2):
         fr.frame_warp(O.params_from(W.frame_params(name, rows, f)))
-    st0 = fr.stats
     t0 = time.perf_counter()
     for f in range(2, 2 + n_warp_frames):
         fr.frame_warp(O.params_from(W.frame_params(name, rows, f)))
     t_warp = time.perf_counter() - t0
-    st1 = fr.stats
-    rays = st1["rays"] - st0["rays"]
-    fetch = st1["fetch_words"] - st0["fetch_words"]
     fr2 = O.Frame(rx, ry, words, root, depth, nthreads=threads)
     t0 = time.perf_counter()
     for f in range(2, 2 + n_full_frames):
         fr2.frame_full(O.params_from(W.frame_params(name, rows, f)))
     t_full = time.perf_counter() - t0
-    fst = fr2.stats
     fr.close()
     fr2.close()
     return dict(warp_fps=n_warp_frames / t_warp, full_fps=n_full_frames / t_full, full_frames=n_full_frames,
-                fetch_per_ray_warp=fetch / max(1, rays),
-                fetch_per_ray_full=fst["fetch_words"] / max(1, fst["rays"]),
                 t=t_warp + t_full, frames=n_warp_frames)
 
 
@@ -478,6 +571,8 @@ def main():
     ap.add_argument("--no-learned", dest="learned", action="store_false", help="skip the learned-fill leg")
     ap.add_argument("--max-cnn-holes", type=int, default=64)
     ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the batched-views leg")
+    ap.add_argument("--no-gate-curve", dest="gate_curve", action="store_false",
+                    help="skip the learned-fill frames/s vs PSNR sweep of the gate's depth rule")
     ap.add_argument("--no-stress", dest="stress", action="store_false",
                     help="skip the divergence-stress leg (depth-12 cave)")
     ap.add_argument("--batch-warmup", type=int, default=8)
@@ -492,6 +587,11 @@ def main():
         raise SystemExit("--warmup must be >= 2 (frames 0-1 are full clears)")
 
     g = run_gpu(args, args.workload)
+    curve = None
+    if args.learned and args.legs == "all" and args.gate_curve:
+        tl = g.get("tuned_learned")
+        curve = {"flythrough_2048": run_gate_curve(args, args.workload, tl),
+                 "flythrough_2048_flat": run_gate_curve(args, "flythrough_2048_flat", tl)}
     batch = run_batch(args) if (args.batch and args.legs == "all") else None
     stress = run_stress(args) if (args.stress and args.legs == "all") else None
     if stress:
@@ -499,11 +599,15 @@ def main():
     # dominant kernel: ray traversal in the warp frames
     ray_ms = g["stage_mean"].get("rays", float("nan"))
     cpu = None
+    host = host_cpu()
+    # the CPU legs use the CPUs this process may run on, at most the GPU
+    # box's CPU share of 16 per GPU
+    threads = max(1, min(CPU_SHARE, host["allowed"]))
+    fbar = fetch_words_per_ray(args.workload, args.warmup, args.warmup + args.steps, threads)
+    log(f"[bench] F-bar over warp frames {args.warmup}..{args.warmup + args.steps - 1}: {fbar:.2f} words/ray")
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
         cpu = run_cpu_baseline(args.workload, args.cpu_frames, args.cpu_full_frames, threads)
-        log(f"[bench] cpu baseline: {cpu}")
-    fbar = cpu["fetch_per_ray_warp"] if cpu else None
+        log(f"[bench] cpu baseline: {cpu} on {host}")
     roof = None
     if fbar is not None and ray_ms == ray_ms:
         # algorithmic bytes per launch: per ray 4 B id + 4 B word + 12 B xyz + 4 B per octree word
@@ -512,7 +616,8 @@ def main():
         roof = {"kernel": "k_rays", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "per_launch_bytes": round(bytes_launch), "avg_launch_ms": round(ray_ms, 4),
-                "fetch_words_per_ray": round(fbar, 2)}
+                "fetch_words_per_ray": round(fbar, 2),
+                "fetch_words_frames": [args.warmup, args.warmup + args.steps - 1]}
         # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
         # runs of this workload, scaled by a copy kernel of known size
         tpath = os.path.join(ROOT, "profiles", "k_rays_traffic.json")
@@ -561,16 +666,23 @@ def main():
                          "peak": BF16_PEAK_TFLOPS,
                          "frac": round(g["cnn_tiles"] * CNN_TILE_FLOPS / (g["cnn_ms"] * 1e-3) / 1e12
                                        / BF16_PEAK_TFLOPS, 5),
-                         "avg_launch_ms": round(g["cnn_ms"], 4), "flops_per_tile": CNN_TILE_FLOPS}},
+                         "avg_launch_ms": round(g["cnn_ms"], 4), "flops_per_tile": CNN_TILE_FLOPS},
+            # frames/s vs PSNR along the gate's depth rule (tiles with a warped
+            # pixel deeper than gate_depth world units raycast whole); the flat
+            # scene is the same terrain without per-voxel colour jitter
+            "gate_depth_curve": curve},
         "octree_build": g["octree_build"],
         "batched_views": batch,
         "divergence_stress": stress,
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["warp_fps"], 3), "unit": "frames/s", "cores": threads, "kind": "port",
+            "host_nproc": host["nproc"], "host_cpus_allowed": host["allowed"], "cpu_model": host["model"],
+            "full_fps": round(cpu["full_fps"], 3),
             "sample": f"{cpu['frames']} warp frames (after 2 clear frames) + {cpu['full_frames']} full "
-                      f"frames of the same workload, oracle restatement, {threads} threads, "
-                      f"{cpu['t']:.1f} s; full raycast {cpu['full_fps']:.3f} frames/s"},
+                      f"frames of the same workload, oracle restatement, {threads} threads on "
+                      f"{host['allowed']} allowed of {host['nproc']} host CPUs ({host['model']}), "
+                      f"{cpu['t']:.1f} s"},
     }
     print(json.dumps(out), flush=True)
 

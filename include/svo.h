@@ -140,13 +140,15 @@ int svo_colorize(svo_ctx* ctx);
 /* ocl_copy_to_host (ocl.h:215-221): blocking read of `bytes` at byte_ofs. */
 int svo_read(svo_ctx* ctx, int buf, size_t byte_ofs, size_t bytes, void* dst);
 /* Non-blocking copy of the last frame's colorized image (SVO_BUF_IMAGE,
- * N uint32) into dst, on a copy stream of its own: the next frame renders
+ * N uint32) into dst, on a copy stream of its own: the next call that
+ * rewrites the whole image (a frame, a batch frame or svo_colorize) renders
  * into a second image buffer meanwhile, so the copy overlaps it (the
  * reference uploads the image to a GL pixel buffer instead, raycast.h:
- * 425-455).  dst should be pinned host memory (svo_host_alloc) for the
- * copy to run asynchronously; it holds the image once svo_sync returns.  A
- * buffer is reused two frames later, after its copy has completed.  For a
- * view of a batch, call svo_batch_sync first. */
+ * 425-455); calls that write part of the image or none keep the buffer.
+ * dst should be pinned host memory (svo_host_alloc) for the copy to run
+ * asynchronously; it holds the image once svo_sync returns.  A buffer is
+ * reused two frames later, after its copy has completed.  For a view of a
+ * batch, call svo_batch_sync first. */
 int svo_read_image_async(svo_ctx* ctx, uint32_t* dst);
 /* Pinned (page-locked) host memory for svo_read_image_async destinations,
  * and its release.  No reference counterpart (ocl_copy_to_host, ocl.h:215,
@@ -217,12 +219,15 @@ int svo_image_save_png(const char* path, const uint32_t* image, int w, int h);
 /* Seeded synthetic scenes (voxel streams for svo_build_octree).
  *   kind 1: value-noise heightfield shell (2 octaves)
  *   kind 2: 4-octave fractal terrain shell + n_spheres sphere shells
- *   kind 3: 3-D fractal-noise cave, boundary voxels only
+ *   kind 3: 3-D fractal-noise cave in a slab of 1/8 of the world's height,
+ *           boundary voxels only
+ *   kind 4: the same cave over the whole height of the world
+ *   kind 5: kind 2 without the per-voxel colour jitter (colour by height)
  * *xyz / *rgba are malloc'ed (svo_free). */
 int svo_scene_generate(int kind, uint64_t seed, int depth, int n_spheres,
                        uint16_t** xyz, uint32_t** rgba, size_t* n);
-/* svo_scene_generate for kind 3 (the cave) on a GPU: the same voxel stream,
- * bit for bit, in seconds at depth 12 (the host needs hours there). */
+/* svo_scene_generate for kinds 3 and 4 (the caves) on a GPU: the same voxel
+ * stream, bit for bit, in seconds at depth 12 (the host needs hours there). */
 int svo_scene_generate_gpu(int device, int kind, uint64_t seed, int depth,
                            uint16_t** xyz, uint32_t** rgba, size_t* n);
 
@@ -356,8 +361,17 @@ int svo_cnn_set_fused(svo_cnn* cnn, int mode);
  * instead (same results).  cnn may be NULL: the gate alone, fillhole2
  * colours everywhere. */
 int svo_frame_warp_learned(svo_ctx* ctx, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
+/* The gate's confidence rule for reprojected pixels (learned frames; no
+ * counterpart in the reference): a 16x16 tile holding a reprojected pixel
+ * whose camera depth exceeds max_depth world units (where a pixel spans
+ * several voxels, so the warped sample's colour is rarely the one the
+ * pixel's ray meets) is raycast whole in the frame's ray launch -- its
+ * residual holes and its reprojected pixels outside the refresh rectangle.
+ * 0 (default) = off.  Trades frames/s for PSNR against full raycasting. */
+int svo_set_gate_depth(svo_ctx* ctx, float max_depth);
 /* Gate outcome of the last svo_frame_warp_learned (blocking): tiles sent to
- * the CNN and residual-hole pixels sent back to raycasting. */
+ * the CNN and pixels sent back to raycasting (residual holes and, with the
+ * depth rule, the reprojected pixels of whole tiles). */
 int svo_gate_counts(svo_ctx* ctx, uint32_t* cnn_tiles, uint32_t* gate_rays);
 
 /* ---- batched views (no counterpart in the reference) -------------------

@@ -91,7 +91,7 @@ _tiles = _sig("orc_tile_schedule", None, C.c_int, C.c_int, C.c_int, C.POINTER(C.
               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int))
 _warp = _sig("orc_frame_warp", None, _S, _P, C.c_int)
 _full = _sig("orc_frame_full", None, _S, _P)
-_gated = _sig("orc_frame_warp_gated", None, _S, _P, C.c_int)
+_gated = _sig("orc_frame_warp_gated", None, _S, _P, C.c_int, C.c_float)
 bitcount8 = _sig("orc_bitcount8", C.c_uint32, C.c_uint32)
 colorize_word = _sig("orc_colorize_word", C.c_uint32, C.c_uint32)
 
@@ -299,5 +299,5 @@ class Frame:
     def frame_full(self, p):
         _full(C.byref(self.s), C.byref(p))
 
-    def frame_warp_gated(self, p, max_holes):
-        _gated(C.byref(self.s), C.byref(p), int(max_holes))
+    def frame_warp_gated(self, p, max_holes, max_depth=0.0):
+        _gated(C.byref(self.s), C.byref(p), int(max_holes), float(max_depth))

@@ -963,7 +963,7 @@ void orc_frame_warp(orc_state* s, const orc_params* p, int inplace_filter)
  * those pixels raycast along with the hole and tile rays.  Rays write only
  * their own pixel, so their order does not matter.  The CNN part of the
  * learned fill is not restated here. */
-void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes)
+void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes, float max_depth)
 {
     const int rx = s->res_x, ry = s->res_y;
     size_t N = (size_t)rx * ry;
@@ -993,20 +993,38 @@ void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes)
             if (x >= ax && x < ax + ex && y >= ay && y < ay + ey) continue;
             resid[o] = 1;
         }
+    /* the depth rule: reprojected pixels outside the refresh rectangle, and
+     * whether one lies deeper than max_depth (camera depth = coordinate w) */
+    uint8_t* proj = (uint8_t*)calloc(N, 1);
+    for (int y = 0; max_depth > 0.0f && y < ry; y++)
+        for (int x = 0; x < rx; x++) {
+            size_t o = (size_t)y * rx + x;
+            if (s->screen[o] == SCREEN_HOLE) continue;
+            if (x >= ax && x < ax + ex && y >= ay && y < ay + ey) continue;
+            proj[o] = s->coords[4 * o + 3] > max_depth ? 2 : 1;
+        }
     int tx = (rx + 15) / 16, ty = (ry + 15) / 16;
     uint32_t* list = (uint32_t*)malloc(N * 4);
     size_t n = 0;
     for (int t = 0; t < tx * ty; t++) {
-        int x0 = (t % tx) * 16, y0 = (t / tx) * 16, h = 0;
+        int x0 = (t % tx) * 16, y0 = (t / tx) * 16, h = 0, far = 0;
         for (int j = 0; j < 16; j++)
             for (int i = 0; i < 16; i++)
-                if (x0 + i < rx && y0 + j < ry && resid[(size_t)(y0 + j) * rx + x0 + i]) h++;
-        if (h <= max_holes) continue;
+                if (x0 + i < rx && y0 + j < ry) {
+                    size_t o = (size_t)(y0 + j) * rx + x0 + i;
+                    h += resid[o];
+                    far += proj[o] == 2;
+                }
+        if (!far && h <= max_holes) continue;  /* CNN tile (or nothing to do) */
         for (int j = 0; j < 16; j++)
-            for (int i = 0; i < 16; i++)
-                if (x0 + i < rx && y0 + j < ry && resid[(size_t)(y0 + j) * rx + x0 + i])
+            for (int i = 0; i < 16; i++) {
+                if (x0 + i >= rx || y0 + j >= ry) continue;
+                size_t o = (size_t)(y0 + j) * rx + x0 + i;
+                if (resid[o] || (far && proj[o]))
                     list[n++] = (uint32_t)(x0 + i) | ((uint32_t)(y0 + j) << 16);
+            }
     }
+    free(proj);
     free(resid);
     if (s->ids[0] > 0) orc_raycast_holes(s, p);
     orc_raycast_rect(s, p, ax, ay, ex, ey);

@@ -20,16 +20,19 @@
  * Parity pins (what checks each function; see tests/):
  *   orc_build_octree    word-for-word vs the reference octree.h compiled on
  *                       the host (oracle/_ref) + hand-derived 1/8-voxel arrays
+ *                       (like the reference, its 24-bit node handles wrap
+ *                       above 2^24 pointer nodes, octree.h:74-76)
  *   orc_trace_pixel     dense-grid FP64 first-hit DDA on small scenes
  *                       (hit voxel + colour byte); LOD hits against the
  *                       octree's node colours (creator voxel, first leaf);
  *                       pop levels against the geometric rule (one above
- *                       the highest changed position bit) via the pop log;
- *                       trajectory quirks (the XOR pop on edge and corner
- *                       crossings, when the LOD increments, miss colour)
- *                       follow kernel.cl:114-214 and are PARTIALLY PINNED
- *                       (no reference run of kernel.cl is possible on this
- *                       host: no CPU OpenCL runtime)
+ *                       the highest changed position bit) via the step log;
+ *                       the whole step/pop sequence, the LOD stop level,
+ *                       the exit kind, exit node and colour against an
+ *                       independent FP64 model of the traversal, with
+ *                       planted LOD and exit-node mistakes failing it
+ *                       (orc_set_mutation)
+ *   orc_frame_warp_gated the gate's depth rule against full raycasts
  *   orc_proj            zero-motion invariant + literal sequential semantics
  *   orc_counthole/...   closed forms + brute force
  *   orc_fillhole2_*     closed-form per-pixel rules
@@ -150,8 +153,10 @@ void orc_frame_full(orc_state* s, const orc_params* p);
 /* gated warp frame (learned-fill gate, no reference counterpart): tiles with
  * more than max_holes residual holes -- hole pixels that neither the hole
  * list nor the refresh rectangle traces -- have them raycast with the frame's
- * hole and tile rays */
-void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes);
+ * hole and tile rays; with max_depth > 0 a tile holding a reprojected pixel
+ * (outside the rectangle) of camera depth above max_depth has its residual
+ * holes and all those reprojected pixels raycast */
+void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes, float max_depth);
 
 /* exhaustive-check helpers */
 uint32_t orc_bitcount8(uint32_t b);

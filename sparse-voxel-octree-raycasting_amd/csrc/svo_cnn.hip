@@ -351,9 +351,10 @@ __device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 
 // one group then cover 16 distinct 8-byte slots
 __device__ __forceinline__ int swz1(int p, int k) { return p * 16 + ((k ^ ((p >> 2) & 3)) << 2); }
 
-struct TileArgs {
-    // frame mode (x == null): features of the crop at tile*16 - 4 from the
-    // CNN input planes, 256 colorized words per tile to `staged`
+// One view's CNN tiles in frame mode: features of the crop at tile*16 - 4
+// from the view's CNN input planes, 256 colorized words per tile to `staged`;
+// the apply step writes them into `image` where plane 2 still holds holes.
+struct TileView {
     const uint32_t* words;
     const float4* coords;
     int res_x, res_y, tiles_x;
@@ -362,6 +363,18 @@ struct TileArgs {
     int max_tiles;
     float m0x, m0y, m0z, r20, r21, r22;
     uint32_t* staged;
+    const uint32_t* plane2;
+    uint32_t* image;
+};
+
+struct TileArgs {
+    // frame mode (x == null): one view, or a device table of n_views views
+    // whose tile lists are concatenated (first[i] = the first tile of view i,
+    // first[n_views] = all tiles)
+    TileView v;
+    const TileView* views;
+    const uint32_t* first;
+    int n_views;
     // explicit mode: x [n][24][24][16] bf16 -> out [n][16][16][3] float
     const bf16* x;
     float* out;
@@ -369,6 +382,32 @@ struct TileArgs {
     const bf16* w[4];  // [9][32][16], [9][32][32], [9][32][32], [9][16][32]
     const float* b[4];
 };
+
+// first tile of each batched view (after their gates), one thread
+__global__ void k_tile_first(const TileView* views, int n, uint32_t* first)
+{
+    uint32_t acc = 0;
+    for (int i = 0; i < n; i++) {
+        first[i] = acc;
+        acc += (uint32_t)min((int)*views[i].count, views[i].max_tiles);
+    }
+    first[n] = acc;
+}
+
+// k_cnn_apply for every view of a batch (grid y = view)
+__global__ __launch_bounds__(256) void k_cnn_apply_views(const TileView* views)
+{
+    const TileView& V = views[blockIdx.y];
+    const int nb = min((int)*V.count, V.max_tiles);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nb * 256; i += gridDim.x * 256) {
+        const int c = i / 256, q = i % 256;
+        const int t = (int)V.tiles[c];
+        const int x = (t % V.tiles_x) * 16 + (q & 15), y = (t / V.tiles_x) * 16 + (q >> 4);
+        if (x >= V.res_x || y >= V.res_y) continue;
+        const size_t o = (size_t)y * V.res_x + x;
+        if (V.plane2[o] == HOLE_WORD) V.image[o] = V.staged[i];
+    }
+}
 
 // Both tile layers compute the transposed product D[channel][pixel] (the
 // weights are the MFMA's A operand, the pixels its B operand), so a lane
@@ -493,7 +532,7 @@ __global__ __launch_bounds__(NW * 64) void k_cnn_tiles(TileArgs a)
     float* OF = (float*)P;  // 256 x 3 floats, written by layer 4 after layer 3 has consumed P
     constexpr int NT = NW * 64;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int n = a.x ? a.n : min((int)*a.count, a.max_tiles);
+    const int n = a.x ? a.n : a.views ? (int)a.first[a.n_views] : min((int)*a.v.count, a.v.max_tiles);
     if (RESIDENT) {
         stage_weights(Wl, a.w[0], 9 * 32, false, tid, NT);
         stage_weights(Wl + W_OFS1, a.w[1], 9 * 32, true, tid, NT);
@@ -502,11 +541,22 @@ __global__ __launch_bounds__(NW * 64) void k_cnn_tiles(TileArgs a)
     }
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
         __syncthreads();  // the previous crop is done with every buffer (and the weights are staged)
-        int ox = 0, oy = 0;
+        int ox = 0, oy = 0, lc = c;
+        const TileView* V = &a.v;
+        if (a.views) {  // the last view whose first tile is <= c
+            int lo = 0, hi = a.n_views - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((int)a.first[mid] <= c) lo = mid;
+                else hi = mid - 1;
+            }
+            V = &a.views[lo];
+            lc = c - (int)a.first[lo];
+        }
         if (!a.x) {
-            const int t = (int)a.tiles[c];
-            ox = (t % a.tiles_x) * 16 - 4;
-            oy = (t / a.tiles_x) * 16 - 4;
+            const int t = (int)V->tiles[lc];
+            ox = (t % V->tiles_x) * 16 - 4;
+            oy = (t / V->tiles_x) * 16 - 4;
         }
         for (int q = tid; q < 576; q += NT) {
             uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
@@ -517,16 +567,16 @@ __global__ __launch_bounds__(NW * 64) void k_cnn_tiles(TileArgs a)
             } else {
                 const int x = ox + q % 24, y = oy + q / 24;
                 float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-                if (x >= 0 && y >= 0 && x < a.res_x && y < a.res_y) {
-                    const size_t o = (size_t)y * a.res_x + x;
-                    const uint32_t w = a.words[o];
+                if (x >= 0 && y >= 0 && x < V->res_x && y < V->res_y) {
+                    const size_t o = (size_t)y * V->res_x + x;
+                    const uint32_t w = V->words[o];
                     if (w == HOLE_WORD) {
                         f[4] = 1.f;
                     } else {
                         rgb_of(w, f[0], f[1], f[2]);
-                        const float4 cc = a.coords[o];
-                        const float qx = cc.x - a.m0x, qy = cc.y - a.m0y, qz = cc.z - a.m0z;
-                        const float z = fmaxf((qx * a.r20 + qy * a.r21) + qz * a.r22, 0.f);
+                        const float4 cc = V->coords[o];
+                        const float qx = cc.x - V->m0x, qy = cc.y - V->m0y, qz = cc.z - V->m0z;
+                        const float z = fmaxf((qx * V->r20 + qy * V->r21) + qz * V->r22, 0.f);
                         f[3] = z / (z + 64.f);
                     }
                 }
@@ -569,7 +619,7 @@ __global__ __launch_bounds__(NW * 64) void k_cnn_tiles(TileArgs a)
                 o[1] = g;
                 o[2] = b;
             } else {
-                a.staged[(size_t)c * 256 + q] = rgb_word(r, g, b);
+                V->staged[(size_t)lc * 256 + q] = rgb_word(r, g, b);
             }
         }
     }
@@ -621,6 +671,7 @@ struct CnnLayer {
 
 struct svo_cnn {
     int device = 0;
+    int n_cu = 256;        // compute units of the device (tile kernel grid)
     hipStream_t stream = nullptr;
     CnnLayer L[4];
     int step = 0;
@@ -742,6 +793,9 @@ extern "C" int svo_cnn_create(svo_cnn** out, int device, uint64_t seed)
     HIPCHECK(hipSetDevice(device));
     svo_cnn* n = new svo_cnn();
     n->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        n->n_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&n->loss, 4) != hipSuccess || hipMalloc(&n->mask_count, 4) != hipSuccess) {
         svo_cnn_destroy(n);
@@ -1040,12 +1094,8 @@ static TileArgs tile_args(svo_cnn* n)
     return a;
 }
 
-static unsigned tile_grid(int max_tiles, int device, int per_cu)
+static unsigned tile_grid(int max_tiles, int cus, int per_cu)
 {
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-        cus = prop.multiProcessorCount;
     return (unsigned)std::max(1, std::min(max_tiles, per_cu * cus));
 }
 
@@ -1053,10 +1103,10 @@ static unsigned tile_grid(int max_tiles, int device, int per_cu)
 // per CU); 1 = 8 waves, weights resident (one workgroup per CU)
 // (grid > 0: that many workgroups, at most one per tile; the workgroups loop
 // over the tile list, so any count covers every tile)
-static int launch_tiles(const TileArgs& a, int mode, int max_tiles, int device, hipStream_t stream, int grid)
+static int launch_tiles(const TileArgs& a, int mode, int max_tiles, int cus, hipStream_t stream, int grid)
 {
-    const unsigned g1 = grid > 0 ? (unsigned)std::max(1, std::min(max_tiles, grid)) : tile_grid(max_tiles, device, 1);
-    const unsigned g2 = grid > 0 ? (unsigned)std::max(1, std::min(max_tiles, grid)) : tile_grid(max_tiles, device, 2);
+    const unsigned g1 = grid > 0 ? (unsigned)std::max(1, std::min(max_tiles, grid)) : tile_grid(max_tiles, cus, 1);
+    const unsigned g2 = grid > 0 ? (unsigned)std::max(1, std::min(max_tiles, grid)) : tile_grid(max_tiles, cus, 2);
     if (mode == 1)
         hipLaunchKernelGGL((k_cnn_tiles<8, true>), dim3(g1), dim3(512), 0, stream, a);
     else
@@ -1065,23 +1115,34 @@ static int launch_tiles(const TileArgs& a, int mode, int max_tiles, int device, 
     return SVO_OK;
 }
 
+static TileView tile_view(svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
+                          const uint32_t* tile_count, int max_tiles)
+{
+    TileView v;
+    memset(&v, 0, sizeof(v));
+    v.words = c->d_cnn_words;
+    v.coords = c->d_cnn_coords;
+    v.res_x = c->res_x;
+    v.res_y = c->res_y;
+    v.tiles_x = c->n_tiles_x;
+    v.tiles = tile_list;
+    v.count = tile_count;
+    v.max_tiles = max_tiles;
+    v.m0x = p->m0[0]; v.m0y = p->m0[1]; v.m0z = p->m0[2];
+    v.r20 = p->row2[0]; v.r21 = p->row2[1]; v.r22 = p->row2[2];
+    v.staged = c->d_cnn_out;
+    v.plane2 = c->d_screen + 2 * c->N;
+    v.image = c->d_image;
+    return v;
+}
+
 int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint32_t* tile_list,
                    const uint32_t* tile_count, int max_tiles, hipStream_t stream, int grid)
 {
     if (n->fused) {
         TileArgs a = tile_args(n);
-        a.words = c->d_cnn_words;
-        a.coords = c->d_cnn_coords;
-        a.res_x = c->res_x;
-        a.res_y = c->res_y;
-        a.tiles_x = c->n_tiles_x;
-        a.tiles = tile_list;
-        a.count = tile_count;
-        a.max_tiles = max_tiles;
-        a.m0x = p->m0[0]; a.m0y = p->m0[1]; a.m0z = p->m0[2];
-        a.r20 = p->row2[0]; a.r21 = p->row2[1]; a.r22 = p->row2[2];
-        a.staged = c->d_cnn_out;
-        return launch_tiles(a, n->tile_mode, max_tiles, c->device, stream, grid);
+        a.v = tile_view(c, p, tile_list, tile_count, max_tiles);
+        return launch_tiles(a, n->tile_mode, max_tiles, n->n_cu, stream, grid);
     }
     int rc = ensure_ws(n, (size_t)max_tiles * 576);
     if (rc) return rc;
@@ -1100,6 +1161,52 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
     }
     n->stream = keep;
     return rc;
+}
+
+size_t cnn_tile_view_bytes() { return sizeof(TileView); }
+
+// Every view of a batch in one tile launch: the views' gate tile lists are
+// concatenated through a table (host staging h_tab, device copy d_tab, both
+// n_views entries; first: n_views + 1 words) written here and copied on
+// `stream` ahead of the launch.  The per-layer kernels (fused mode off) run
+// view by view.
+int cnn_fill_tiles_batch(svo_cnn* n, svo_ctx* const* views, const svo_frame_params* params, int n_views,
+                         void* h_tab, void* d_tab, uint32_t* first, hipStream_t stream)
+{
+    TileView* hv = (TileView*)h_tab;
+    int max_all = 0;
+    for (int i = 0; i < n_views; i++) {
+        svo_ctx* c = views[i];
+        const int mt = c->n_tiles_x * c->n_tiles_y;
+        hv[i] = tile_view(c, &params[i], c->d_gate_tiles, c->d_gate, mt);
+        max_all += mt;
+    }
+    if (!n->fused) {
+        for (int i = 0; i < n_views; i++) {
+            svo_ctx* c = views[i];
+            RCC(cnn_fill_tiles(n, c, &params[i], c->d_gate_tiles, c->d_gate, c->n_tiles_x * c->n_tiles_y, stream));
+        }
+        HIPCHECK(hipMemcpyAsync(d_tab, h_tab, sizeof(TileView) * n_views, hipMemcpyHostToDevice, stream));
+        return SVO_OK;
+    }
+    HIPCHECK(hipMemcpyAsync(d_tab, h_tab, sizeof(TileView) * n_views, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_tile_first, dim3(1), dim3(1), 0, stream, (const TileView*)d_tab, n_views, first);
+    HIPCHECK(hipGetLastError());
+    TileArgs a = tile_args(n);
+    a.views = (const TileView*)d_tab;
+    a.first = first;
+    a.n_views = n_views;
+    return launch_tiles(a, n->tile_mode, max_all, n->n_cu, stream, 0);
+}
+
+// the staged colours of every view of a batch into their images (the table
+// cnn_fill_tiles_batch copied to the device)
+int cnn_apply_tiles_batch(const void* d_tab, int n_views, int max_tiles, hipStream_t stream)
+{
+    const unsigned g = (unsigned)std::max(1, std::min(2048, (max_tiles * 256 + 255) / 256));
+    hipLaunchKernelGGL(k_cnn_apply_views, dim3(g, (unsigned)n_views), dim3(256), 0, stream, (const TileView*)d_tab);
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
 }
 
 int cnn_apply_tiles(svo_ctx* c, const uint32_t* tile_list, const uint32_t* tile_count, int max_tiles,
@@ -1133,7 +1240,7 @@ extern "C" int svo_cnn_forward_tiles(svo_cnn* n, const uint16_t* x, int count, f
     a.x = n->xt;
     a.out = n->ot;
     a.n = count;
-    RCC(launch_tiles(a, n->tile_mode, count, n->device, n->stream, 0));
+    RCC(launch_tiles(a, n->tile_mode, count, n->n_cu, n->stream, 0));
     HIPCHECK(hipMemcpyAsync(out, n->ot, (size_t)count * 256 * 3 * 4, hipMemcpyDeviceToHost, n->stream));
     HIPCHECK(hipStreamSynchronize(n->stream));
     return SVO_OK;

@@ -73,6 +73,9 @@ struct svo_ctx {
     // learned frames: workgroups of the CNN tile kernel while it runs
     // alongside the ray launch (0 = two per CU); set to half the CUs at creation
     int cnn_overlap_wgs = 0;
+    // learned frames: tiles with a reprojected pixel deeper than this (world
+    // units) are raycast whole (0 = off)
+    float gate_depth = 0.0f;
     uint32_t* d_block_long = nullptr;  // two flag arrays (previous / current frame)
     uint32_t* d_long_list = nullptr;   // pixel words (x | y << 16) of the long blocks
     uint32_t* d_long_count = nullptr;
@@ -105,3 +108,10 @@ int cnn_fill_tiles(svo_cnn* n, svo_ctx* c, const svo_frame_params* p, const uint
                    const uint32_t* tile_count, int max_tiles, hipStream_t stream, int grid = 0);
 int cnn_apply_tiles(svo_ctx* c, const uint32_t* tile_list, const uint32_t* tile_count, int max_tiles,
                     hipStream_t stream);
+// a batch's CNN: one tile launch over every view through a table of
+// cnn_tile_view_bytes() per view (host staging h_tab, device copy d_tab;
+// first: n_views + 1 words), then one apply launch over the same table
+size_t cnn_tile_view_bytes();
+int cnn_fill_tiles_batch(svo_cnn* n, svo_ctx* const* views, const svo_frame_params* params, int n_views,
+                         void* h_tab, void* d_tab, uint32_t* first, hipStream_t stream);
+int cnn_apply_tiles_batch(const void* d_tab, int n_views, int max_tiles, hipStream_t stream);

@@ -147,6 +147,7 @@ struct View {
     uint32_t* gate_tiles;
     uint32_t* gate_rays;
     int gate_tiles_x, gate_n_tiles;
+    float gate_depth;  // > 0: a tile with a reprojected pixel farther than this is raycast whole
 };
 
 constexpr uint32_t QUAD_NONE = 0xffffffffu;  // list entry without a ray
@@ -185,6 +186,7 @@ struct Ray {
     uint32_t node, before, lroot, test;
     uint32_t pix;
     uint32_t guard;
+    uint32_t desync;  // nonzero: a step left stale nodes on the stack (no fused trips from then on)
     int view;
     uint32_t blk;  // 8x8 screen block of the ray (long-ray flags of full frames), or NO_BLOCK
 };
@@ -287,6 +289,7 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
     r.lroot = 0;
     r.test = 0;
     r.guard = 0;
+    r.desync = 0u;
     return true;
 }
 
@@ -372,20 +375,44 @@ __device__ __forceinline__ uint32_t step_ray(Ray& r, const RayArgs& a, OctBuf ob
         const int DAND = (1 << D) - 1;
         const int x0ry = r.iy ^ ny;
         const uint32_t xm = (uint32_t)((r.ix ^ nx) ^ x0ry ^ (r.iz ^ nz)) & (uint32_t)DAND;
+        const uint32_t chg = (uint32_t)((r.ix ^ nx) | x0ry | (r.iz ^ nz)) & ((2u << D) - 1u);  // bits 0..D
         r.ix = nx;
         r.iy = ny;
         r.iz = nz;
         const bool far = r.dist > 400000.0f;
         const bool pop = !far && xm >= (1u << rek);
         const int nr = pop ? 32 - __clz((int)xm) : rek;  // rek_new + 1, or stay
-        r.rek = nr;
-        r.node = STK(nr);
-        r.before = STK(nr + 1);
         // LOD switch, evaluated on pops only (kernel.cl:209)
         const bool l = pop && r.dist > r.lodsw;
         r.lodsw = l ? r.lodsw * 2.0f : r.lodsw;
         r.lod += l ? 1 : 0;
-        done = (uint32_t)far | ((uint32_t)x0ry >> (D + 1) & 1u);
+        const bool ycross = ((uint32_t)x0ry >> (D + 1) & 1u) != 0u;
+        // The reference's next trip, fused into this one.  After a pop to
+        // level nr that left every position bit from nr to D unchanged (the
+        // XOR level can sit below the highest changed bit, and bit D is
+        // masked out of it, so this is checked on the OR of the changes),
+        // the cell at level nr holding the ray is the one it came from: the
+        // next trip tests that child, finds it set, and descends into it
+        // (kernel.cl:160-179), fetching the ref the stack still holds in row
+        // nr-1 -- or stops there as a hit when nr <= lod.  local_root already
+        // belongs to that path.  Counted as two trips; not fused when the
+        // second trip would pass the trip cap.
+        // A step that changed a position bit above the level it resumes at
+        // (the XOR hid it) leaves the nodes there, and the stack rows below,
+        // describing cells the ray has left; the reference then carries on
+        // with those stale nodes, and so does this kernel, but no later
+        // step of the ray is fused.
+        r.desync |= chg >> (nr + 1);
+        const bool fuse = pop && r.desync == 0u && !ycross && (chg >> nr) == 0u && r.guard + 2u <= SVO_MAX_RAY_ITERS;
+        const bool hit = fuse && nr <= r.lod;
+        const int base = fuse ? nr - 1 : nr;
+        r.node = STK(base);
+        r.before = STK(base + 1);
+        r.rek = (fuse && !hit) ? nr - 1 : nr;
+        r.guard += fuse ? 1u : 0u;
+        // a hit's child_test is consumed by fetchColor at level 1 only, and
+        // nr >= 2 here
+        done = (uint32_t)far | (uint32_t)ycross | (uint32_t)hit;
     }
     return done;
 }
@@ -883,14 +910,20 @@ __global__ void k_gate_reset(View v, const View* views)
 // holes: hole pixels that neither the hole list (all-hole 2x2 quads of the
 // full 16x16 blocks) nor the refresh rectangle will trace.  0 < residual <=
 // max_cnn -> the tile goes to the CNN; more -> its residual pixels join the
-// frame's ray launch as a second list.  A workgroup handles GATE_TILES tiles
-// (each wave two, lane = 4 pixels) and appends its CNN tiles and gate pixels
-// with one atomic per list, so the counters see few, uncontended atomics.
+// frame's ray launch as a second list.  With a depth limit (gate_depth > 0)
+// a tile also counts its reprojected pixels (outside the refresh rectangle)
+// whose camera depth (coordinate w, written by the warp) exceeds the limit:
+// where a pixel spans several voxels, the warped sample is one of them and
+// its colour rarely the one the pixel's ray meets, so such a tile is raycast
+// whole -- its residual holes and its reprojected pixels join the list.  A
+// workgroup handles GATE_TILES tiles (each wave two, lane = 4 pixels) and
+// appends its CNN tiles and gate pixels with one atomic per list, so the
+// counters see few, uncontended atomics.
 constexpr int GATE_TILES = 8;
 __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max_cnn)
 {
     VIEW_OF(blockIdx.y);
-    __shared__ uint32_t s_n[GATE_TILES], s_base[2];
+    __shared__ uint32_t s_n[GATE_TILES], s_proj[GATE_TILES], s_base[2];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int res_x = V.res_x, res_y = V.res_y;
     const int tiles_x = V.gate_tiles_x, n_tiles = V.gate_n_tiles;
@@ -899,30 +932,46 @@ __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max
     uint32_t* rays = V.gate_rays;
     const uint32_t max_tiles = (uint32_t)n_tiles;
     const int fx = (res_x / 16) * 16, fy = (res_y / 16) * 16;
-    unsigned long long m[2][4];
+    const bool depth_gate = V.gate_depth > 0.0f;
+    // per pixel: residual hole; reprojected pixel outside the rectangle
+    unsigned long long m[2][4], mp[2][4];
     for (int h = 0; h < 2; h++) {
         const int lt = wave * 2 + h, t = blockIdx.x * GATE_TILES + lt;
-        uint32_t n = 0;
+        uint32_t n = 0, np = 0, nf = 0;
         for (int j = 0; j < 4; j++) {
             const int i = lane + 64 * j;
             const int x = (t % tiles_x) * 16 + (i & 15), y = (t / tiles_x) * 16 + (i >> 4);
-            bool hole = t < n_tiles && x < res_x && y < res_y && V.screen[(size_t)y * res_x + x] == HOLE;
+            const bool in = t < n_tiles && x < res_x && y < res_y;
+            const size_t o = (size_t)y * res_x + x;
+            const uint32_t w = in ? V.screen[o] : HOLE;
+            const bool in_rect = x >= V.add_x && x < V.add_x + V.ext_x && y >= V.add_y && y < V.add_y + V.ext_y;
+            bool hole = in && w == HOLE && !in_rect;
             if (hole && x < fx && y < fy && quad_is_hole(V.screen, res_x, x & ~1, y & ~1))
                 hole = false;  // in the hole list
-            if (hole && x >= V.add_x && x < V.add_x + V.ext_x && y >= V.add_y && y < V.add_y + V.ext_y)
-                hole = false;  // in the refresh rectangle
+            const bool proj = depth_gate && in && w != HOLE && !in_rect;
+            const bool far = proj && V.coords[o].w > V.gate_depth;
             m[h][j] = __ballot(hole);
+            mp[h][j] = __ballot(proj);
             n += (uint32_t)__popcll(m[h][j]);
+            np += (uint32_t)__popcll(mp[h][j]);
+            nf += (uint32_t)__popcll(__ballot(far));
         }
-        if (lane == 0) s_n[lt] = n;
+        if (lane == 0) {
+            s_n[lt] = n;
+            s_proj[lt] = nf ? np : 0u;  // a tile with a far pixel sends all its reprojected pixels too
+        }
     }
     __syncthreads();
+    // pixels a tile sends back to raycasting; 0 with residual holes = a CNN tile
+    auto tile_rays = [&](int k) -> uint32_t {
+        return s_proj[k] ? s_n[k] + s_proj[k] : ((int)s_n[k] > max_cnn ? s_n[k] : 0u);
+    };
     if (threadIdx.x == 0) {
         uint32_t nc = 0, nr = 0;
         for (int k = 0; k < GATE_TILES; k++) {
-            if (s_n[k] == 0) continue;
-            if ((int)s_n[k] <= max_cnn) nc++;
-            else nr += s_n[k];
+            const uint32_t tr = tile_rays(k);
+            nr += tr;
+            nc += (!tr && s_n[k]) ? 1u : 0u;
         }
         s_base[0] = nc ? atomicAdd(&gate[0], nc) : 0u;
         s_base[1] = nr ? atomicAdd(&gate[GATE_RAYS], nr) : 0u;
@@ -930,21 +979,22 @@ __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max
     __syncthreads();
     for (int h = 0; h < 2; h++) {
         const int lt = wave * 2 + h, t = blockIdx.x * GATE_TILES + lt;
-        const uint32_t n = s_n[lt];
-        if (n == 0) continue;
-        // this tile's slot: tiles of lower index in the block come first
+        const uint32_t tr = tile_rays(lt);
+        if (!tr && !s_n[lt]) continue;
+        // this tile's slots: tiles of lower index in the block come first
         uint32_t oc = s_base[0], orr = s_base[1];
         for (int k = 0; k < lt; k++) {
-            if (s_n[k] == 0) continue;
-            if ((int)s_n[k] <= max_cnn) oc++;
-            else orr += s_n[k];
+            const uint32_t tk = tile_rays(k);
+            orr += tk;
+            oc += (!tk && s_n[k]) ? 1u : 0u;
         }
-        if ((int)n <= max_cnn) {
+        if (!tr) {  // a CNN tile
             if (lane == 0 && oc < max_tiles) tiles[oc] = (uint32_t)t;
             continue;
         }
+        const bool whole = s_proj[lt] != 0u;
         for (int j = 0; j < 4; j++) {
-            const unsigned long long mm = m[h][j];
+            const unsigned long long mm = m[h][j] | (whole ? mp[h][j] : 0ull);
             if ((mm >> lane) & 1ull) {
                 const int i = lane + 64 * j;
                 const int x = (t % tiles_x) * 16 + (i & 15), y = (t / tiles_x) * 16 + (i >> 4);
@@ -1210,6 +1260,8 @@ static void take_image(svo_ctx* c)
     c->d_image = c->image_buf[s];
 }
 
+static void view_camera(View& v, const svo_frame_params* p);
+
 static View make_view(svo_ctx* c, const svo_frame_params* p)
 {
     View v;
@@ -1230,6 +1282,13 @@ static View make_view(svo_ctx* c, const svo_frame_params* p)
     v.gate_rays = c->d_gate_rays;
     v.gate_tiles_x = c->n_tiles_x;
     v.gate_n_tiles = c->n_tiles_x * c->n_tiles_y;
+    v.gate_depth = c->gate_depth;
+    view_camera(v, p);
+    return v;
+}
+
+static void view_camera(View& v, const svo_frame_params* p)
+{
     if (p) {
         v.m0x = p->m0[0]; v.m0y = p->m0[1]; v.m0z = p->m0[2];
         v.r00 = p->row0[0]; v.r01 = p->row0[1]; v.r02 = p->row0[2];
@@ -1241,7 +1300,6 @@ static View make_view(svo_ctx* c, const svo_frame_params* p)
         v.fovx = p->fovx;
         v.fovy = p->fovy;
     }
-    return v;
 }
 
 static void set_rays(View& v, const uint32_t* list, const uint32_t* list_count, int add_x, int add_y, int ext_x,
@@ -1631,6 +1689,14 @@ extern "C" int svo_set_long_first(svo_ctx* c, int long_trips)
     return SVO_OK;
 }
 
+extern "C" int svo_set_gate_depth(svo_ctx* c, float max_depth)
+{
+    if (!c) return svo_fail(SVO_ERR_ARG, "null context");
+    if (!(max_depth >= 0.0f) || max_depth > 1e30f) return svo_fail(SVO_ERR_ARG, "gate depth must be >= 0");
+    c->gate_depth = max_depth;
+    return SVO_OK;
+}
+
 extern "C" int svo_set_cnn_overlap(svo_ctx* c, int workgroups)
 {
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
@@ -1778,6 +1844,10 @@ struct svo_batch {
     std::vector<hipEvent_t> pre;  // per view: its stream's queued work
     uint32_t* d_first = nullptr;
     uint32_t* d_queue = nullptr;
+    // learned frames: the CNN tile table of all views (2 slots, like the views)
+    char* d_cnn_tab = nullptr;
+    char* h_cnn_tab = nullptr;
+    uint32_t* d_cnn_first = nullptr;
     int seq = 0;
     int frame_no = 0;
     uint32_t maxN = 0, maxS = 0;
@@ -1793,6 +1863,9 @@ extern "C" int svo_batch_destroy(svo_batch* b)
     (void)hipHostFree(b->h_views);
     (void)hipFree(b->d_first);
     (void)hipFree(b->d_queue);
+    (void)hipFree(b->d_cnn_tab);
+    (void)hipHostFree(b->h_cnn_tab);
+    (void)hipFree(b->d_cnn_first);
     for (auto& e : b->slot_done)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : b->pre)
@@ -1839,6 +1912,9 @@ extern "C" int svo_batch_create(svo_batch** out, svo_ctx* const* views, int n_vi
               hipMalloc(&b->d_views, sizeof(View) * 2 * n_views) == hipSuccess &&
               hipHostMalloc(&b->h_views, sizeof(View) * 2 * n_views) == hipSuccess &&
               hipMalloc(&b->d_first, 4 * (n_views + 1)) == hipSuccess && hipMalloc(&b->d_queue, 64) == hipSuccess &&
+              hipMalloc(&b->d_cnn_tab, cnn_tile_view_bytes() * 2 * n_views) == hipSuccess &&
+              hipHostMalloc(&b->h_cnn_tab, cnn_tile_view_bytes() * 2 * n_views) == hipSuccess &&
+              hipMalloc(&b->d_cnn_first, 4 * (n_views + 1)) == hipSuccess &&
               hipMemset(b->d_queue, 0, 64) == hipSuccess &&
               hipEventCreateWithFlags(&b->slot_done[0], hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&b->slot_done[1], hipEventDisableTiming) == hipSuccess &&
@@ -1935,14 +2011,12 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
                            (const View*)dv);
         HIPCHECK(hipGetLastError());
         for (auto* c : b->views) c->cnn_input = true;
-        if (cnn) {  // every view's CNN on the batch's second stream, alongside the ray launch
+        if (cnn) {  // all views' CNN tiles in one launch on the batch's second stream, alongside the rays
             HIPCHECK(hipEventRecord(b->cnn_fork, b->stream));
             HIPCHECK(hipStreamWaitEvent(b->cnn_stream, b->cnn_fork, 0));
-            for (int i = 0; i < n; i++) {
-                svo_ctx* c = b->views[i];
-                RC(cnn_fill_tiles(cnn, c, &params[i], c->d_gate_tiles, c->d_gate, c->n_tiles_x * c->n_tiles_y,
-                                  b->cnn_stream));
-            }
+            const size_t tb = cnn_tile_view_bytes() * n;
+            RC(cnn_fill_tiles_batch(cnn, b->views.data(), params, n, b->h_cnn_tab + slot * tb,
+                                    b->d_cnn_tab + slot * tb, b->d_cnn_first, b->cnn_stream));
             HIPCHECK(hipEventRecord(b->cnn_join, b->cnn_stream));
         }
     }
@@ -1954,10 +2028,9 @@ static int batch_frame(svo_batch* b, const svo_frame_params* params, svo_cnn* cn
     RC(launch_fill(L, dummy, nullptr, 1, 1));
     if (learned && cnn) {
         HIPCHECK(hipStreamWaitEvent(b->stream, b->cnn_join, 0));
-        for (int i = 0; i < n; i++) {
-            svo_ctx* c = b->views[i];
-            RC(cnn_apply_tiles(c, c->d_gate_tiles, c->d_gate, c->n_tiles_x * c->n_tiles_y, b->stream));
-        }
+        int max_tiles = 0;
+        for (auto* c : b->views) max_tiles = std::max(max_tiles, c->n_tiles_x * c->n_tiles_y);
+        RC(cnn_apply_tiles_batch(b->d_cnn_tab + slot * cnn_tile_view_bytes() * n, n, max_tiles, b->stream));
     }
     HIPCHECK(hipEventRecord(b->slot_done[slot], b->stream));
     for (auto* c : b->views) HIPCHECK(hipStreamWaitEvent(c->stream, b->slot_done[slot], 0));

@@ -690,7 +690,7 @@ int emit_voxels(VoxOut& vo, uint16_t** xyz, uint32_t** rgba, size_t* n)
 }
 
 // heightfield shell: columns in z -> x -> y order (like Rle4.cpp:95-123)
-void heightfield(VoxOut& vo, const std::vector<int>& H, int size, uint64_t seed, int hmin, int hmax)
+void heightfield(VoxOut& vo, const std::vector<int>& H, int size, uint64_t seed, int hmin, int hmax, int jamp)
 {
     auto h = [&](int x, int z) { return H[(size_t)((z + size) % size) * size + (x + size) % size]; };
     for (int z = 0; z < size; z++)
@@ -699,9 +699,9 @@ void heightfield(VoxOut& vo, const std::vector<int>& H, int size, uint64_t seed,
             int lo = std::min(top, std::min(std::min(h(x - 1, z), h(x + 1, z)), std::min(h(x, z - 1), h(x, z + 1)))) - 1;
             if (lo < 0) lo = 0;
             for (int y = lo; y <= top; y++) {
-                int luma = 40 + (y - hmin) * 215 / std::max(1, hmax - hmin) + jitter(seed, 1, x, y, z, 8);
-                int g = luma * 3 / 4 + jitter(seed, 2, x, y, z, 8);
-                int b = luma / 2 + jitter(seed, 3, x, y, z, 8);
+                int luma = 40 + (y - hmin) * 215 / std::max(1, hmax - hmin) + jitter(seed, 1, x, y, z, jamp);
+                int g = luma * 3 / 4 + jitter(seed, 2, x, y, z, jamp);
+                int b = luma / 2 + jitter(seed, 3, x, y, z, jamp);
                 vo.add(x, y, z, pack_rgba(luma, 1, g, b));
             }
         }
@@ -715,7 +715,10 @@ extern "C" int svo_scene_generate(int kind, uint64_t seed, int depth, int n_sphe
     if (depth < 8 || depth > 13) return svo_fail(SVO_ERR_ARG, "depth %d outside 8..13", depth);
     const int size = 1 << depth;
     VoxOut vo;
-    if (kind == 1 || kind == 2) {
+    if (kind == 1 || kind == 2 || kind == 5) {
+        // kind 5: kind 2's terrain and spheres with no per-voxel colour jitter
+        const int jamp = kind == 5 ? 0 : 8;
+        if (kind == 5) kind = 2;
         int octaves = (kind == 1) ? 2 : 4;
         int L0 = (kind == 1) ? size / 4 : size / 8;
         int hmin = (kind == 1) ? size / 8 : size / 16;
@@ -736,7 +739,7 @@ extern "C" int svo_scene_generate(int kind, uint64_t seed, int depth, int n_sphe
                 H[(size_t)z * size + x] = hmin + (int)floor(v * hspan);
             }
         });
-        heightfield(vo, H, size, seed, hmin, hmin + hspan);
+        heightfield(vo, H, size, seed, hmin, hmin + hspan, jamp);
         if (kind == 2) {
             Pcg32 rng(seed, 7);
             for (int s = 0; s < n_spheres; s++) {
@@ -766,10 +769,11 @@ extern "C" int svo_scene_generate(int kind, uint64_t seed, int depth, int n_sphe
                         }
             }
         }
-    } else if (kind == 3) {
-        // cave: solid where 5-octave 3-D noise > threshold inside a slab of
-        // height size/8 at the bottom of the world; boundary voxels only.
-        int slab = size / 8;
+    } else if (kind == 3 || kind == 4) {
+        // cave: solid where 5-octave 3-D noise > threshold inside a slab at
+        // the bottom of the world (height size/8 for kind 3, the whole height
+        // for kind 4); boundary voxels only.
+        int slab = kind == 4 ? size : size / 8;
         auto solid = [&](int x, int y, int z) {
             if (y < 0) return true;
             if (y >= slab) return false;

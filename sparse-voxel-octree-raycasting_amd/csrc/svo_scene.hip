@@ -1,8 +1,10 @@
-// svo_scene.hip -- the 3-D fractal-noise cave scene (scene kind 3) on the GPU.
+// svo_scene.hip -- the 3-D fractal-noise cave scenes (scene kinds 3 and 4) on
+// the GPU.
 //
-// Emits exactly the voxel stream of the host generator (svo_host.cpp, kind 3):
-// solid where 5 octaves of periodic 3-D value noise exceed 0.12 inside a slab
-// of height size/8 at the bottom of the world, boundary voxels only (a solid
+// Emits exactly the voxel stream of the host generator (svo_host.cpp, kinds 3
+// and 4): solid where 5 octaves of periodic 3-D value noise exceed 0.12 inside
+// a slab at the bottom of the world (height size/8 for kind 3, the whole
+// height for kind 4), boundary voxels only (a solid
 // voxel with a non-solid 6-neighbour; x and z wrap, below the slab counts as
 // solid, above as empty), stream order z -> x -> y, colour by height with a
 // seeded jitter.  The host evaluates the noise per voxel; here the lattice
@@ -206,13 +208,13 @@ extern "C" int svo_scene_generate_gpu(int device, int kind, uint64_t seed, int d
                                       uint32_t** rgba_out, size_t* n_out)
 {
     if (!xyz_out || !rgba_out || !n_out) return svo_fail(SVO_ERR_ARG, "null output pointer");
-    if (kind != 3) return svo_fail(SVO_ERR_ARG, "scene kind %d is generated on the host only", kind);
+    if (kind != 3 && kind != 4) return svo_fail(SVO_ERR_ARG, "scene kind %d is generated on the host only", kind);
     if (depth < 8 || depth > 13) return svo_fail(SVO_ERR_ARG, "depth %d outside 8..13", depth);
     int ndev = 0;
     GCHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return svo_fail(SVO_ERR_ARG, "device %d not present", device);
     GCHECK(hipSetDevice(device));
-    const int size = 1 << depth, slab = size / 8;
+    const int size = 1 << depth, slab = kind == 4 ? size : size / 8;
     CaveArgs a;
     memset(&a, 0, sizeof(a));
     a.seed = seed;

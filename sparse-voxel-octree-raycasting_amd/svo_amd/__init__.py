@@ -71,6 +71,7 @@ _f = {
     "frame_full": _sig("svo_frame_full", C.c_int, _ctxp, C.POINTER(FrameParams)),
     "memset": _sig("svo_memset", C.c_int, _ctxp, C.c_size_t, C.c_uint32, C.c_size_t),
     "proj": _sig("svo_proj", C.c_int, _ctxp, C.c_int, C.POINTER(FrameParams)),
+    "set_gate_depth": _sig("svo_set_gate_depth", C.c_int, _ctxp, C.c_float),
     "counthole": _sig("svo_counthole", C.c_int, _ctxp),
     "sumids": _sig("svo_sumids", C.c_int, _ctxp),
     "writeids": _sig("svo_writeids", C.c_int, _ctxp),
@@ -164,6 +165,7 @@ EXPORTED = [
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
     "svo_set_ray_lanes", "svo_set_long_first", "svo_set_ray_block", "svo_raycast_quads", "svo_fillhole_punch",
     "svo_image_save_png", "svo_set_cnn_overlap", "svo_read_image_async", "svo_host_alloc", "svo_host_free",
+    "svo_set_gate_depth",
 ]
 
 
@@ -250,7 +252,7 @@ def scene(kind, seed, depth, n_spheres=0):
 
 
 def scene_gpu(kind, seed, depth, device=0):
-    """scene() for the cave (kind 3) generated on the GPU: the same stream."""
+    """scene() for the caves (kinds 3 and 4) generated on the GPU: the same stream."""
     xyz = _u16p()
     rgba = _u32p()
     n = C.c_size_t()
@@ -482,6 +484,11 @@ class Context:
         had a ray of at least long_trips trips (0 = raster order only).
         Results are the same either way."""
         _check(_f["set_long_first"](self._h, int(long_trips)))
+
+    def set_gate_depth(self, max_depth):
+        """Learned frames: raycast whole every tile holding a reprojected
+        pixel deeper than max_depth world units (0 = off)."""
+        _check(_f["set_gate_depth"](self._h, float(max_depth)))
 
     def set_cnn_overlap(self, workgroups):
         """Workgroups of the CNN tile kernel while it runs alongside the

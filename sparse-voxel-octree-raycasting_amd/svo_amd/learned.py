@@ -67,10 +67,12 @@ def _path(name, seed_off):
                        w["pitch"])
 
 
-def evaluate(cnn, name, seed_off=1000, n_frames=40, skip=8, max_cnn_holes=64):
+def evaluate(cnn, name, seed_off=1000, n_frames=40, skip=8, max_cnn_holes=64, gate_depth=0.0):
     """PSNR (dB, vs the full raycast) of the plain warp (fillhole2), the gate
     alone, and the gate + CNN, over n_frames of a held-out path; mean over the
-    frames after the first `skip`, on all pixels and on the hole pixels."""
+    frames after the first `skip`, on all pixels and on the hole pixels.
+    gate_depth: the gate's depth rule (svo_set_gate_depth) for both gated
+    modes."""
     words, root, depth, _ = W.octree(name)
     rx, ry = W.WORKLOADS[name]["res"]
     rows = _path(name, seed_off)
@@ -78,6 +80,8 @@ def evaluate(cnn, name, seed_off=1000, n_frames=40, skip=8, max_cnn_holes=64):
     for c in ctx.values():
         c.upload_octree(words, root, depth)
         c.resize(rx, ry)
+    ctx["gate"].set_gate_depth(gate_depth)
+    ctx["learned"].set_gate_depth(gate_depth)
     res = {k: [] for k in ("plain", "gate", "learned")}
     hole_res = {k: [] for k in ("plain", "gate", "learned")}
     for f in range(n_frames):

@@ -6,9 +6,12 @@ arguments.  Scene seed = 1000 + n, path seed = 2000 + n for workload n.
   flythrough_2048: depth 11 (2048^3) 4-octave fractal terrain shell + 2000
                    sphere shells (radius 4..64 voxels), 1920x1024, 300 frames,
                    1..8 voxels and <= 1 deg yaw per frame
-  cave_4096:       depth 12 (4096^3) 5-octave 3-D noise cave (boundary voxels of
-                   a slab 512 voxels high), 1920x1024, 200 frames, up to 32
+  cave_4096:       depth 12 (4096^3) 5-octave 3-D noise cave over the whole
+                   volume (boundary voxels), 1920x1024, 200 frames, up to 32
                    voxels and 5 deg yaw per frame (divergence stress)
+  flythrough_2048_flat: flythrough_2048's scene and path without the per-voxel
+                   colour jitter (voxel colours by height only); the jitter
+                   sets the reprojection error of warped pixels
   views_24:        24 independent cameras over the flythrough_2048 scene,
                    per-view resolution drawn from {320x240, 640x480,
                    1280x720, 1920x1024}, per-view motion up to 0.5..16
@@ -27,8 +30,10 @@ WORKLOADS = {
                            vox=2.0, yaw=0.5, height=240.0, pitch=0.35, path_seed=2001),
     "flythrough_2048": dict(kind=2, seed=1002, depth=11, spheres=2000, res=(1920, 1024), frames=300,
                             vox=8.0, yaw=1.0, height=1400.0, pitch=0.30, path_seed=2002),
-    "cave_4096": dict(kind=3, seed=1003, depth=12, spheres=0, res=(1920, 1024), frames=200,
-                      vox=32.0, yaw=5.0, height=256.0, pitch=0.05, path_seed=2003),
+    "flythrough_2048_flat": dict(kind=5, seed=1002, depth=11, spheres=2000, res=(1920, 1024), frames=300,
+                                 vox=8.0, yaw=1.0, height=1400.0, pitch=0.30, path_seed=2002),
+    "cave_4096": dict(kind=4, seed=1003, depth=12, spheres=0, res=(1920, 1024), frames=200,
+                      vox=32.0, yaw=5.0, height=2048.0, pitch=0.05, path_seed=2003),
 }
 
 
@@ -37,7 +42,7 @@ def voxels(name):
     """(xyz, rgba) voxel stream of a workload's scene (the cave on the GPU:
     the host generator needs hours at depth 12; both give the same stream)."""
     w = WORKLOADS[name]
-    if w["kind"] == 3:
+    if w["kind"] in (3, 4):
         return scene_gpu(3, w["seed"], w["depth"])
     return scene(w["kind"], w["seed"], w["depth"], w["spheres"])
 
@@ -48,14 +53,16 @@ def octree(name):
     octree is built on the GPU; the host builder gives the same words)."""
     w = WORKLOADS[name]
     xyz, rgba = voxels(name)
-    build = build_octree_gpu if w["kind"] == 3 else build_octree
+    build = build_octree_gpu if w["kind"] in (3, 4) else build_octree
     words, root = build(xyz, rgba, w["depth"])
     return words, root, w["depth"], len(rgba)
 
 
-def path(name, n_frames=None):
+def path(name, n_frames=None, seed_off=0):
+    """The workload's seeded camera path; seed_off != 0 gives a held-out path
+    of the same motion bounds (used to tune without seeing the timed frames)."""
     w = WORKLOADS[name]
-    return camera_path(w["path_seed"], w["depth"], n_frames or w["frames"], w["vox"], w["yaw"],
+    return camera_path(w["path_seed"] + seed_off, w["depth"], n_frames or w["frames"], w["vox"], w["yaw"],
                        w["height"], w["pitch"])
 
 
@@ -82,7 +89,7 @@ def batch_views(n_views=24, seed=5005, frames=120, scene="flythrough_2048"):
     return out
 
 
-def view_path(view):
+def view_path(view, seed_off=0):
     w = WORKLOADS[view["scene"]]
-    return camera_path(view["path_seed"], w["depth"], view["frames"], view["vox"], view["yaw"],
+    return camera_path(view["path_seed"] + seed_off, w["depth"], view["frames"], view["vox"], view["yaw"],
                        w["height"], w["pitch"])

@@ -112,15 +112,19 @@ def test_batch_rejects_mismatched_octrees():
         b.close()
 
 
-def test_batch_learned_matches_single_views():
+@pytest.mark.parametrize("max_depth", [0.0, 60.0])
+def test_batch_learned_matches_single_views(max_depth):
     """Batched learned frames (gate of all views in one launch, gate pixels in
-    the shared ray launch, CNN per view) equal per-view learned frames."""
+    the shared ray launch, the CNN tiles of all views in one launch) equal
+    per-view learned frames, with and without the gate's depth rule."""
     from svo_amd import learned as L
     words, root = scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
     pr = path(CONFIG1, 32)
     res = RES[:4]
     batched = make_views(words, root, res)
     single = make_views(words, root, res)
+    for c in batched + single:
+        c.set_gate_depth(max_depth)
     cnn = S.Cnn(0)
     L.load(cnn, L.WEIGHTS)
     batch = S.Batch(batched)

@@ -15,7 +15,11 @@ pytestmark = pytest.mark.gpu
 NAME = "flythrough_256"
 
 
-def test_gate_all_raycast_bitexact():
+@pytest.mark.parametrize("max_holes,max_depth", [(0, 0.0), (0, 60.0), (64, 60.0)])
+def test_gate_all_raycast_bitexact(max_holes, max_depth):
+    """The gate's ray lists against the oracle's gated frame: every residual
+    hole raycast (max_holes 0), with and without the depth rule; and the
+    depth rule with CNN tiles left unfilled (no network: fillhole2 colours)."""
     words, root, depth, _ = W.octree(NAME)
     rx, ry = W.WORKLOADS[NAME]["res"]
     rows = W.path(NAME, 10)
@@ -23,15 +27,17 @@ def test_gate_all_raycast_bitexact():
     with S.Context(0) as c:
         c.upload_octree(words, root, depth)
         c.resize(rx, ry)
+        c.set_gate_depth(max_depth)
         for f in range(10):
             p = W.frame_params(NAME, rows, f)
-            c.frame_warp_learned(None, p, 0)
-            o.frame_warp_gated(O.params_from(p), 0)
+            c.frame_warp_learned(None, p, max_holes)
+            o.frame_warp_gated(O.params_from(p), max_holes, max_depth)
             c.sync()
             assert_equal_u32(c.screen(), o.screen, f"frame {f} screen")
             assert_equal_f32_bits(c.coords(), o.coords, f"frame {f} coords")
             assert_equal_u32(c.image(), o.image, f"frame {f} image")
-            assert not (c.screen(2) == 0xFFFFFF00).any()
+            if max_holes == 0:
+                assert not (c.screen(2) == 0xFFFFFF00).any()
     o.close()
 
 

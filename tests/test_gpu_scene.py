@@ -1,5 +1,6 @@
-"""The cave scene (kind 3) generated on the GPU is the host generator's voxel
-stream, bit for bit (stream order, coordinates and colour words)."""
+"""The cave scenes (kind 3: a slab of 1/8 of the height, kind 4: the whole
+height) generated on the GPU are the host generator's voxel streams, bit for
+bit (stream order, coordinates and colour words)."""
 import time
 
 import numpy as np
@@ -10,14 +11,15 @@ import svo_amd as S
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("seed", [1003, 77])
-def test_cave_gpu_matches_host_depth8(seed):
+@pytest.mark.parametrize("kind,seed", [(3, 1003), (3, 77), (4, 1003)])
+def test_cave_gpu_matches_host_depth8(kind, seed):
     t = time.time()
-    xyz_h, rgba_h = S.scene(3, seed, 8)
+    xyz_h, rgba_h = S.scene(kind, seed, 8)
     th = time.time() - t
     t = time.time()
-    xyz_g, rgba_g = S.scene_gpu(3, seed, 8)
-    print(f"cave depth 8 seed {seed}: {len(xyz_h)} voxels, host {th:.1f}s, gpu {time.time() - t:.2f}s", flush=True)
+    xyz_g, rgba_g = S.scene_gpu(kind, seed, 8)
+    print(f"cave kind {kind} depth 8 seed {seed}: {len(xyz_h)} voxels, host {th:.1f}s, gpu {time.time() - t:.2f}s",
+          flush=True)
     assert len(xyz_g) == len(xyz_h)
     assert np.array_equal(xyz_g, xyz_h)
     assert np.array_equal(rgba_g, rgba_h)

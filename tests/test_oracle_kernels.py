@@ -366,3 +366,34 @@ def test_punch_without_motion_is_a_no_op():
     o.fillhole_punch(None, None)
     assert np.array_equal(o.screen[0], before)
     o.close()
+
+
+def test_gated_frame_depth_rule():
+    """The gate's depth rule on the oracle (no reference counterpart): a limit
+    no reprojected pixel reaches changes nothing; a limit every reprojected
+    pixel exceeds sends every reprojected pixel to raycasting (inside the
+    refresh rectangle it is traced anyway), so after the frame's rays plane 2
+    (the unfiltered frame) holds only raycast words, each equal to the full
+    raycast's, and holes."""
+    from helpers import CONFIG1, params, path, scene_octree
+    words, root = scene_octree(CONFIG1["kind"], CONFIG1["seed"], CONFIG1["depth"])
+    rx, ry = 256, 128
+    pr = path(CONFIG1, 6)
+    frames = {k: O.Frame(rx, ry, words, root, CONFIG1["depth"], nthreads=8) for k in ("plain", "far", "near", "full")}
+    try:
+        for f in range(6):
+            p = O.params_from(params(CONFIG1, pr, f))
+            frames["plain"].frame_warp_gated(p, 64)
+            frames["far"].frame_warp_gated(p, 64, 1e9)
+            frames["near"].frame_warp_gated(p, 64, 1e-3)
+            frames["full"].frame_full(p)
+            assert np.array_equal(frames["far"].screen, frames["plain"].screen)
+            near2, full0 = frames["near"].screen[2], frames["full"].screen[0]
+            traced = near2 != O.HOLE
+            assert np.array_equal(near2[traced], full0[traced]), f"frame {f}"
+            if f >= 2:  # the plain gated frame keeps warped words that the depth rule replaces
+                plain2 = frames["plain"].screen[2]
+                assert (plain2 != full0).sum() > 100 and traced.mean() > 0.9
+    finally:
+        for fr in frames.values():
+            fr.close()

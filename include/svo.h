@@ -97,6 +97,19 @@ int svo_memset(svo_ctx* ctx, size_t byte_ofs, uint32_t val, size_t bytes);
  * plane in 1..3.  Deterministic: the first source (row-major) with the
  * smallest depth key wins a target pixel. */
 int svo_proj(svo_ctx* ctx, int plane, const svo_frame_params* p);
+/* svo_proj without any allocation, on the caller's device buffers: screen
+ * (4 planes x N uint32) and coords (4 planes x N float4, 16-byte aligned) of
+ * one res_x x res_y frame on `device`, launched on `stream` (a hipStream_t,
+ * NULL = the null stream) and not waited for.  Same results as svo_proj on a
+ * context holding those planes.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc); it calls no hipMalloc, hipFree or
+ * allocating library routine.  Its warp keys live in a static 16 MiB device
+ * array of the library (one per GPU, held until the process ends), so
+ * N = res_x * res_y may not exceed 2^21 pixels (SVO_ERR_ARG beyond); calls
+ * that overlap on one GPU share that array and race. */
+int svo_proj_noalloc(int device, uint32_t* screen, float* coords, int res_x, int res_y, int plane,
+                     const svo_frame_params* p, void* stream);
 /* raycast_counthole (kernel.cl:234-274) */
 int svo_counthole(svo_ctx* ctx);
 /* raycast_sumids (kernel.cl:276-296); the total stays on the device in

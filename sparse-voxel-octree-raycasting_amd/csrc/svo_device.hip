@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "svo_ctx.h"
@@ -547,6 +548,14 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
 
 // ---------------------------------------------------------------- warp
 __global__ __launch_bounds__(256) void k_memset(uint32_t* __restrict__ dst, uint32_t val, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n; i += stride) dst[i] = val;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* __restrict__ dst, unsigned long long val,
+                                                  size_t n)
 {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     size_t stride = (size_t)gridDim.x * 256;
@@ -2271,4 +2280,72 @@ extern "C" int svo_set_tuning(svo_ctx* c, int refill_threshold, int wg_per_cu, i
     if (iters_per_refill > 0) c->iters = iters_per_refill;
     if (lds_cache_words >= 0) c->cache_req = (uint32_t)lds_cache_words;
     return SVO_OK;
+}
+
+// ---------------------------------------------------------------- warp without allocation
+// svo_proj_noalloc keeps its 64-bit warp keys (depth | source | colour, one
+// per target pixel) in this file-scope device array instead of a context's
+// buffer: every GPU the library runs on holds its own copy (16 MiB) until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address; the address comes from hipGetSymbolAddress once per device
+// (with that device current) and only that pointer reaches kernels.
+static __device__ __align__(256) unsigned long long scratch[SVO_NOALLOC_MAX_PIXELS];
+
+static std::mutex g_scratch_mu;
+static unsigned long long* g_scratch_dev[64];
+
+static int scratch_for_device(int device, unsigned long long** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return svo_fail(SVO_ERR_HIP, "key array of device %d: %s", device, hipGetErrorString(e));
+        g_scratch_dev[device] = (unsigned long long*)ptr;
+    }
+    *out = g_scratch_dev[device];
+    return SVO_OK;
+}
+
+extern "C" int svo_proj_noalloc(int device, uint32_t* screen, float* coords, int res_x, int res_y, int plane,
+                                const svo_frame_params* p, void* stream)
+{
+    if (!screen || !coords || !p) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (plane < 1 || plane > 3) return svo_fail(SVO_ERR_ARG, "plane %d outside 1..3", plane);
+    if (res_x < 16 || res_y < 16 || res_x > 8192 || res_y > 8192)
+        return svo_fail(SVO_ERR_ARG, "resolution %dx%d outside 16..8192", res_x, res_y);
+    const size_t N = (size_t)res_x * res_y;
+    if (N > SVO_NOALLOC_MAX_PIXELS)
+        return svo_fail(SVO_ERR_ARG, "%dx%d is %zu pixels, more than the %u svo_proj_noalloc takes", res_x, res_y, N,
+                        (unsigned)SVO_NOALLOC_MAX_PIXELS);
+    if (((uintptr_t)coords & 15) || ((uintptr_t)screen & 3))
+        return svo_fail(SVO_ERR_ARG, "screen must be 4-byte and coords 16-byte aligned");
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n || device >= 64) return svo_fail(SVO_ERR_ARG, "device %d not present", device);
+    HIPCHECK(hipSetDevice(device));
+    unsigned long long* keys = nullptr;
+    RC(scratch_for_device(device, &keys));
+    View v;
+    memset(&v, 0, sizeof(v));
+    v.screen = screen;
+    v.coords = (float4*)coords;
+    v.keys = keys;
+    v.res_x = res_x;
+    v.res_y = res_y;
+    v.N = (uint32_t)N;
+    v.S = (uint32_t)((res_x / 16) * (res_y / 16));
+    v.tiles_x = 1;
+    view_camera(v, p);
+    v.plane_lo = plane;
+    v.n_planes = 1;
+    const hipStream_t st = (hipStream_t)stream;
+    // the keys start at all ones; the resolve resets every key it reads, but
+    // each call sets them again and so depends on no earlier call
+    const unsigned blocks = (unsigned)std::min<size_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, st, keys, ~0ull, N);
+    HIPCHECK(hipGetLastError());
+    const Launch L{st, nullptr, 1, (uint32_t)N, v.S};
+    return launch_scatter_resolve(L, v, 1, 0);
 }

@@ -22,3 +22,7 @@ int svo_device_layout(const uint32_t* words, size_t n_words, uint32_t root, int 
 // Far above what any ray of a valid scene needs: every step advances the
 // ray to a cell face and rays stop at 400000 ray units.
 #define SVO_MAX_RAY_ITERS (1u << 22)
+
+// Pixels svo_proj_noalloc takes at most (its key array: 8 bytes per pixel,
+// 16 MiB per device): 1920x1024 and 2048x1024 fit.
+#define SVO_NOALLOC_MAX_PIXELS (1u << 21)

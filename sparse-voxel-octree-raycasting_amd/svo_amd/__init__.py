@@ -72,6 +72,8 @@ _f = {
     "memset": _sig("svo_memset", C.c_int, _ctxp, C.c_size_t, C.c_uint32, C.c_size_t),
     "proj": _sig("svo_proj", C.c_int, _ctxp, C.c_int, C.POINTER(FrameParams)),
     "set_gate_depth": _sig("svo_set_gate_depth", C.c_int, _ctxp, C.c_float),
+    "proj_noalloc": _sig("svo_proj_noalloc", C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                         C.POINTER(FrameParams), _vp),
     "counthole": _sig("svo_counthole", C.c_int, _ctxp),
     "sumids": _sig("svo_sumids", C.c_int, _ctxp),
     "writeids": _sig("svo_writeids", C.c_int, _ctxp),
@@ -165,7 +167,7 @@ EXPORTED = [
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
     "svo_set_ray_lanes", "svo_set_long_first", "svo_set_ray_block", "svo_raycast_quads", "svo_fillhole_punch",
     "svo_image_save_png", "svo_set_cnn_overlap", "svo_read_image_async", "svo_host_alloc", "svo_host_free",
-    "svo_set_gate_depth",
+    "svo_proj_noalloc", "svo_set_gate_depth",
 ]
 
 
@@ -249,6 +251,14 @@ def scene(kind, seed, depth, n_spheres=0):
     _f["free"](C.cast(xyz, _vp))
     _f["free"](C.cast(rgba, _vp))
     return a, b
+
+
+def proj_noalloc(device, screen_ptr, coords_ptr, res_x, res_y, plane, p, stream=0):
+    """svo_proj_noalloc on caller-owned device buffers (integer device
+    addresses: 4 planes of N uint32 screen words, 4 planes of N float4);
+    asynchronous on `stream` (0 = the null stream)."""
+    _check(_f["proj_noalloc"](int(device), _vp(int(screen_ptr)), _vp(int(coords_ptr)), int(res_x), int(res_y),
+                              int(plane), C.byref(p), _vp(int(stream)) if stream else None))
 
 
 def scene_gpu(kind, seed, depth, device=0):

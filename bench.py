@@ -16,6 +16,14 @@ import time
 
 import numpy as np
 
+# The warp frames' pretrace runs each of its result sets on a stream of its
+# own; HIP maps a process's streams onto at most GPU_MAX_HW_QUEUES hardware
+# queues (4 by default) and streams that share a queue run one after the
+# other.  Read at the first HIP call, so set before the library loads.
+HW_QUEUES = 16
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < HW_QUEUES:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sparse-voxel-octree-raycasting_amd"))
 
@@ -38,6 +46,11 @@ def log(*a):
 
 def pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else float("nan")
+
+
+def num(v, nd=4):
+    """round for the JSON line; None for NaN"""
+    return None if v != v else round(v, nd)
 
 
 def run_gpu(args, name):
@@ -87,18 +100,32 @@ def run_gpu(args, name):
             res[f"tuned_{mode}"] = dict(best, ms=round(ms, 4))
             log(f"[bench] autotune {mode}: {best} -> {ms:.4f} ms/frame")
 
-    # ---- pass A: throughput, frames back to back (no per-frame host sync)
+    # ---- pass A: throughput, frames back to back (no per-frame host sync);
+    # an event at each frame's end gives the frame-to-frame completion times
     for f in range(args.warmup):
         warp.frame_warp(params[f])
     warp.sync()
-    warp.ray_count(reset=True)
+    warp.ray_stats(reset=True)
+    warp.pretrace_stats(reset=True)
+    warp.enable_timing(mode=3)
     t0 = time.perf_counter()
     for f in range(args.warmup, total):
         warp.frame_warp(params[f])
     warp.sync()
     t_warp = time.perf_counter() - t0
-    rays_warp = warp.ray_count(reset=True)
-    log(f"[bench] warp: {args.steps} frames in {t_warp * 1e3:.1f} ms")
+    res["warp_intervals"] = warp.frame_intervals()
+    warp.enable_timing(False)
+    rays_traced, longest_frame = warp.ray_stats(reset=True)
+    pre_rays, pre_merged, pre_longest = warp.pretrace_stats(reset=True)
+    rays_warp = rays_traced + pre_merged  # the frames' hole and tile rays
+    cfg = warp.get_pretrace()
+    res["pretrace"] = {"sets": cfg.sets, "long_trips": cfg.long_trips, "wg_per_cu": cfg.wg_per_cu,
+                       "only_queued": cfg.only_queued, "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+                       "rays_traced_ahead_per_frame": round(pre_rays / args.steps),
+                       "frame_rays_taken_per_frame": round(pre_merged / args.steps),
+                       "frame_rays_traced_per_frame": round(rays_traced / args.steps),
+                       "longest_ray_traced_ahead": pre_longest, "longest_ray_traced_in_frame": longest_frame}
+    log(f"[bench] warp: {args.steps} frames in {t_warp * 1e3:.1f} ms; pretrace {res['pretrace']}")
 
     t_full, rays_full = float("nan"), 0
     if args.legs == "all":
@@ -106,11 +133,14 @@ def run_gpu(args, name):
             full.frame_full(params[f])
         full.sync()
         full.ray_count(reset=True)
+        full.enable_timing(mode=3)
         t0 = time.perf_counter()
         for f in range(args.warmup, total):
             full.frame_full(params[f])
         full.sync()
         t_full = time.perf_counter() - t0
+        res["full_intervals"] = full.frame_intervals()
+        full.enable_timing(False)
         rays_full = full.ray_count(reset=True)
         log(f"[bench] full: {args.steps} frames in {t_full * 1e3:.1f} ms")
 
@@ -164,6 +194,17 @@ def run_gpu(args, name):
             f"(last copy equals the device image: {ok})")
         if not ok:
             raise SystemExit("async image copy differs from the device image")
+
+    # the same frames back to back without the pretrace (graph launches)
+    wn = fresh()
+    wn.set_pretrace(sets=0)
+    t0 = time.perf_counter()
+    for f in range(args.warmup, total):
+        wn.frame_warp(params[f])
+    wn.sync()
+    res["warp_fps_no_pretrace"] = args.steps / (time.perf_counter() - t0)
+    wn.close()
+    log(f"[bench] warp without pretrace: {res['warp_fps_no_pretrace']:.1f} frames/s")
 
     warp2 = fresh()
     warp2.enable_timing(mode=2)
@@ -267,7 +308,7 @@ def run_gpu(args, name):
     return res
 
 
-GATE_DEPTHS = [0.0, 4096.0, 1024.0, 256.0, 64.0]
+GATE_DEPTHS = [0.0, 4096.0, 1024.0, 256.0, 128.0, 96.0, 64.0]
 
 
 def run_gate_curve(args, name, tuned=None):
@@ -307,11 +348,14 @@ def run_gate_curve(args, name, tuned=None):
         fps = args.steps / (time.perf_counter() - t0)
         gate_rays = c.gate_counts()[1]
         c.close()
-        allp, holes = L.evaluate(cnn, name, seed_off=1000, n_frames=24, skip=8, max_cnn_holes=args.max_cnn_holes,
-                                 gate_depth=d)
-        row = {"gate_depth": d, "fps": round(fps, 2), "psnr_learned_db": round(allp["learned"], 2),
-               "psnr_gate_db": round(allp["gate"], 2), "psnr_plain_db": round(allp["plain"], 2),
-               "gate_rays_last_frame": int(gate_rays)}
+        allp, holes, seq = L.evaluate(cnn, name, seed_off=1000, n_frames=24, skip=8,
+                                      max_cnn_holes=args.max_cnn_holes, gate_depth=d, detail=True)
+        db = lambda v: None if v is None or not np.isfinite(v) else round(float(v), 2)
+        row = {"gate_depth": d, "fps": round(fps, 2), "psnr_learned_db": db(allp["learned"]),
+               "psnr_gate_db": db(allp["gate"]), "psnr_plain_db": db(allp["plain"]),
+               "seq_psnr_learned_db": db(seq["learned"]["seq_db"]), "seq_psnr_gate_db": db(seq["gate"]["seq_db"]),
+               "identical_frames_learned": seq["learned"]["identical_frames"],
+               "eval_frames": seq["learned"]["frames"], "gate_rays_last_frame": int(gate_rays)}
         log(f"[bench] gate curve {name}: {row}")
         out.append(row)
     cnn.close()
@@ -346,13 +390,16 @@ def run_stress(args, name="cave_4096", n_timed=190):
     for f in range(10):
         warp.frame_warp(params[f])
     warp.sync()
-    warp.ray_count(reset=True)
+    warp.ray_stats(reset=True)
+    warp.pretrace_stats(reset=True)
     t0 = time.perf_counter()
     for f in range(10, total):
         warp.frame_warp(params[f])
     warp.sync()
     t_warp = time.perf_counter() - t0
-    rays = warp.ray_count(reset=True)
+    rays, _ = warp.ray_stats(reset=True)
+    pre_rays, merged, _ = warp.pretrace_stats(reset=True)
+    rays += merged  # the frames' hole and tile rays
     # per-frame GPU time and hole fraction on fresh state
     w2 = S.Context(0)
     w2.upload_octree(words, root, depth)
@@ -368,11 +415,13 @@ def run_stress(args, name="cave_4096", n_timed=190):
         ms.append(w2.frame_time())
         holes.append(w2.hole_count() / (rx * ry))
     full.enable_timing(mode=2)
+    full.ray_stats(reset=True)
     fm = []
     for f in range(10, total):
         full.frame_full(params[f])
         full.sync()
         fm.append(full.frame_time())
+    _, longest = full.ray_stats(reset=True)  # every pixel's ray of the timed frames
     for c in (warp, full, w2):
         c.close()
     return {"workload": f"{name}: {rx}x{ry}, SVO depth {depth}, 3-D noise cave shell over the whole volume, up to "
@@ -384,7 +433,8 @@ def run_stress(args, name="cave_4096", n_timed=190):
             "frames": total, "timed_frames": n_timed,
             "hole_list_fraction_mean": round(float(np.mean(holes)), 4),
             "hole_list_fraction_max": round(float(np.max(holes)), 4),
-            "rays_per_warp_frame": round(rays / n_timed)}
+            "rays_per_warp_frame": round(rays / n_timed), "rays_traced_ahead_per_frame": round(pre_rays / n_timed),
+            "longest_ray_trips": int(longest)}
 
 
 def run_batch(args):
@@ -408,6 +458,7 @@ def run_batch(args):
         c = S.Context(0)
         c.upload_octree(words, root, depth)
         c.resize(*v["res"])
+        c.set_pretrace(sets=0)  # view-by-view frames as the batch runs them: no pretrace
         ctxs.append(c)
     pixels = sum(v["res"][0] * v["res"][1] for v in views)
     warm = max(2, min(args.batch_warmup, n_frames - 1))
@@ -638,12 +689,21 @@ def main():
         "config": {"workload": f"{args.workload}: {wl['rx']}x{wl['ry']}, SVO depth {wl['depth']}, "
                                "warp+fill frames along the seeded flythrough",
                    "res": [wl["rx"], wl["ry"]], "octree_words": wl["words"]},
+        # per-frame GPU time of frames issued one at a time (host waits for
+        # each; no pretrace then), and the time between the completions of
+        # successive frames issued back to back (the timed frames)
         "warp_p50_ms": round(pct(g["warp_ms"], 50), 4), "warp_p99_ms": round(pct(g["warp_ms"], 99), 4),
+        "warp_interval_p50_ms": round(pct(g["warp_intervals"], 50), 4),
+        "warp_interval_p99_ms": round(pct(g["warp_intervals"], 99), 4),
+        "warp_fps_no_pretrace": round(g["warp_fps_no_pretrace"], 2),
+        "pretrace": g["pretrace"],
         "warp_fps_image_to_host": round(g["warp_fps_image_to_host"], 2) if "warp_fps_image_to_host" in g else None,
         "warp_fps_image_to_host_async": (round(g["warp_fps_image_to_host_async"], 2)
                                          if "warp_fps_image_to_host_async" in g else None),
         "full_fps": round(g["full_fps"], 2),
         "full_p50_ms": round(pct(g["full_ms"], 50), 4), "full_p99_ms": round(pct(g["full_ms"], 99), 4),
+        "full_interval_p50_ms": num(pct(g.get("full_intervals", []), 50)),
+        "full_interval_p99_ms": num(pct(g.get("full_intervals", []), 99)),
         "speedup_warp_vs_full": round(g["warp_fps"] / g["full_fps"], 3),
         "mrays_s_warp": round(g["mrays_warp"], 1), "mrays_s_full": round(g["mrays_full"], 1),
         "rays_per_warp_frame": round(g["rays_per_frame"]),

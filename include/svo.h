@@ -259,19 +259,35 @@ int svo_camera_path(uint64_t seed, int depth, int n_frames,
  *           (ms) in stage order and returns the number written (names via
  *           svo_stage_name)
  *   mode 2  per frame: an event pair around each frame (graph launches
- *           included)
+ *           included; a warp frame with a pretrace starts at its pretrace)
+ *   mode 3  frame completions: an event at the end of each of the first 512
+ *           frames after the mode is set; svo_frame_intervals returns the
+ *           times between successive completions (frames issued back to
+ *           back: the frame period the GPU sustains)
  * In modes 1 and 2, svo_frame_time gives the last frame's GPU time (ms)
- * once svo_sync has returned. */
+ * once svo_sync has returned.  Stage timing (mode 1) runs warp frames
+ * without a pretrace. */
 int svo_enable_timing(svo_ctx* ctx, int mode);
 int svo_frame_time(svo_ctx* ctx, float* ms);
-/* svo_frame_warp runs frames >= 2 as one captured HIP graph per context
- * (launch sequence fixed, the frame's camera and tile in a device-side view
- * record written before each launch).  On by default; 0 makes every frame
- * use direct launches.  The graph is re-captured after svo_resize,
- * svo_upload_octree and svo_set_tuning.  Results are identical either way. */
+/* Timing mode 3: up to max_n intervals (ms) between the completions of
+ * successive frames since the mode was set; *n receives the count
+ * (blocking). */
+int svo_frame_intervals(svo_ctx* ctx, float* ms, int max_n, int* n);
+/* svo_frame_warp runs frames >= 2 that have no pretrace (svo_set_pretrace)
+ * as one captured HIP graph per context (launch sequence fixed, the frame's
+ * camera and tile in a device-side view record written before each launch;
+ * a few instances of the graph launched in turn).  On by default; 0 makes
+ * every frame use direct launches.  The graph is re-captured after
+ * svo_resize, svo_upload_octree and svo_set_tuning.  Results are identical
+ * either way. */
 int svo_set_graph(svo_ctx* ctx, int enable);
 /* Rays traced by the ray kernels since the last reset (blocking). */
 int svo_ray_count(svo_ctx* ctx, uint64_t* rays, int reset);
+/* The same count and the loop trips of the longest of those rays (one trip
+ * is one pass of the reference's traversal loop, kernel.cl:152-211), both
+ * since the last reset of either (blocking).  No counterpart in the
+ * reference. */
+int svo_ray_stats(svo_ctx* ctx, uint64_t* rays, uint32_t* max_trips, int reset);
 int svo_stage_times(svo_ctx* ctx, float* ms, int n);
 const char* svo_stage_name(int stage);
 
@@ -312,6 +328,43 @@ int svo_set_ray_block(svo_ctx* ctx, int width);
  * 16x16-workgroup grid over its rectangle (raycast.h:361-366), so no
  * counterpart.  Default 1024; results do not depend on it. */
 int svo_set_long_first(svo_ctx* ctx, int long_trips);
+/* Warp frames (svo_frame_warp, frames >= 2, timing mode 0 or 2): each frame's
+ * refresh rectangle and every pixel of the 8x8 blocks whose last traced rays
+ * include one of at least long_trips loop trips are traced ahead, into one
+ * of `sets` result sets; each set has a stream of its own and is reused every
+ * `sets` frames, once the frame that read it has completed.  The frame's own
+ * ray launch waits for its pretrace and copies the stored result of each of
+ * its rays that was traced ahead, so the long rays of a frame overlap the
+ * frames before it.  The pretrace hands out its rays longest first (by the
+ * trips of the last ray traced at each pixel); with split_trips > 0 the
+ * pixels whose last ray had at least that many trips go to up to half of
+ * its waves, split_lanes lanes per wave.  Warp frames with a pretrace run as
+ * direct launches (svo_set_graph applies to the others).  With only_queued
+ * a frame is pretraced only while the warp frame before it is still on the
+ * GPU (the caller issues frames ahead): a frame the GPU starts at once has
+ * no earlier frame to overlap, and its own rays finish sooner without one.
+ * Defaults: 6 sets, long_trips 256, 1 workgroup per CU, no split,
+ * only_queued 1.  A primary ray depends
+ * only on the camera, the pixel and the octree (kernel.cl:594-694,
+ * :846-942): results do not depend on these settings.  HIP maps streams onto
+ * at most GPU_MAX_HW_QUEUES hardware queues (4 unless the environment sets
+ * it before the first HIP call); pretraces whose streams share one run one
+ * after the other.  The reference traces each frame's rays in the frame
+ * (raycast.h:296-387), so no counterpart. */
+typedef struct {
+    int sets;         /* 0 = off, 2..8 */
+    int long_trips;   /* >= 1 */
+    int wg_per_cu;    /* persistent workgroups per CU, 1..32; 0 = svo_set_tuning's */
+    int split_trips;  /* 0 = no split */
+    int split_lanes;  /* power of two, 1..64 */
+    int only_queued;  /* 1: pretrace a frame only while the warp frame before it is still running */
+} svo_pretrace_config;
+int svo_set_pretrace(svo_ctx* ctx, const svo_pretrace_config* cfg);
+int svo_get_pretrace(svo_ctx* ctx, svo_pretrace_config* cfg);
+/* Since the last reset (blocking): rays traced ahead, the loop trips of the
+ * longest of them, and the frame rays that took a result traced ahead
+ * (svo_ray_stats counts only the frame rays traced in the frame). */
+int svo_pretrace_stats(svo_ctx* ctx, uint64_t* rays, uint64_t* merged, uint32_t* max_trips, int reset);
 /* Learned frames (svo_frame_warp_learned): the number of workgroups the
  * fused CNN tile kernel runs with while it shares the GPU with the ray
  * launch (0 = two per CU, its grid when it runs alone).  Fewer workgroups

@@ -33,7 +33,7 @@ class Stats(C.Structure):
                 ("steps", C.c_uint64), ("pops", C.c_uint64), ("fetch_words", C.c_uint64),
                 ("desc_by_rek", C.c_uint64 * 16), ("last_rek", C.c_int64), ("last_lod", C.c_int64),
                 ("last_hit", C.c_int64), ("last_ix", C.c_int64), ("last_iy", C.c_int64), ("last_iz", C.c_int64),
-                ("last_sign", C.c_int64)]
+                ("last_sign", C.c_int64), ("max_iters", C.c_uint64)]
 
 
 class State(C.Structure):

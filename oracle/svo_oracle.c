@@ -518,6 +518,8 @@ static uint32_t trace_impl(const uint32_t* oct, size_t n_words, uint32_t root,
     if (st) {
         st->last_rek = rek; st->last_lod = lod; st->last_hit = node_test != 0;
         st->last_ix = ix; st->last_iy = iy; st->last_iz = iz; st->last_sign = sign_xyz;
+        const uint64_t trips = guard > (1u << 22) ? (1u << 22) : guard;
+        if (trips > st->max_iters) st->max_iters = trips;
     }
     return 0xff000000u + col;
 }
@@ -675,6 +677,7 @@ static void add_stats(orc_stats* a, const orc_stats* b)
     a->rays += b->rays; a->iters += b->iters; a->descends += b->descends;
     a->steps += b->steps; a->pops += b->pops; a->fetch_words += b->fetch_words;
     for (int i = 0; i < 16; i++) a->desc_by_rek[i] += b->desc_by_rek[i];
+    if (b->max_iters > a->max_iters) a->max_iters = b->max_iters;
 }
 
 /* kernel.cl:594-694: one ray per id in ids[2S .. 2S+ids[0]) */

@@ -83,6 +83,7 @@ typedef struct {
      * the octant-folded frame and the fold (sign_xyz); not accumulated */
     int64_t last_rek, last_lod, last_hit;
     int64_t last_ix, last_iy, last_iz, last_sign;
+    uint64_t max_iters;         /* loop trips of the longest ray (accumulated as a max) */
 } orc_stats;
 
 /* One primary ray for pixel (idx,idy); returns the screen word

@@ -15,6 +15,14 @@ static const char* const kStageNames[ST_N] = {"clear",    "proj_scatter", "proj_
                                               "sumids",   "writeids",     "gate",          "rays",
                                               "copy",     "fill_colorize", "cnn",          "cnn_apply"};
 
+constexpr int kPreMaxSets = 8;  // pretrace result sets at most
+constexpr int kViewSlots = 8;   // pinned staging slots of the graph frame's view record
+// instances of the warp frame's graph, launched in turn: a relaunch of one
+// instance waits on the host for its previous launch, so successive frames
+// use different instances and the host can run frames ahead
+constexpr int kGraphExecs = 4;
+constexpr int kEndRing = 512;  // timing mode 3: completion events of the first frames
+
 struct svo_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -43,7 +51,11 @@ struct svo_ctx {
     unsigned long long* d_keys = nullptr;
     uint32_t* d_snap = nullptr;
     int32_t* d_motion = nullptr;  // svo_fillhole_punch: motion x (N) then y (N)
-    uint32_t* d_queue = nullptr;  // [0..1] ray-queue counters, [2..3] the graph's, [8..9] ray count (u64)
+    // [0..1] ray-queue counters, [2..3] the graph's, [4..5] the pretrace's;
+    // frame ray launches: [8..9] rays (u64), [10] longest ray (trips), [12..13]
+    // rays merged from a pretrace (u64); pretrace launches: [16..17] rays,
+    // [18] longest ray
+    uint32_t* d_queue = nullptr;
     // learned-fill gate: [0] CNN tile count, [16] gate ray count
     uint32_t* d_gate = nullptr;
     uint32_t* d_gate_tiles = nullptr;  // tile indices sent to the CNN
@@ -81,17 +93,51 @@ struct svo_ctx {
     uint32_t* d_long_count = nullptr;
     int long_parity = 0;
     bool long_valid = false;
+    // pretrace of warp frames (svo_set_pretrace): the refresh rectangle and
+    // the 8x8 blocks whose last rays ran at least pre_trips trips are traced
+    // into one of pre_sets result sets (set s on pre_streams[s]) while
+    // earlier frames still run; the frame's own ray launch copies those of
+    // its rays
+    int pre_sets = 6;              // 0 = off
+    uint32_t pre_trips = 256;
+    int pre_wg = 1;                // workgroups per CU of the pretrace launch (0: wg_per_cu)
+    int pre_split_trips = 0;       // blocks with a known ray this long go to the split waves (0: no split)
+    int pre_split_shift = 3;       // log2 of the lanes per split wave
+    bool pre_only_queued = true;   // pretrace a frame only while the one before it is still running
+    hipEvent_t ev_frame_done = nullptr;  // the last warp frame's end
+    bool frame_done_recorded = false;
+    hipStream_t pre_streams[kPreMaxSets] = {};  // created on first use
+    uint32_t* d_known = nullptr;      // N: loop trips of the last ray traced per pixel
+    uint32_t* d_pre_stamp = nullptr;  // pre_sets x N
+    uint32_t* d_pre_word = nullptr;   // pre_sets x N
+    float4* d_pre_xyz = nullptr;      // pre_sets x N
+    uint32_t* d_pre_list = nullptr;   // per set: list header and pixel words (x | y << 16)
+    uint32_t* d_pre_blk = nullptr;    // per set: one word per 8x8 block
+    uint32_t* d_pre_queue = nullptr;  // per set: four ray-queue counters
+    uint8_t* d_pre_bkt = nullptr;     // per set: one byte per pixel
+    int pre_alloc_sets = 0;           // sets the buffers hold (0: not set up)
+    hipEvent_t ev_pre[kPreMaxSets] = {};   // a set's pretrace is complete
+    hipEvent_t ev_main[kPreMaxSets] = {};  // a set's frame is complete
+    bool main_recorded[kPreMaxSets] = {};
+    uint32_t pre_frames = 0;  // frames pretraced since the sets were cleared; the stamp is this count
+    int pre_seq[kPreMaxSets] = {};  // ray-queue slot parity per set
+    bool pre_active = false;  // the frame being issued has a pretrace in set pre_set, stamp pre_id
+    int pre_set = 0;
+    uint32_t pre_id = 0;
     // per-frame graph of the warp frame (frames >= 2): its kernels read the
-    // frame's view from d_view, refreshed from a pinned 2-slot staging ring
+    // frame's view from d_view, refreshed from a pinned staging ring
     bool use_graph = true;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[kGraphExecs] = {};
     void* d_view = nullptr;  // one view descriptor (device)
-    void* h_view = nullptr;  // two view descriptors (pinned host)
-    hipEvent_t view_done[2] = {};
+    void* h_view = nullptr;  // kViewSlots view descriptors (pinned host)
+    hipEvent_t view_done[kViewSlots] = {};
     uint32_t* d_first = nullptr;  // [0] = 0, [1] = rays of the frame
     int graph_frames = 0;
-    // timing: 0 off, 1 per stage (direct launches), 2 per frame
+    // timing: 0 off, 1 per stage (direct launches), 2 per frame, 3 frame
+    // completion times (end_ring)
     int timing = 0;
+    hipEvent_t end_ring[kEndRing] = {};
+    int end_count = 0;
     hipEvent_t ev[ST_N + 1] = {};
     float stage_ms[ST_N] = {};
     bool stage_used[ST_N] = {};

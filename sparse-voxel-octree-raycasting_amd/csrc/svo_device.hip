@@ -149,6 +149,16 @@ struct View {
     uint32_t* gate_rays;
     int gate_tiles_x, gate_n_tiles;
     float gate_depth;  // > 0: a tile with a reprojected pixel farther than this is raycast whole
+    // pretrace of warp frames: a ray launch on a second stream traces part
+    // of a frame's rays ahead (into one of a few result sets, each pixel
+    // stamped with the frame's id); the frame's own ray launch then copies
+    // the stored result for every ray whose pixel carries its id
+    const uint32_t* pre_stamp;  // the frame's set: stamps, words, positions (or null)
+    const uint32_t* pre_word;
+    const float4* pre_xyz;
+    uint32_t* stamp_out;        // pretrace launches: stamp per traced pixel (or null)
+    uint32_t pre_id;
+    uint32_t* trips_out;        // per pixel: loop trips of the last ray traced there (or null)
 };
 
 constexpr uint32_t QUAD_NONE = 0xffffffffu;  // list entry without a ray
@@ -169,6 +179,16 @@ struct RayArgs {
     int refill;
     int iters;
     unsigned long long* ray_count;  // rays traced, accumulated across launches
+    uint32_t* max_trips;            // loop trips of the longest ray, a max across launches
+    unsigned long long* merged;     // rays taken from a pretrace, accumulated
+    // split hand-out (list order only): the first *split_count rays go to the
+    // grid's first split_waves waves, at most 2^split_shift lanes of a wave
+    // at a time, from their own counters split_queue[0..1]; the other waves
+    // take the rest from queue
+    const uint32_t* split_count;    // or null
+    uint32_t* split_queue;
+    uint32_t split_waves;
+    int split_shift;
     uint32_t cache_words;           // octree prefix held in LDS
     uint32_t cache_last;            // max(cache_words, 1) - 1
     uint32_t oct_bytes;             // buffer extent of the octree (words + pad)
@@ -214,8 +234,11 @@ __device__ __forceinline__ uint32_t view_rays(const View& V)
     return (V.list ? min(*V.list_count, V.N) : 0u) + (V.list2 ? min(*V.list2_count, V.N) : 0u) + V.n_rect;
 }
 
-// Ray set-up: kernel.cl:883-908 and CAST_RAY prologue :115-150.
-__device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
+// Ray set-up: kernel.cl:883-908 and CAST_RAY prologue :115-150.  Returns
+// RAY_NONE (no ray), RAY_TRACE, or RAY_MERGED (the pixel's result came from
+// the frame's pretrace and has been written).
+constexpr int RAY_NONE = 0, RAY_TRACE = 1, RAY_MERGED = 2;
+__device__ __forceinline__ int init_ray(Ray& r, uint32_t k, const RayArgs& a)
 {
     int vi = 0;
     if (a.views) {  // last view whose first ray is <= k
@@ -234,27 +257,40 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
     int x, y;
     if (k < H + H2) {
         const uint32_t w = (k < H) ? V.list[k] : V.list2[k - H];
-        if (w == QUAD_NONE) return false;
+        if (w == QUAD_NONE) return RAY_NONE;
         x = (int)(w & 0xffffu);
         y = (int)(w >> 16);
         r.blk = V.block_long ? (uint32_t)((y >> (6 - V.blk_shift)) * V.tiles_x + (x >> V.blk_shift)) : NO_BLOCK;
     } else {
         k -= H + H2;
         const uint32_t t = k >> 6, w = k & 63u;
-        if (V.rect_skip && V.rect_skip[t]) return false;
+        if (V.rect_skip && V.rect_skip[t]) return RAY_NONE;
         r.blk = t;
         int tx = (int)(t % (uint32_t)V.tiles_x), ty = (int)(t / (uint32_t)V.tiles_x);
         const int bs = V.blk_shift;
         int lx = (tx << bs) + (int)(w & ((1u << bs) - 1u)), ly = (ty << (6 - bs)) + (int)(w >> bs);
-        if (lx >= V.ext_x || ly >= V.ext_y) return false;
+        if (lx >= V.ext_x || ly >= V.ext_y) return RAY_NONE;
         x = V.add_x + lx;
         y = V.add_y + ly;
     }
     // quad rays may pick the pixel one past the row or frame (their flat
     // offset then lies in the next row or in plane 1, inside the buffers)
-    if (!V.edge_ok && (x >= V.res_x || y >= V.res_y)) return false;
+    if (!V.edge_ok && (x >= V.res_x || y >= V.res_y)) return RAY_NONE;
     r.pix = (uint32_t)(y * V.res_x + x);
     r.view = vi;
+    if (V.pre_stamp && V.pre_stamp[r.pix] == V.pre_id) {
+        // traced ahead with this frame's camera: the same word and position
+        // finish_ray would write
+        const uint32_t word = V.pre_word[r.pix];
+        const float4 q = V.pre_xyz[r.pix];
+        V.screen[r.pix] = word;
+        float* c = (float*)V.coords + 4u * r.pix;
+        c[0] = q.x;
+        c[1] = q.y;
+        c[2] = q.z;
+        if (V.ray_image) V.ray_image[r.pix] = colorize_word(word);
+        return RAY_MERGED;
+    }
 
     const int D = a.depth;
     const float SCALE_MAX = (float)(1 << (D + 1));
@@ -291,7 +327,7 @@ __device__ __forceinline__ bool init_ray(Ray& r, uint32_t k, const RayArgs& a)
     r.test = 0;
     r.guard = 0;
     r.desync = 0u;
-    return true;
+    return RAY_TRACE;
 }
 
 // LDS: node stack, rows 0..D+1 x RAY_BLOCK lanes, then the node cache.
@@ -437,6 +473,8 @@ __device__ __forceinline__ void finish_ray(Ray& r, const RayArgs& a, OctBuf ob, 
     c[2] = pz / 16.0f;
     if (V.ray_image) V.ray_image[r.pix] = colorize_word(word);
     if (V.block_long && r.blk != NO_BLOCK && r.guard + 1u >= V.long_trips) V.block_long[r.blk] = 1u;
+    if (V.stamp_out) V.stamp_out[r.pix] = V.pre_id;
+    if (V.trips_out) V.trips_out[r.pix] = r.guard;
 }
 
 // lanes 0, 64/K, 2*64/K, ... of a wave (K = 2^ks lanes)
@@ -479,32 +517,57 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     const unsigned long long elig = ks >= 6 ? ~0ull : lane_pattern(ks);
     const uint32_t P = (a.views && a.n_views > 1) ? total : total - (a.views ? a.views[0].n_rect : a.v.n_rect);
     const uint32_t rows = (P + K - 1u) >> ks;
-    const bool spread = a.spread_shift < ks &&
+    const bool spread = !a.split_count && a.spread_shift < ks &&
                         (!a.spread_fit || total <= gridDim.x * (uint32_t)(RAY_BLOCK / 64) * K);
     const uint32_t qtotal = spread ? rows * K + (total - P) : total;
     const uint32_t gmask = (1u << a.spread_shift) - 1u;
-    const int refill = min(a.refill, (int)K);
     if (blockIdx.x == 0 && tid == 0) a.queue[a.slot ^ 1] = 0u;  // next launch's counter
+    // this wave's queue: positions k < qn map to rays lo + k (list order) or
+    // through the spread order
+    uint32_t* q = a.queue + a.slot;
+    uint32_t lo = 0, qn = qtotal;
+    unsigned long long weli = elig;
+    uint32_t Kw = K;
+    if (a.split_count) {
+        const uint32_t ns = min(*a.split_count, total);
+        const uint32_t wid = blockIdx.x * (uint32_t)(RAY_BLOCK / 64) + (uint32_t)(tid >> 6);
+        if (blockIdx.x == 0 && tid == 0) a.split_queue[a.slot ^ 1] = 0u;
+        // as many split waves as the split rays need, at most split_waves
+        const uint32_t sw = min(a.split_waves, (ns + (1u << a.split_shift) - 1u) >> a.split_shift);
+        if (wid < sw) {
+            q = a.split_queue + a.slot;
+            qn = ns;
+            Kw = 1u << a.split_shift;
+            weli = a.split_shift >= 6 ? ~0ull : lane_pattern(a.split_shift);
+        } else {
+            lo = ns;
+            qn = total - ns;
+        }
+    }
+    const int refill = min(a.refill, (int)Kw);
 
     Ray r;
     bool active = false, exhausted = false;
-    uint32_t traced = 0;  // lane 0 of each wave: rays started by the wave
+    uint32_t traced = 0;   // lane 0 of each wave: rays started by the wave
+    uint32_t merged = 0;   // lane 0 of each wave: pretraced rays the wave copied
+    uint32_t longest = 0;  // trips of the longest ray this lane finished
     for (;;) {
         const unsigned long long idle_all = __ballot(!active);
-        const unsigned long long idle = idle_all & elig;
+        const unsigned long long idle = idle_all & weli;
         const int n_idle = __popcll(idle);
-        if (!exhausted && (n_idle >= refill || n_idle == (int)K)) {
+        if (!exhausted && (n_idle >= refill || n_idle == (int)Kw)) {
             const int leader = __ffsll((long long)idle) - 1;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&a.queue[a.slot], (uint32_t)n_idle);
+            if (lane == leader) base = atomicAdd(q, (uint32_t)n_idle);
             base = __shfl(base, leader);
-            if (base + (uint32_t)n_idle >= qtotal) exhausted = true;
+            if (base + (uint32_t)n_idle >= qn) exhausted = true;
+            bool took = false;
             if ((idle >> lane) & 1ull) {
                 const int rank = __popcll(idle & ((1ull << lane) - 1ull));
                 const uint32_t k = base + (uint32_t)rank;
                 const uint32_t l = k & (K - 1u);
-                uint32_t kr = k;
-                bool valid = k < qtotal;
+                uint32_t kr = lo + k;
+                bool valid = k < qn;
                 if (spread) {
                     if (k < rows * K) {
                         kr = (((l >> a.spread_shift) * rows + (k >> ks)) << a.spread_shift) | (l & gmask);
@@ -513,9 +576,14 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
                         kr = P + (k - rows * K);
                     }
                 }
-                if (valid) active = init_ray(r, kr, a);
+                if (valid) {
+                    const int st = init_ray(r, kr, a);
+                    active = st == RAY_TRACE;
+                    took = st == RAY_MERGED;
+                }
             }
             traced += (uint32_t)__popcll(__ballot(active)) - (uint32_t)__popcll(~idle_all);
+            merged += (uint32_t)__popcll(__ballot(took));
         }
         if (__ballot(active) == 0ull) {
             if (exhausted) break;
@@ -540,10 +608,14 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
         }
         if (active && !run) {
             finish_ray(r, a, ob, stk, cache, tid);
+            longest = max(longest, r.guard);
             active = false;
         }
     }
+    for (int o = 32; o > 0; o >>= 1) longest = max(longest, (uint32_t)__shfl_xor((int)longest, o));
     if (lane == 0 && traced) atomicAdd(a.ray_count, (unsigned long long)traced);
+    if (lane == 0 && merged) atomicAdd(a.merged, (unsigned long long)merged);
+    if (lane == 0 && longest) atomicMax(a.max_trips, longest);
 }
 
 // ---------------------------------------------------------------- warp
@@ -1030,6 +1102,8 @@ __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max
             return svo_fail(SVO_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
     } while (0)
 
+static void free_pre(svo_ctx* c);
+
 static int free_frame(svo_ctx* c)
 {
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);  // no copy reads a freed image
@@ -1069,6 +1143,9 @@ static int free_frame(svo_ctx* c)
     (void)hipFree(c->d_long_count);
     c->d_block_long = c->d_long_list = c->d_long_count = nullptr;
     c->long_valid = false;
+    free_pre(c);
+    (void)hipFree(c->d_known);
+    c->d_known = nullptr;
     c->res_x = c->res_y = 0;
     c->N = c->S = 0;
     return SVO_OK;
@@ -1077,8 +1154,9 @@ static int free_frame(svo_ctx* c)
 // the captured frame graph holds buffer pointers and launch knobs
 static void drop_graph(svo_ctx* c)
 {
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
+    for (auto& g : c->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& g : c->graph) g = nullptr;
 }
 
 extern "C" int svo_create(svo_ctx** out, int device)
@@ -1099,8 +1177,12 @@ extern "C" int svo_create(svo_ctx** out, int device)
               hipStreamCreateWithFlags(&c->cnn_stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&c->cnn_fork, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->cnn_join, hipEventDisableTiming) == hipSuccess &&
-              hipMalloc(&c->d_queue, 64) == hipSuccess && hipMemset(c->d_queue, 0, 64) == hipSuccess;
+              hipMalloc(&c->d_queue, 128) == hipSuccess && hipMemset(c->d_queue, 0, 128) == hipSuccess;
     for (int i = 0; ok && i <= ST_N; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&c->ev_frame_done, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kPreMaxSets; i++)
+        ok = hipEventCreateWithFlags(&c->ev_pre[i], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_main[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         svo_destroy(c);
         return svo_fail(SVO_ERR_HIP, "context set-up failed on device %d", device);
@@ -1115,6 +1197,8 @@ extern "C" int svo_destroy(svo_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->cnn_stream) (void)hipStreamSynchronize(c->cnn_stream);
+    for (auto& st : c->pre_streams)
+        if (st) (void)hipStreamSynchronize(st);
     free_frame(c);
     for (auto& e : c->image_copied)
         if (e) (void)hipEventDestroy(e);
@@ -1133,6 +1217,15 @@ extern "C" int svo_destroy(svo_ctx* c)
     if (c->cnn_fork) (void)hipEventDestroy(c->cnn_fork);
     if (c->cnn_join) (void)hipEventDestroy(c->cnn_join);
     if (c->cnn_stream) (void)hipStreamDestroy(c->cnn_stream);
+    for (int i = 0; i < kPreMaxSets; i++) {
+        if (c->ev_pre[i]) (void)hipEventDestroy(c->ev_pre[i]);
+        if (c->ev_main[i]) (void)hipEventDestroy(c->ev_main[i]);
+    }
+    for (auto& st : c->pre_streams)
+        if (st) (void)hipStreamDestroy(st);
+    for (auto& e : c->end_ring)
+        if (e) (void)hipEventDestroy(e);
+    if (c->ev_frame_done) (void)hipEventDestroy(c->ev_frame_done);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SVO_OK;
@@ -1195,7 +1288,8 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
         hipMalloc(&c->d_gate_rays, N * 4) != hipSuccess ||
         hipMalloc(&c->d_block_long, max_ray_blocks(res_x, res_y) * 8) != hipSuccess ||
         hipMalloc(&c->d_long_list, N * 4) != hipSuccess ||
-        hipMalloc(&c->d_long_count, 512) != hipSuccess) {
+        hipMalloc(&c->d_long_count, 512) != hipSuccess ||
+        hipMalloc(&c->d_known, (N + res_x + 64) * 4) != hipSuccess) {
         free_frame(c);
         return svo_fail(SVO_ERR_NOMEM, "frame buffers for %dx%d", res_x, res_y);
     }
@@ -1208,6 +1302,7 @@ extern "C" int svo_resize(svo_ctx* c, int res_x, int res_y)
     HIPCHECK(hipMemset(c->d_snap, 0, N * 4));
     HIPCHECK(hipMemset(c->d_gate, 0, 128));
     HIPCHECK(hipMemset(c->d_long_count, 0, 512));
+    HIPCHECK(hipMemset(c->d_known, 0, (N + res_x + 64) * 4));
     c->n_tiles_x = (res_x + 15) / 16;
     c->n_tiles_y = (res_y + 15) / 16;
     HIPCHECK(hipDeviceSynchronize());
@@ -1244,11 +1339,15 @@ static inline void mark(svo_ctx* c, int st)
 static void frame_begin(svo_ctx* c)
 {
     for (int i = 0; i < ST_N; i++) c->stage_used[i] = false;
-    c->frame_timed = c->timing != 0;
+    c->frame_timed = c->timing == 1 || c->timing == 2;
     if (c->timing == 2) (void)hipEventRecord(c->ev[0], c->stream);
 }
 static int frame_end(svo_ctx* c)
 {
+    if (c->timing == 3) {
+        if (c->end_count < kEndRing) HIPCHECK(hipEventRecord(c->end_ring[c->end_count++], c->stream));
+        return SVO_OK;
+    }
     if (c->timing) HIPCHECK(hipEventRecord(c->ev[ST_N], c->stream));
     return SVO_OK;
 }
@@ -1292,6 +1391,15 @@ static View make_view(svo_ctx* c, const svo_frame_params* p)
     v.gate_tiles_x = c->n_tiles_x;
     v.gate_n_tiles = c->n_tiles_x * c->n_tiles_y;
     v.gate_depth = c->gate_depth;
+    // quad rays may write one row past the frame: d_known holds N + res_x + 64
+    v.trips_out = c->d_known;
+    if (c->pre_active) {
+        const size_t o = (size_t)c->pre_set * c->N;
+        v.pre_stamp = c->d_pre_stamp + o;
+        v.pre_word = c->d_pre_word + o;
+        v.pre_xyz = c->d_pre_xyz + o;
+        v.pre_id = c->pre_id;
+    }
     view_camera(v, p);
     return v;
 }
@@ -1402,8 +1510,22 @@ static int launch_fill(const Launch& L, const View& v, const uint32_t* snap, int
 }
 
 // persistent ray launch over one view (views == null) or a batch table
+// counters: [0..1] rays traced (u64), [2] longest ray (trips), [4..5] rays
+// merged from a pretrace (u64); default the context's frame counters
+// spread_shift: the hand-out spread (default the context's; 6 = list order);
+// wg_per_cu: persistent workgroups per CU (default the context's); split:
+// the split hand-out (list order) with its counters split_queue[0..1] --
+// half of the grid's waves take the first *split_count rays, 2^split_shift
+// lanes at a time
+struct Split {
+    const uint32_t* count = nullptr;
+    uint32_t* queue = nullptr;
+    int shift = 6;
+};
 static int launch_rays_v(svo_ctx* c, hipStream_t stream, const View& v, const View* views,
-                         const uint32_t* view_first, int n_views, uint32_t* queue, int& seq)
+                         const uint32_t* view_first, int n_views, uint32_t* queue, int& seq,
+                         uint32_t* counters = nullptr, int spread_shift = -1, int wg_per_cu = 0,
+                         const Split& split = Split())
 {
     RayArgs a;
     memset(&a, 0, sizeof(a));
@@ -1419,14 +1541,23 @@ static int launch_rays_v(svo_ctx* c, hipStream_t stream, const View& v, const Vi
     seq++;
     a.refill = c->refill;
     a.iters = c->iters;
-    a.spread_shift = c->spread_shift;
+    a.spread_shift = spread_shift >= 0 ? spread_shift : c->spread_shift;
     a.spread_fit = c->spread_fit;
     a.lanes_shift = c->lanes_shift;
-    a.ray_count = (unsigned long long*)(c->d_queue + 8);
+    if (!counters) counters = c->d_queue + 8;
+    a.ray_count = (unsigned long long*)counters;
+    a.max_trips = counters + 2;
+    a.merged = (unsigned long long*)(counters + 4);
     a.cache_words = std::min(c->cache_req, c->normal_words);
     a.cache_last = a.cache_words ? a.cache_words - 1 : 0;
     a.oct_bytes = (uint32_t)std::min<size_t>((c->n_words + kOctPad) * 4, 0xfffffffcull);
-    const unsigned grid = (unsigned)(c->n_cu * c->wg_per_cu);
+    const unsigned grid = (unsigned)(c->n_cu * (wg_per_cu > 0 ? wg_per_cu : c->wg_per_cu));
+    if (split.count) {
+        a.split_count = split.count;
+        a.split_queue = split.queue;
+        a.split_shift = split.shift;
+        a.split_waves = grid * (unsigned)(RAY_BLOCK / 64) / 2u;
+    }
     const size_t lds = ((size_t)(c->depth + STK_ROWS_EXTRA) * RAY_BLOCK + std::max(a.cache_words, 1u)) * 4;
     if (a.cache_words)
         hipLaunchKernelGGL(k_rays<true>, dim3(grid), dim3(RAY_BLOCK), lds, stream, a);
@@ -1487,9 +1618,9 @@ static int build_graph(svo_ctx* c)
 {
     if (!c->d_view) {
         bool ok = hipMalloc(&c->d_view, sizeof(View)) == hipSuccess &&
-                  hipHostMalloc(&c->h_view, 2 * sizeof(View)) == hipSuccess &&
+                  hipHostMalloc(&c->h_view, kViewSlots * sizeof(View)) == hipSuccess &&
                   hipMalloc(&c->d_first, 8) == hipSuccess;
-        for (int i = 0; ok && i < 2; i++)
+        for (int i = 0; ok && i < kViewSlots; i++)
             ok = hipEventCreateWithFlags(&c->view_done[i], hipEventDisableTiming) == hipSuccess &&
                  hipEventRecord(c->view_done[i], c->stream) == hipSuccess;
         if (!ok) return svo_fail(SVO_ERR_HIP, "frame graph buffers");
@@ -1509,32 +1640,257 @@ static int build_graph(svo_ctx* c)
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (!rc && e != hipSuccess) rc = svo_fail(SVO_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-    if (!rc && hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0) != hipSuccess) {
-        c->graph = nullptr;
-        rc = svo_fail(SVO_ERR_HIP, "graph instantiation failed");
-    }
+    for (int i = 0; !rc && i < kGraphExecs; i++)
+        if (hipGraphInstantiate(&c->graph[i], g, nullptr, nullptr, 0) != hipSuccess) {
+            c->graph[i] = nullptr;
+            rc = svo_fail(SVO_ERR_HIP, "graph instantiation failed");
+        }
+    if (rc) drop_graph(c);
     if (g) (void)hipGraphDestroy(g);
     return rc;
 }
 
 static int frame_warp_graph(svo_ctx* c, const svo_frame_params* p)
 {
-    if (!c->graph) RC(build_graph(c));
+    if (!c->graph[0]) RC(build_graph(c));
     View v = make_view(c, p);
     v.plane_lo = c->plane1_holes ? 2 : 1;
     v.n_planes = 3 - v.plane_lo;
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
     set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
-    const int slot = c->graph_frames & 1;
+    const int slot = c->graph_frames % kViewSlots;
     View* hv = (View*)c->h_view + slot;
     HIPCHECK(hipEventSynchronize(c->view_done[slot]));  // its previous copy has been consumed
     *hv = v;
     HIPCHECK(hipMemcpyAsync(c->d_view, hv, sizeof(View), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipEventRecord(c->view_done[slot], c->stream));
-    HIPCHECK(hipGraphLaunch(c->graph, c->stream));
+    HIPCHECK(hipGraphLaunch(c->graph[c->graph_frames % kGraphExecs], c->stream));
     c->graph_frames++;
     return SVO_OK;
+}
+
+// ---------------------------------------------------------------- pretrace
+// Per set: pixel list of N + kPreHdr words ([0] length, [1] split count,
+// [8..31] rays per bucket, [32..55] bucket cursors, pixel words x | y << 16
+// from kPreHdr), one word per 8x8 block, one byte per pixel (its sort
+// bucket), four ray-queue counters.
+constexpr int kPreHdr = 64;
+constexpr int kPreBuckets = 24;  // bucket b: blocks whose longest known ray has floor(log2(trips)) = b
+
+static size_t pre_blocks(const svo_ctx* c) { return (size_t)((c->res_x + 7) / 8) * (size_t)((c->res_y + 7) / 8); }
+
+static void free_pre(svo_ctx* c)
+{
+    (void)hipFree(c->d_pre_stamp);
+    (void)hipFree(c->d_pre_word);
+    (void)hipFree(c->d_pre_xyz);
+    (void)hipFree(c->d_pre_list);
+    (void)hipFree(c->d_pre_blk);
+    (void)hipFree(c->d_pre_queue);
+    (void)hipFree(c->d_pre_bkt);
+    c->d_pre_stamp = c->d_pre_word = c->d_pre_list = c->d_pre_blk = c->d_pre_queue = nullptr;
+    c->d_pre_bkt = nullptr;
+    c->d_pre_xyz = nullptr;
+    c->pre_alloc_sets = 0;
+    c->pre_frames = 0;
+    for (auto& m : c->main_recorded) m = false;
+}
+
+static int ensure_pre(svo_ctx* c)
+{
+    if (c->pre_alloc_sets == c->pre_sets) return SVO_OK;
+    HIPCHECK(hipStreamSynchronize(c->stream));  // no frame reads the old sets
+    free_pre(c);
+    const size_t K = (size_t)c->pre_sets, n = K * c->N;
+    for (int i = 0; i < c->pre_sets; i++)
+        if (!c->pre_streams[i]) HIPCHECK(hipStreamCreateWithFlags(&c->pre_streams[i], hipStreamNonBlocking));
+    if (hipMalloc(&c->d_pre_stamp, n * 4) != hipSuccess || hipMalloc(&c->d_pre_word, n * 4) != hipSuccess ||
+        hipMalloc(&c->d_pre_xyz, n * 16) != hipSuccess ||
+        hipMalloc(&c->d_pre_list, K * (c->N + kPreHdr) * 4) != hipSuccess ||
+        hipMalloc(&c->d_pre_blk, K * pre_blocks(c) * 4) != hipSuccess ||
+        hipMalloc(&c->d_pre_queue, K * 16) != hipSuccess || hipMalloc(&c->d_pre_bkt, n) != hipSuccess) {
+        free_pre(c);
+        return svo_fail(SVO_ERR_NOMEM, "%d pretrace sets of %zu pixels", c->pre_sets, c->N);
+    }
+    HIPCHECK(hipMemset(c->d_pre_stamp, 0, n * 4));  // stamps start at 1
+    HIPCHECK(hipMemset(c->d_pre_queue, 0, K * 16));
+    HIPCHECK(hipDeviceSynchronize());
+    c->pre_alloc_sets = c->pre_sets;
+    for (auto& q : c->pre_seq) q = 0;
+    return SVO_OK;
+}
+
+__device__ __forceinline__ uint32_t trip_bucket(uint32_t t) { return t ? 31u - (uint32_t)__clz(t) : 0u; }
+
+// Pretrace list, pass 1.  A workgroup of 256 covers 8 x 4 screen blocks of
+// 8x8 pixels (one column of 8 pixels per thread, coalesced rows).  A block
+// is pretraced whole when the longest known ray among its pixels (the
+// `known` map: trips of the last ray traced at each pixel, read while other
+// frames' rays update it -- any value read only changes which rays are
+// traced ahead) has at least `trips` trips; otherwise only its pixels inside
+// the refresh rectangle are.  blk[b] = (whole << 31) | pixels (0: none);
+// bkt[pixel] = k = floor(log2(trips)) of the pixel's own known ray (the
+// value pass 2 sorts by; the known map may change in between), and hdr[8 +
+// k] counts the pretraced pixels of bucket k.
+__global__ __launch_bounds__(256) void k_pre_scan(const uint32_t* __restrict__ known, int res_x, int res_y, int ax,
+                                                  int ay, int ex, int ey, uint32_t trips, uint32_t* __restrict__ blk,
+                                                  uint8_t* __restrict__ bkt, uint32_t* __restrict__ hdr)
+{
+    __shared__ uint32_t hist[kPreBuckets];
+    const int t = threadIdx.x;
+    if (t < kPreBuckets) hist[t] = 0;
+    __syncthreads();
+    const int x = blockIdx.x * 64 + (t & 63);
+    const int by = blockIdx.y * 4 + (t >> 6);  // block row
+    const int y0 = by * 8;
+    const bool col_ok = x < res_x && y0 < res_y;
+    const int h = col_ok ? min(8, res_y - y0) : 0;
+    uint32_t kn[8];
+    uint32_t m = 0, in_tile = 0;
+    for (int j = 0; j < 8; j++) {
+        kn[j] = j < h ? known[(size_t)(y0 + j) * res_x + x] : 0u;
+        m = max(m, kn[j]);
+        in_tile += (j < h && x >= ax && x < ax + ex && y0 + j >= ay && y0 + j < ay + ey) ? 1u : 0u;
+    }
+    uint32_t cnt = (uint32_t)h;
+    for (int o = 1; o < 8; o <<= 1) {  // over the block's 8 columns
+        m = max(m, (uint32_t)__shfl_xor((int)m, o));
+        in_tile += (uint32_t)__shfl_xor((int)in_tile, o);
+        cnt += (uint32_t)__shfl_xor((int)cnt, o);
+    }
+    const bool whole = m >= trips && cnt > 0;
+    for (int j = 0; j < h; j++)
+        if (whole || (x >= ax && x < ax + ex && y0 + j >= ay && y0 + j < ay + ey)) {
+            const uint32_t b = trip_bucket(kn[j]);
+            bkt[(size_t)(y0 + j) * res_x + x] = (uint8_t)b;
+            atomicAdd(&hist[b], 1u);
+        }
+    const int nbx = (res_x + 7) >> 3;
+    const int bx = x >> 3;
+    if ((t & 7) == 0 && bx < nbx && y0 < res_y) {
+        const uint32_t n = whole ? cnt : in_tile;
+        blk[(size_t)by * nbx + bx] = n ? (((uint32_t)whole << 31) | n) : 0u;
+    }
+    __syncthreads();
+    if (t < kPreBuckets && hist[t]) atomicAdd(&hdr[8 + t], hist[t]);
+}
+
+// Pretrace list, pass 2: the pixels in descending order of their own known
+// trips' bucket (the longest rays first, so the ray launch hands them out
+// first); within a bucket in no defined order.  hdr[0] = list length,
+// hdr[1] = the pixels of buckets >= split_bucket (0 when split_bucket is
+// outside 0..23).
+__global__ __launch_bounds__(256) void k_pre_emit(const uint8_t* __restrict__ bkt, int res_x, int res_y, int ax,
+                                                  int ay, int ex, int ey, int split_bucket,
+                                                  const uint32_t* __restrict__ blk, uint32_t* __restrict__ hdr,
+                                                  uint32_t* __restrict__ list)
+{
+    __shared__ uint32_t first[kPreBuckets];  // this workgroup's first list position per bucket
+    __shared__ uint32_t hist[kPreBuckets];
+    const int t = threadIdx.x;
+    if (t < kPreBuckets) hist[t] = 0;
+    __syncthreads();
+    const int x = blockIdx.x * 64 + (t & 63);
+    const int by = blockIdx.y * 4 + (t >> 6);
+    const int y0 = by * 8;
+    const int nbx = (res_x + 7) >> 3;
+    const int bx = x >> 3;
+    const bool col_ok = x < res_x && y0 < res_y && bx < nbx;
+    const uint32_t w = col_ok ? blk[(size_t)by * nbx + bx] : 0u;
+    const bool whole = (w >> 31) != 0u;
+    const int h = (col_ok && (w & 0xffffu)) ? min(8, res_y - y0) : 0;
+    uint32_t bk[8];
+    uint32_t take = 0;  // bit j: pixel (x, y0 + j) is pretraced
+    for (int j = 0; j < h; j++) {
+        const int y = y0 + j;
+        if (whole || (x >= ax && x < ax + ex && y >= ay && y < ay + ey)) {
+            take |= 1u << j;
+            bk[j] = bkt[(size_t)y * res_x + x];
+            atomicAdd(&hist[bk[j]], 1u);
+        }
+    }
+    __syncthreads();
+    if (t < kPreBuckets) {
+        uint32_t before = 0;  // pixels of the longer buckets
+        for (int b = kPreBuckets - 1; b > t; b--) before += hdr[8 + b];
+        first[t] = hist[t] ? before + atomicAdd(&hdr[32 + t], hist[t]) : 0u;
+        hist[t] = 0;
+        if (blockIdx.x == 0 && blockIdx.y == 0) {
+            if (t == 0) hdr[0] = before + hdr[8];  // list length
+            if (t == split_bucket) hdr[1] = before + hdr[8 + t];
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < h; j++)
+        if ((take >> j) & 1u) list[first[bk[j]] + atomicAdd(&hist[bk[j]], 1u)] = (uint32_t)x | ((uint32_t)(y0 + j) << 16);
+}
+
+// Pretrace a warp frame on the set's own stream: the refresh rectangle and
+// the long blocks (k_pre_scan), longest known rays first, with the frame's
+// camera, into set s, stamped with the frame's id.  Successive frames use
+// successive sets and streams, so their pretraces overlap one another and
+// the frames before them; a set is reused after the frame that read it last
+// has completed, and the frame's own work on the main stream waits for its
+// pretrace.  A primary ray depends only on the camera, the pixel and the
+// octree (kernel.cl:594-694, :846-942), so the copied results are those the
+// frame's own rays would compute.
+static int pretrace(svo_ctx* c, const svo_frame_params* p)
+{
+    RC(ensure_pre(c));
+    const int s = (int)(c->pre_frames % (uint32_t)c->pre_sets);
+    const uint32_t id = c->pre_frames + 1u;
+    const size_t N = c->N;
+    hipStream_t st = c->pre_streams[s];
+    if (c->timing == 2) (void)hipEventRecord(c->ev[0], st);  // the frame starts with its pretrace
+    if (c->main_recorded[s]) HIPCHECK(hipStreamWaitEvent(st, c->ev_main[s], 0));
+    int ax, ay, ex, ey;
+    svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
+    uint32_t* hdr = c->d_pre_list + (size_t)s * (N + kPreHdr);
+    uint32_t* list = hdr + kPreHdr;
+    uint32_t* blk = c->d_pre_blk + (size_t)s * pre_blocks(c);
+    hipLaunchKernelGGL(k_memset, dim3(1), dim3(256), 0, st, hdr, 0u, (size_t)kPreHdr);
+    HIPCHECK(hipGetLastError());
+    const dim3 grid((unsigned)((c->res_x + 63) / 64), (unsigned)(((c->res_y + 7) / 8 + 3) / 4));
+    uint8_t* bkt = c->d_pre_bkt + (size_t)s * N;
+    hipLaunchKernelGGL(k_pre_scan, grid, dim3(256), 0, st, (const uint32_t*)c->d_known, c->res_x, c->res_y, ax, ay,
+                       ex, ey, c->pre_trips, blk, bkt, hdr);
+    HIPCHECK(hipGetLastError());
+    const int split_bucket = c->pre_split_trips > 0 ? 31 - __builtin_clz((unsigned)c->pre_split_trips) : -1;
+    hipLaunchKernelGGL(k_pre_emit, grid, dim3(256), 0, st, (const uint8_t*)bkt, c->res_x, c->res_y, ax, ay, ex, ey,
+                       split_bucket, (const uint32_t*)blk, hdr, list);
+    HIPCHECK(hipGetLastError());
+    View v = make_view(c, p);  // no pretrace fields: pre_active is clear between frames
+    set_rays(v, list, hdr, 0, 0, 0, 0, nullptr);
+    v.screen = c->d_pre_word + s * N;
+    v.coords = c->d_pre_xyz + s * N;
+    v.stamp_out = c->d_pre_stamp + s * N;
+    v.pre_id = id;
+    // list order (the longest rays first, neighbours together), no spread;
+    // the longest go to waves of their own, a few lanes each
+    Split split;
+    if (split_bucket >= 0) {
+        split.count = hdr + 1;
+        split.queue = c->d_pre_queue + 4 * s + 2;
+        split.shift = c->pre_split_shift;
+    }
+    RC(launch_rays_v(c, st, v, nullptr, nullptr, 1, c->d_pre_queue + 4 * s, c->pre_seq[s], c->d_queue + 16, 6,
+                     c->pre_wg, split));
+    HIPCHECK(hipEventRecord(c->ev_pre[s], st));
+    HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[s], 0));
+    c->pre_frames++;
+    c->pre_active = true;
+    c->pre_set = s;
+    c->pre_id = id;
+    return SVO_OK;
+}
+
+// after the frame's work is queued: its set may be reused once it completes
+static void pretrace_done(svo_ctx* c)
+{
+    if (!c->pre_active) return;
+    c->pre_active = false;
+    if (hipEventRecord(c->ev_main[c->pre_set], c->stream) == hipSuccess) c->main_recorded[c->pre_set] = true;
 }
 
 extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
@@ -1543,18 +1899,34 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     if (!p) return svo_fail(SVO_ERR_ARG, "null frame params");
     HIPCHECK(hipSetDevice(c->device));
     take_image(c);
+    // a pretrace frame runs direct launches: a graph launch here waits on the
+    // host for the GPU, which would keep the host from issuing the next
+    // frames' pretraces ahead
+    // with only_queued, a frame is pretraced only while the frame before it
+    // is still on the GPU (frames issued ahead of it): a frame the GPU
+    // starts right away has no earlier frame to overlap
+    bool pre = c->pre_sets > 0 && c->timing != 1 && p->frame >= 2;
+    if (pre && c->pre_only_queued)
+        pre = c->frame_done_recorded && hipEventQuery(c->ev_frame_done) == hipErrorNotReady;
+    const bool graph = c->use_graph && c->timing != 1 && p->frame >= 2 && !pre;
+    if (graph && !c->graph[0]) RC(build_graph(c));
     frame_begin(c);
-    if (c->use_graph && c->timing != 1 && p->frame >= 2) {
-        RC(frame_warp_graph(c, p));
-        return frame_end(c);
+    if (pre) RC(pretrace(c, p));
+    int rc;
+    if (graph) {
+        rc = frame_warp_graph(c, p);
+    } else {
+        rc = warp_front(c, p);
+        const Launch L = single(c);
+        const View v = make_view(c, p);
+        if (!rc) mark(c, ST_COPY);
+        if (!rc) rc = launch_copy(L, v, 2);
+        if (!rc) mark(c, ST_FILTER);
+        if (!rc) rc = launch_fill(L, v, nullptr, 1, 1);
     }
-    RC(warp_front(c, p));
-    const Launch L = single(c);
-    const View v = make_view(c, p);
-    mark(c, ST_COPY);
-    RC(launch_copy(L, v, 2));
-    mark(c, ST_FILTER);
-    RC(launch_fill(L, v, nullptr, 1, 1));
+    pretrace_done(c);
+    RC(rc);
+    if (hipEventRecord(c->ev_frame_done, c->stream) == hipSuccess) c->frame_done_recorded = true;
     return frame_end(c);
 }
 
@@ -1683,6 +2055,60 @@ extern "C" int svo_set_ray_block(svo_ctx* c, int width)
         c->blk_shift = shift;
         c->long_valid = false;  // the long-block flags index blocks of the old shape
     }
+    return SVO_OK;
+}
+
+extern "C" int svo_set_pretrace(svo_ctx* c, const svo_pretrace_config* cfg)
+{
+    if (!c || !cfg) return svo_fail(SVO_ERR_ARG, "null argument");
+    const int sets = cfg->sets, long_trips = cfg->long_trips, wg_per_cu = cfg->wg_per_cu;
+    if (wg_per_cu < 0 || wg_per_cu > 32) return svo_fail(SVO_ERR_ARG, "wg_per_cu %d outside 0..32", wg_per_cu);
+    if (cfg->split_trips < 0 || cfg->split_trips > SVO_MAX_RAY_ITERS)
+        return svo_fail(SVO_ERR_ARG, "split_trips %d outside 0..%d", cfg->split_trips, (int)SVO_MAX_RAY_ITERS);
+    const int sl = cfg->split_lanes;
+    if (sl < 1 || sl > 64 || (sl & (sl - 1)))
+        return svo_fail(SVO_ERR_ARG, "split_lanes %d is not a power of two in 1..64", sl);
+    if (sets < 0 || sets == 1 || sets > kPreMaxSets)
+        return svo_fail(SVO_ERR_ARG, "pretrace sets %d: 0 (off) or 2..%d", sets, kPreMaxSets);
+    if (long_trips < 1 || long_trips > SVO_MAX_RAY_ITERS)
+        return svo_fail(SVO_ERR_ARG, "long_trips %d outside 1..%d", long_trips, (int)SVO_MAX_RAY_ITERS);
+    HIPCHECK(hipSetDevice(c->device));
+    if (sets != c->pre_sets) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        free_pre(c);
+        c->pre_sets = sets;
+    }
+    c->pre_trips = (uint32_t)long_trips;
+    c->pre_wg = wg_per_cu;
+    c->pre_split_trips = cfg->split_trips;
+    c->pre_split_shift = 31 - __builtin_clz((unsigned)sl);
+    c->pre_only_queued = cfg->only_queued != 0;
+    return SVO_OK;
+}
+
+extern "C" int svo_get_pretrace(svo_ctx* c, svo_pretrace_config* cfg)
+{
+    if (!c || !cfg) return svo_fail(SVO_ERR_ARG, "null argument");
+    cfg->sets = c->pre_sets;
+    cfg->long_trips = (int)c->pre_trips;
+    cfg->wg_per_cu = c->pre_wg;
+    cfg->split_trips = c->pre_split_trips;
+    cfg->split_lanes = 1 << c->pre_split_shift;
+    cfg->only_queued = c->pre_only_queued ? 1 : 0;
+    return SVO_OK;
+}
+
+extern "C" int svo_pretrace_stats(svo_ctx* c, uint64_t* rays, uint64_t* merged, uint32_t* max_trips, int reset)
+{
+    if (!c || !rays || !merged || !max_trips) return svo_fail(SVO_ERR_ARG, "null argument");
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    uint32_t w[10];
+    HIPCHECK(hipMemcpy(w, c->d_queue + 12, 40, hipMemcpyDeviceToHost));
+    memcpy(merged, w, 8);
+    memcpy(rays, w + 4, 8);
+    *max_trips = w[6];
+    if (reset) HIPCHECK(hipMemset(c->d_queue + 12, 0, 40));
     return SVO_OK;
 }
 
@@ -2190,19 +2616,46 @@ extern "C" int svo_hole_count(svo_ctx* c, uint32_t* holes)
 
 extern "C" int svo_ray_count(svo_ctx* c, uint64_t* rays, int reset)
 {
-    if (!c || !rays) return svo_fail(SVO_ERR_ARG, "null argument");
+    if (!rays) return svo_fail(SVO_ERR_ARG, "null argument");
+    uint32_t longest = 0;
+    return svo_ray_stats(c, rays, &longest, reset);
+}
+
+extern "C" int svo_ray_stats(svo_ctx* c, uint64_t* rays, uint32_t* max_trips, int reset)
+{
+    if (!c || !rays || !max_trips) return svo_fail(SVO_ERR_ARG, "null argument");
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    HIPCHECK(hipMemcpy(rays, c->d_queue + 8, 8, hipMemcpyDeviceToHost));
-    if (reset) HIPCHECK(hipMemset(c->d_queue + 8, 0, 8));
+    uint32_t w[3];
+    HIPCHECK(hipMemcpy(w, c->d_queue + 8, 12, hipMemcpyDeviceToHost));
+    memcpy(rays, w, 8);
+    *max_trips = w[2];
+    if (reset) HIPCHECK(hipMemset(c->d_queue + 8, 0, 12));
     return SVO_OK;
 }
 
 extern "C" int svo_enable_timing(svo_ctx* c, int mode)
 {
     if (!c) return svo_fail(SVO_ERR_ARG, "null context");
-    if (mode < 0 || mode > 2) return svo_fail(SVO_ERR_ARG, "timing mode %d outside 0..2", mode);
+    if (mode < 0 || mode > 3) return svo_fail(SVO_ERR_ARG, "timing mode %d outside 0..3", mode);
+    if (mode == 3) {
+        HIPCHECK(hipSetDevice(c->device));
+        for (auto& e : c->end_ring)
+            if (!e) HIPCHECK(hipEventCreate(&e));
+        c->end_count = 0;
+    }
     c->timing = mode;
+    return SVO_OK;
+}
+
+extern "C" int svo_frame_intervals(svo_ctx* c, float* ms, int max_n, int* n)
+{
+    if (!c || !n || (max_n > 0 && !ms) || max_n < 0) return svo_fail(SVO_ERR_ARG, "bad argument");
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    const int k = std::min(std::max(c->end_count - 1, 0), max_n);
+    for (int i = 0; i < k; i++) HIPCHECK(hipEventElapsedTime(&ms[i], c->end_ring[i], c->end_ring[i + 1]));
+    *n = k;
     return SVO_OK;
 }
 

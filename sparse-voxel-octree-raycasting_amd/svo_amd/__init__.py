@@ -89,6 +89,13 @@ _f = {
     "sync": _sig("svo_sync", C.c_int, _ctxp),
     "hole_count": _sig("svo_hole_count", C.c_int, _ctxp, _u32p),
     "ray_count": _sig("svo_ray_count", C.c_int, _ctxp, C.POINTER(C.c_uint64), C.c_int),
+    "ray_stats": _sig("svo_ray_stats", C.c_int, _ctxp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.c_int),
+    "set_pretrace": _sig("svo_set_pretrace", C.c_int, _ctxp, C.c_void_p),
+    "get_pretrace": _sig("svo_get_pretrace", C.c_int, _ctxp, C.c_void_p),
+    "frame_intervals": _sig("svo_frame_intervals", C.c_int, _ctxp, C.POINTER(C.c_float), C.c_int,
+                            C.POINTER(C.c_int)),
+    "pretrace_stats": _sig("svo_pretrace_stats", C.c_int, _ctxp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                           C.POINTER(C.c_uint32), C.c_int),
     "enable_timing": _sig("svo_enable_timing", C.c_int, _ctxp, C.c_int),
     "stage_times": _sig("svo_stage_times", C.c_int, _ctxp, C.POINTER(C.c_float), C.c_int),
     "stage_name": _sig("svo_stage_name", C.c_char_p, C.c_int),
@@ -167,8 +174,15 @@ EXPORTED = [
     "svo_gate_counts", "svo_batch_frame_warp_learned", "svo_scene_generate_gpu",
     "svo_set_ray_lanes", "svo_set_long_first", "svo_set_ray_block", "svo_raycast_quads", "svo_fillhole_punch",
     "svo_image_save_png", "svo_set_cnn_overlap", "svo_read_image_async", "svo_host_alloc", "svo_host_free",
-    "svo_proj_noalloc", "svo_set_gate_depth",
+    "svo_proj_noalloc", "svo_set_gate_depth", "svo_ray_stats", "svo_set_pretrace", "svo_pretrace_stats",
+    "svo_get_pretrace", "svo_frame_intervals",
 ]
+
+
+class PretraceConfig(C.Structure):
+    """svo_pretrace_config"""
+    _fields_ = [("sets", C.c_int), ("long_trips", C.c_int), ("wg_per_cu", C.c_int), ("split_trips", C.c_int),
+                ("split_lanes", C.c_int), ("only_queued", C.c_int)]
 
 
 def _take_u32(p, n):
@@ -512,9 +526,50 @@ class Context:
         _check(_f["ray_count"](self._h, C.byref(v), 1 if reset else 0))
         return v.value
 
+    def set_pretrace(self, sets=None, long_trips=None, wg_per_cu=None, split_trips=None, split_lanes=None,
+                     only_queued=None):
+        """Trace each warp frame's refresh rectangle and long-ray blocks
+        ahead on streams of their own (sets = 0: off); arguments left None
+        keep their current values.  Results are the same."""
+        cfg = self.get_pretrace()
+        for k, v in (("sets", sets), ("long_trips", long_trips), ("wg_per_cu", wg_per_cu),
+                     ("split_trips", split_trips), ("split_lanes", split_lanes), ("only_queued", only_queued)):
+            if v is not None:
+                setattr(cfg, k, int(v))
+        _check(_f["set_pretrace"](self._h, C.byref(cfg)))
+
+    def get_pretrace(self):
+        cfg = PretraceConfig()
+        _check(_f["get_pretrace"](self._h, C.byref(cfg)))
+        return cfg
+
+    def pretrace_stats(self, reset=False):
+        """(rays traced ahead, frame rays that took their result, loop trips
+        of the longest ray traced ahead) since the last reset."""
+        r = C.c_uint64()
+        m = C.c_uint64()
+        t = C.c_uint32()
+        _check(_f["pretrace_stats"](self._h, C.byref(r), C.byref(m), C.byref(t), 1 if reset else 0))
+        return r.value, m.value, t.value
+
+    def ray_stats(self, reset=False):
+        """(rays traced, loop trips of the longest of them) since the last
+        reset."""
+        v = C.c_uint64()
+        m = C.c_uint32()
+        _check(_f["ray_stats"](self._h, C.byref(v), C.byref(m), 1 if reset else 0))
+        return v.value, m.value
+
     def enable_timing(self, on=True, mode=None):
         """on: per-stage timing (mode 1); mode=2: one event pair per frame."""
         _check(_f["enable_timing"](self._h, int(mode) if mode is not None else (1 if on else 0)))
+
+    def frame_intervals(self, max_n=511):
+        """Timing mode 3: ms between the completions of successive frames."""
+        buf = (C.c_float * max(1, max_n))()
+        n = C.c_int()
+        _check(_f["frame_intervals"](self._h, buf, int(max_n), C.byref(n)))
+        return np.array(buf[:n.value], dtype=np.float64)
 
     def frame_time(self):
         """GPU ms of the last frame (timing mode 1 or 2, after sync())."""

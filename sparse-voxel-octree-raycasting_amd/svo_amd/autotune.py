@@ -10,12 +10,14 @@ top levels kept in LDS) and the shape of the blocks in which rectangle rays
 are handed out (svo_set_ray_block); full frames also search the trip count
 that puts a screen block on the long-rays-first list (svo_set_long_first),
 and learned frames the CNN's workgroups alongside the ray launch
-(svo_set_cnn_overlap).  Every candidate replays the same frame window from the same starting
-state (the window starts with the two clearing frames), so candidates see
-identical work; the score is the mean GPU time per frame over the timed
-frames (frames/s follows the mean), measured with the library's per-frame
-HIP-event pair.  Results do not
-depend on the knobs (the tests check bit-exactness under several settings).
+(svo_set_cnn_overlap); warp frames also search the pretrace's result sets
+and block threshold (svo_set_pretrace).  Every candidate replays the same
+frame window from the same starting state (the window starts with the two
+clearing frames), so candidates see identical work; the frames are issued
+back to back and the score is the mean time between the completions of
+successive timed frames (timing mode 3: the frame period the GPU sustains,
+which frames/s follows).  Results do not depend on the knobs (the tests
+check bit-exactness under several settings).
 """
 from __future__ import annotations
 
@@ -49,14 +51,12 @@ def _frame_time(ctx, params, warmup, mode, cnn=None, max_cnn_holes=64):
     for p in params[:warmup]:
         frame(p)
     ctx.sync()
-    ctx.enable_timing(mode=2)  # event pair per frame, graph launches included
-    ts = []
+    ctx.enable_timing(mode=3)  # completion events, frames back to back
     for p in params[warmup:]:
         frame(p)
-        ctx.sync()
-        ts.append(ctx.frame_time())
+    ts = ctx.frame_intervals()
     ctx.enable_timing(False)
-    return statistics.mean(ts)  # frames/s is set by the mean
+    return float(statistics.mean(ts))  # frames/s is set by the mean
 
 
 def _search(space, coordinate, measure):
@@ -83,6 +83,8 @@ def _search(space, coordinate, measure):
     return best, best_ms
 
 
+# warp frames also search the pretrace's sets and block threshold
+WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[6, 4, 8], pre_long=[256, 128, 384])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
@@ -102,6 +104,8 @@ def apply(ctx, cfg):
         ctx.set_ray_block(int(cfg["ray_block"]))
     if "cnn_wgs" in cfg:
         ctx.set_cnn_overlap(int(cfg["cnn_wgs"]))
+    if "pre_sets" in cfg or "pre_long" in cfg:
+        ctx.set_pretrace(sets=cfg.get("pre_sets"), long_trips=cfg.get("pre_long"))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None, cnn=None, max_cnn_holes=64):
@@ -113,7 +117,7 @@ def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=No
     two sweeps); otherwise the full grid is measured.  Applies and returns
     (best_setting, best_ms, trials)."""
     if space is None:
-        space = {"full": FULL_SPACE, "learned": LEARNED_SPACE}.get(mode, DEFAULT_SPACE)
+        space = {"full": FULL_SPACE, "learned": LEARNED_SPACE, "warp": WARP_SPACE}.get(mode, DEFAULT_SPACE)
     space = dict(space)
     trials = []
 

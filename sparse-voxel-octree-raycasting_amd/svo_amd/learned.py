@@ -67,12 +67,15 @@ def _path(name, seed_off):
                        w["pitch"])
 
 
-def evaluate(cnn, name, seed_off=1000, n_frames=40, skip=8, max_cnn_holes=64, gate_depth=0.0):
+def evaluate(cnn, name, seed_off=1000, n_frames=40, skip=8, max_cnn_holes=64, gate_depth=0.0, detail=False):
     """PSNR (dB, vs the full raycast) of the plain warp (fillhole2), the gate
     alone, and the gate + CNN, over n_frames of a held-out path; mean over the
-    frames after the first `skip`, on all pixels and on the hole pixels.
+    frames after the first `skip` that differ from the full raycast, on all
+    pixels and on the hole pixels (NaN when every frame is identical).
     gate_depth: the gate's depth rule (svo_set_gate_depth) for both gated
-    modes."""
+    modes.  detail=True adds a third dict per mode: the PSNR of the mean
+    squared error over all those frames ("seq_db", None when it is zero) and
+    the number of frames identical to the full raycast."""
     words, root, depth, _ = W.octree(name)
     rx, ry = W.WORKLOADS[name]["res"]
     rows = _path(name, seed_off)
@@ -82,8 +85,12 @@ def evaluate(cnn, name, seed_off=1000, n_frames=40, skip=8, max_cnn_holes=64, ga
         c.resize(rx, ry)
     ctx["gate"].set_gate_depth(gate_depth)
     ctx["learned"].set_gate_depth(gate_depth)
-    res = {k: [] for k in ("plain", "gate", "learned")}
-    hole_res = {k: [] for k in ("plain", "gate", "learned")}
+    modes = ("plain", "gate", "learned")
+    res = {k: [] for k in modes}
+    hole_res = {k: [] for k in modes}
+    sq = {k: 0.0 for k in modes}
+    same = {k: 0 for k in modes}
+    n_eval = 0
     for f in range(n_frames):
         p = W.frame_params(name, rows, f % len(rows), frame_index=f)
         ctx["plain"].frame_warp(p)
@@ -92,17 +99,30 @@ def evaluate(cnn, name, seed_off=1000, n_frames=40, skip=8, max_cnn_holes=64, ga
         ctx["full"].frame_full(p)
         if f < skip:
             continue
+        n_eval += 1
         ref = image_rgb(ctx["full"].image())
         holes = ctx["learned"].screen(2) == 0xFFFFFF00
         for k in res:
             img = image_rgb(ctx[k].image())
+            d = img.astype(np.float64) - ref.astype(np.float64)
+            sq[k] += float(np.mean(d * d))
+            same[k] += int(not d.any())
             res[k].append(psnr(img, ref))
             if holes.any():
                 hole_res[k].append(psnr(img[holes], ref[holes]))
     for c in ctx.values():
         c.close()
-    fin = lambda v: float(np.mean([x for x in v if np.isfinite(x)])) if v else float("nan")
-    return {k: fin(v) for k, v in res.items()}, {k: fin(v) for k, v in hole_res.items()}
+
+    def fin(v):
+        v = [x for x in v if np.isfinite(x)]
+        return float(np.mean(v)) if v else float("nan")
+
+    out = ({k: fin(v) for k, v in res.items()}, {k: fin(v) for k, v in hole_res.items()})
+    if not detail:
+        return out
+    seq = {k: (10.0 * np.log10(255.0 ** 2 / (sq[k] / max(1, n_eval))) if sq[k] > 0 else None) for k in modes}
+    return out + ({k: {"seq_db": None if seq[k] is None else float(seq[k]), "identical_frames": same[k],
+                       "frames": n_eval} for k in modes},)
 
 
 def save(cnn, path=WEIGHTS, **meta):

@@ -43,7 +43,7 @@ def voxels(name):
     the host generator needs hours at depth 12; both give the same stream)."""
     w = WORKLOADS[name]
     if w["kind"] in (3, 4):
-        return scene_gpu(3, w["seed"], w["depth"])
+        return scene_gpu(w["kind"], w["seed"], w["depth"])
     return scene(w["kind"], w["seed"], w["depth"], w["spheres"])
 
 

@@ -33,6 +33,11 @@ def test_full_space_adds_long_first():
     assert all(k in A.FULL_SPACE for k in A.DEFAULT_SPACE)
 
 
+def test_warp_space_adds_pretrace():
+    assert "pre_sets" in A.WARP_SPACE and "pre_long" in A.WARP_SPACE and "pre_sets" not in A.DEFAULT_SPACE
+    assert all(k in A.WARP_SPACE for k in A.DEFAULT_SPACE)
+
+
 def test_learned_space_adds_cnn_workgroups():
     assert "cnn_wgs" in A.LEARNED_SPACE and "cnn_wgs" not in A.DEFAULT_SPACE
     assert all(k in A.LEARNED_SPACE for k in A.DEFAULT_SPACE)
@@ -49,6 +54,9 @@ def test_tuned_context_stays_bitexact(mode):
     space = {"spread": [1, 64], "refill": [8, 32], "wg_per_cu": [4], "iters": [16, 64], "lds_cache_words": [0, 736]}
     if mode == "full":
         space["long_trips"] = [0, 16]
+    else:
+        space["pre_sets"] = [6, 3]
+        space["pre_long"] = [256, 64]
     c = S.Context(0)
     o = O.Frame(rx, ry, words, root, 8, nthreads=8)
     try:

@@ -43,7 +43,8 @@ def test_workload_frames_bitexact(big):
 def test_workload_properties(big):
     """Over 40 frames: the id list is the ascending-block-order list of hole
     quads of plane 0 (4 ids per quad, counts = 4 x quads), and the frame's
-    rays = holes + tile pixels."""
+    rays (traced in the frame or taken from its pretrace) = holes + tile
+    pixels."""
     words, root, depth = big
     rx, ry = W.WORKLOADS[NAME]["res"]
     rows = W.path(NAME, 40)
@@ -54,6 +55,7 @@ def test_workload_properties(big):
         for f in range(40):
             p = W.frame_params(NAME, rows, f)
             c.ray_count(reset=True)
+            c.pretrace_stats(reset=True)
             c.frame_warp(p)
             c.sync()
             if f % 8 != 5:
@@ -61,7 +63,7 @@ def test_workload_properties(big):
             ids = c.ids()
             H = int(ids[0])
             _, _, ex, ey = S.tile_schedule(rx, ry, f)
-            assert c.ray_count() == H + ex * ey
+            assert c.ray_count() + c.pretrace_stats()[1] == H + ex * ey
             offs = ids[S_:2 * S_]
             assert offs[0] == 0 and np.all(np.diff(offs.astype(np.int64)) >= 0)
             assert H % 4 == 0 and H <= rx * ry
@@ -117,3 +119,36 @@ def test_maximum_resolution_sampled_vs_oracle(big, res):
                 sel = ys.astype(np.int64) * rx + xs
                 assert np.array_equal(a[sel], b[sel])
                 assert int(c.ids()[0]) == int(d.ids()[0])
+
+
+def test_ray_stats_match_the_oracle(big):
+    """svo_ray_stats: per frame, the rays traced and the loop trips of the
+    longest of them equal the oracle's count of rays and its longest ray
+    (warp frames after the clears, then a full frame; no pretrace, so every
+    ray of the frame is traced in it)."""
+    words, root, depth = big
+    rx, ry = W.WORKLOADS[NAME]["res"]
+    rows = W.path(NAME, 10)
+    o = O.Frame(rx, ry, words, root, depth, nthreads=16)
+    try:
+        with S.Context(0) as c:
+            c.upload_octree(words, root, depth)
+            c.resize(rx, ry)
+            c.set_pretrace(0)
+            for f in range(9):
+                p = W.frame_params(NAME, rows, f)
+                print(f"ray stats frame {f}", flush=True)
+                c.ray_stats(reset=True)
+                o.s.st = O.Stats()
+                if f < 8:
+                    c.frame_warp(p)
+                    o.frame_warp(O.params_from(p))
+                else:
+                    c.frame_full(p)
+                    o.frame_full(O.params_from(p))
+                rays, longest = c.ray_stats()
+                st = o.stats
+                assert (rays, longest) == (st["rays"], st["max_iters"]), f"frame {f}"
+                assert longest > 100
+    finally:
+        o.close()

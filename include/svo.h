@@ -328,7 +328,8 @@ int svo_set_ray_block(svo_ctx* ctx, int width);
  * 16x16-workgroup grid over its rectangle (raycast.h:361-366), so no
  * counterpart.  Default 1024; results do not depend on it. */
 int svo_set_long_first(svo_ctx* ctx, int long_trips);
-/* Warp frames (svo_frame_warp, frames >= 2, timing mode 0 or 2): each frame's
+/* Warp and learned frames (svo_frame_warp, svo_frame_warp_learned; frames
+ * >= 2, timing mode 0, 2 or 3): each frame's
  * refresh rectangle and every pixel of the 8x8 blocks whose last traced rays
  * include one of at least long_trips loop trips are traced ahead, into one
  * of `sets` result sets; each set has a stream of its own and is reused every

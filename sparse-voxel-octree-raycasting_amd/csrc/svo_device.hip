@@ -1601,6 +1601,8 @@ static int warp_compact(svo_ctx* c, const svo_frame_params* p)
     return launch_write(L, v);
 }
 
+static int wait_pretrace(svo_ctx* c);
+
 // the warp stages of raycast_draw up to and including the hole and tile rays
 static int warp_front(svo_ctx* c, const svo_frame_params* p)
 {
@@ -1608,6 +1610,8 @@ static int warp_front(svo_ctx* c, const svo_frame_params* p)
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
     mark(c, ST_RAYS);
+    // the frame's projection and compaction overlap the end of its pretrace
+    RC(wait_pretrace(c));
     return launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
 }
 
@@ -1876,8 +1880,7 @@ static int pretrace(svo_ctx* c, const svo_frame_params* p)
     }
     RC(launch_rays_v(c, st, v, nullptr, nullptr, 1, c->d_pre_queue + 4 * s, c->pre_seq[s], c->d_queue + 16, 6,
                      c->pre_wg, split));
-    HIPCHECK(hipEventRecord(c->ev_pre[s], st));
-    HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[s], 0));
+    HIPCHECK(hipEventRecord(c->ev_pre[s], st));  // the frame's ray launch waits for it
     c->pre_frames++;
     c->pre_active = true;
     c->pre_set = s;
@@ -1888,9 +1891,28 @@ static int pretrace(svo_ctx* c, const svo_frame_params* p)
 // after the frame's work is queued: its set may be reused once it completes
 static void pretrace_done(svo_ctx* c)
 {
+    if (hipEventRecord(c->ev_frame_done, c->stream) == hipSuccess) c->frame_done_recorded = true;
     if (!c->pre_active) return;
     c->pre_active = false;
     if (hipEventRecord(c->ev_main[c->pre_set], c->stream) == hipSuccess) c->main_recorded[c->pre_set] = true;
+}
+
+// whether a warp or learned frame gets a pretrace: frames >= 2 outside stage
+// timing; with only_queued, only while the frame before it is still on the
+// GPU (frames issued ahead of it) -- a frame the GPU starts right away has
+// no earlier frame to overlap
+static bool want_pretrace(svo_ctx* c, const svo_frame_params* p)
+{
+    if (c->pre_sets <= 0 || c->timing == 1 || p->frame < 2) return false;
+    if (!c->pre_only_queued) return true;
+    return c->frame_done_recorded && hipEventQuery(c->ev_frame_done) == hipErrorNotReady;
+}
+
+// the frame's ray launch copies its pretrace's results: it waits for them
+static int wait_pretrace(svo_ctx* c)
+{
+    if (c->pre_active) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[c->pre_set], 0));
+    return SVO_OK;
 }
 
 extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
@@ -1902,12 +1924,7 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     // a pretrace frame runs direct launches: a graph launch here waits on the
     // host for the GPU, which would keep the host from issuing the next
     // frames' pretraces ahead
-    // with only_queued, a frame is pretraced only while the frame before it
-    // is still on the GPU (frames issued ahead of it): a frame the GPU
-    // starts right away has no earlier frame to overlap
-    bool pre = c->pre_sets > 0 && c->timing != 1 && p->frame >= 2;
-    if (pre && c->pre_only_queued)
-        pre = c->frame_done_recorded && hipEventQuery(c->ev_frame_done) == hipErrorNotReady;
+    const bool pre = want_pretrace(c, p);
     const bool graph = c->use_graph && c->timing != 1 && p->frame >= 2 && !pre;
     if (graph && !c->graph[0]) RC(build_graph(c));
     frame_begin(c);
@@ -1926,7 +1943,6 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     }
     pretrace_done(c);
     RC(rc);
-    if (hipEventRecord(c->ev_frame_done, c->stream) == hipSuccess) c->frame_done_recorded = true;
     return frame_end(c);
 }
 
@@ -1954,6 +1970,8 @@ static int ensure_cnn_buffers(svo_ctx* c)
 // replaces the filtered colour of their hole pixels in the image.  Screen
 // words keep the reference filter's result.  cnn == NULL keeps the filter's
 // colours (gate only).
+static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
+
 extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
 {
     RC(need_octree(c));
@@ -1963,10 +1981,20 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     RC(ensure_cnn_buffers(c));
     take_image(c);
     frame_begin(c);
+    if (want_pretrace(c, p)) RC(pretrace(c, p));
+    const int rc = learned_body(c, cnn, p, max_cnn_holes);
+    pretrace_done(c);
+    RC(rc);
+    return frame_end(c);
+}
+
+// the learned frame after its pretrace (if any) is issued
+static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
+{
     RC(warp_compact(c, p));
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
-    View v = make_view(c, p);
+    View v = make_view(c, p);  // with the pretrace's set when the frame has one
     set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
     v.list2 = c->d_gate_rays;
     v.list2_count = c->d_gate + GATE_RAYS;
@@ -1994,6 +2022,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     }
     // hole list, gate rays and the refresh tile in one ray launch
     mark(c, ST_RAYS);
+    RC(wait_pretrace(c));
     RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
     const Launch L = single(c);
     mark(c, ST_COPY);
@@ -2009,7 +2038,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
         mark(c, ST_CNN_APPLY);
         RC(cnn_apply_tiles(c, c->d_gate_tiles, c->d_gate, tiles, c->stream));
     }
-    return frame_end(c);
+    return SVO_OK;
 }
 
 extern "C" int svo_frame_full(svo_ctx* c, const svo_frame_params* p)

@@ -10,8 +10,8 @@ top levels kept in LDS) and the shape of the blocks in which rectangle rays
 are handed out (svo_set_ray_block); full frames also search the trip count
 that puts a screen block on the long-rays-first list (svo_set_long_first),
 and learned frames the CNN's workgroups alongside the ray launch
-(svo_set_cnn_overlap); warp frames also search the pretrace's result sets
-and block threshold (svo_set_pretrace).  Every candidate replays the same
+(svo_set_cnn_overlap); warp and learned frames also search the pretrace's
+result sets and block threshold (svo_set_pretrace).  Every candidate replays the same
 frame window from the same starting state (the window starts with the two
 clearing frames), so candidates see identical work; the frames are issued
 back to back and the score is the mean time between the completions of
@@ -88,7 +88,8 @@ WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[6, 4, 8], pre_long=[256, 128, 384])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
-LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512])
+LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[6, 4, 8],
+                     pre_long=[256, 128, 384])
 
 
 def apply(ctx, cfg):

@@ -206,6 +206,41 @@ def test_pretrace_only_for_queued_frames():
         b.close()
 
 
+@pytest.mark.parametrize("max_cnn_holes", [64, 0])
+def test_pretrace_learned_frames_equal_plain_learned_frames(scene1, max_cnn_holes):
+    """Learned frames (gate + CNN) with a pretrace equal learned frames
+    without one: the gate's rays, the holes and the tile take the traced-ahead
+    results; the CNN's input is plane 0 before any ray."""
+    from svo_amd import learned as L
+    words, root = scene1
+    rx, ry = CONFIG1["res"]
+    pr = path(CONFIG1, 14)
+    cnn = S.Cnn(0)
+    L.load(cnn, L.WEIGHTS)
+    a, b = S.Context(0), S.Context(0)
+    try:
+        for c in (a, b):
+            c.upload_octree(words, root, CONFIG1["depth"])
+            c.resize(rx, ry)
+        a.set_pretrace(3, 64, only_queued=0)
+        b.set_pretrace(0)
+        for f in range(14):
+            print(f"pretrace learned frame {f} max_cnn_holes {max_cnn_holes}", flush=True)
+            p = params(CONFIG1, pr, f)
+            a.pretrace_stats(reset=True)
+            a.frame_warp_learned(cnn, p, max_cnn_holes)
+            b.frame_warp_learned(cnn, p, max_cnn_holes)
+            a.sync()
+            if f >= 2:
+                assert a.pretrace_stats()[1] > 0
+            if f % 3 == 1 or f == 13:
+                _same(a, b, f"learned frame {f}")
+    finally:
+        a.close()
+        b.close()
+        cnn.close()
+
+
 def test_pretrace_rejects_bad_settings(scene1):
     with S.Context(0) as c:
         for args in [(1, 512), (9, 512), (-1, 512), (3, 0), (3, (1 << 22) + 1), (3, 512, 33), (3, 512, -1),

@@ -345,13 +345,28 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * GPU (the caller issues frames ahead): a frame the GPU starts at once has
  * no earlier frame to overlap, and its own rays finish sooner without one.
  * Defaults: 6 sets, long_trips 256, 1 workgroup per CU, no split,
- * only_queued 1.  A primary ray depends
+ * only_queued 1, predict 0, whole_trips 0.  A primary ray depends
  * only on the camera, the pixel and the octree (kernel.cl:594-694,
  * :846-942): results do not depend on these settings.  HIP maps streams onto
  * at most GPU_MAX_HW_QUEUES hardware queues (4 unless the environment sets
  * it before the first HIP call); pretraces whose streams share one run one
- * after the other.  The reference traces each frame's rays in the frame
- * (raycast.h:296-387), so no counterpart. */
+ * after the other.
+ * predict = k (1..3) replaces the long blocks by predicted holes: every warp
+ * or learned frame also stores its final plane-0 positions (after its rays,
+ * holes marked) in a ring of 4 snapshots, and frame g's pretrace, once frame
+ * g-k has completed, projects frame g-k's snapshot through the cameras of
+ * frames g-k+1 .. g-1 (keeping the nearest sample per pixel, as the warp
+ * does) and then g, and traces the 2x2 quads no sample reaches -- the hole
+ * quads frame g's warp can leave -- in the 8x8 blocks whose last traced rays
+ * include one of at least long_trips trips, with the refresh rectangle.  A
+ * frame whose frames g-k .. g-1 did not all pass through svo_frame_warp or
+ * svo_frame_warp_learned (consecutive frame counters) falls back to the long
+ * blocks.  With predict > 0 and whole_trips > 0, the 8x8 blocks whose last
+ * traced rays include one of at least whole_trips trips are traced whole in
+ * a second launch of the set that starts as soon as the frame is issued
+ * (their rays are the longest; a prediction would start them too late), and
+ * the predicted holes skip them.  The reference traces each frame's rays in
+ * the frame (raycast.h:296-387), so no counterpart. */
 typedef struct {
     int sets;         /* 0 = off, 2..8 */
     int long_trips;   /* >= 1 */
@@ -359,6 +374,8 @@ typedef struct {
     int split_trips;  /* 0 = no split */
     int split_lanes;  /* power of two, 1..64 */
     int only_queued;  /* 1: pretrace a frame only while the warp frame before it is still running */
+    int predict;      /* 0: long blocks whole; 1..3: holes predicted from the frame that many frames back */
+    int whole_trips;  /* predict > 0: blocks traced whole at once (0 = none) */
 } svo_pretrace_config;
 int svo_set_pretrace(svo_ctx* ctx, const svo_pretrace_config* cfg);
 int svo_get_pretrace(svo_ctx* ctx, svo_pretrace_config* cfg);

@@ -22,6 +22,7 @@ constexpr int kViewSlots = 8;   // pinned staging slots of the graph frame's vie
 // use different instances and the host can run frames ahead
 constexpr int kGraphExecs = 4;
 constexpr int kEndRing = 512;  // timing mode 3: completion events of the first frames
+constexpr int kSnapSlots = 4;  // predicted-hole pretrace: final plane-0 snapshots of recent frames
 
 struct svo_ctx {
     int device = 0;
@@ -106,22 +107,44 @@ struct svo_ctx {
     bool pre_only_queued = true;   // pretrace a frame only while the one before it is still running
     hipEvent_t ev_frame_done = nullptr;  // the last warp frame's end
     bool frame_done_recorded = false;
-    hipStream_t pre_streams[kPreMaxSets] = {};  // created on first use
+    // set s runs part 1 (the long blocks, or the predicted holes) on
+    // pre_streams[2s+1] and, with predicted holes and pre_whole_trips > 0,
+    // part 0 (the longest blocks whole, issued without waiting for a
+    // snapshot) on pre_streams[2s]; both write the set's results
+    hipStream_t pre_streams[2 * kPreMaxSets] = {};  // created on first use
     uint32_t* d_known = nullptr;      // N: loop trips of the last ray traced per pixel
     uint32_t* d_pre_stamp = nullptr;  // pre_sets x N
     uint32_t* d_pre_word = nullptr;   // pre_sets x N
     float4* d_pre_xyz = nullptr;      // pre_sets x N
-    uint32_t* d_pre_list = nullptr;   // per set: list header and pixel words (x | y << 16)
-    uint32_t* d_pre_blk = nullptr;    // per set: one word per 8x8 block
-    uint32_t* d_pre_queue = nullptr;  // per set: four ray-queue counters
-    uint8_t* d_pre_bkt = nullptr;     // per set: one byte per pixel
+    uint32_t* d_pre_list = nullptr;   // per set and part: list header and pixel words (x | y << 16)
+    uint32_t* d_pre_blk = nullptr;    // per set and part: one word per 8x8 block
+    uint32_t* d_pre_queue = nullptr;  // per set and part: four ray-queue counters
+    uint8_t* d_pre_bkt = nullptr;     // per set and part: one byte per pixel
     int pre_alloc_sets = 0;           // sets the buffers hold (0: not set up)
-    hipEvent_t ev_pre[kPreMaxSets] = {};   // a set's pretrace is complete
+    hipEvent_t ev_pre[2 * kPreMaxSets] = {};  // a set's pretrace part is complete
     hipEvent_t ev_main[kPreMaxSets] = {};  // a set's frame is complete
     bool main_recorded[kPreMaxSets] = {};
     uint32_t pre_frames = 0;  // frames pretraced since the sets were cleared; the stamp is this count
-    int pre_seq[kPreMaxSets] = {};  // ray-queue slot parity per set
+    int pre_seq[2 * kPreMaxSets] = {};  // ray-queue slot parity per set and part
     bool pre_active = false;  // the frame being issued has a pretrace in set pre_set, stamp pre_id
+    bool pre_early = false;   // ... with a part 0
+    // predicted-hole pretrace (pre_predict = k > 0): each warp/learned frame's
+    // copy stores its final plane-0 positions (holes as NaN) in snapshot slot
+    // frame % kSnapSlots; frame g's pretrace projects snapshot g-k through
+    // the cameras of frames g-k+1 .. g and traces the hole quads no sample
+    // reaches (and the refresh rectangle)
+    int pre_predict = 0;
+    uint32_t pre_whole_trips = 0;           // part 0's block threshold (0: no part 0)
+    float4* d_snap_xyz = nullptr;           // kSnapSlots x N
+    int snap_frame[kSnapSlots] = {-1, -1, -1, -1};  // frame counter each slot holds (-1: none)
+    hipEvent_t ev_snap[kSnapSlots] = {};    // the slot's frame has completed
+    svo_frame_params cam_hist[kSnapSlots] = {};  // cameras of recent frames, by frame counter
+    int cam_frame[kSnapSlots] = {-1, -1, -1, -1};
+    int snap_pending = -1;                  // slot written by the frame being issued (-1: none)
+    unsigned long long* d_pred_keys = nullptr;  // per set: N nearest-sample keys (all ones between uses)
+    float4* d_pred_xyz = nullptr;               // per set: N positions of an intermediate camera
+    uint8_t* d_pred_cov = nullptr;              // per set: N coverage bytes (zero between uses)
+    int pred_alloc_sets = 0;
     int pre_set = 0;
     uint32_t pre_id = 0;
     // per-frame graph of the warp frame (frames >= 2): its kernels read the

@@ -159,6 +159,7 @@ struct View {
     uint32_t* stamp_out;        // pretrace launches: stamp per traced pixel (or null)
     uint32_t pre_id;
     uint32_t* trips_out;        // per pixel: loop trips of the last ray traced there (or null)
+    float4* snap_out;           // warp frames' copy: final plane-0 positions, holes as NaN (or null)
 };
 
 constexpr uint32_t QUAD_NONE = 0xffffffffu;  // list entry without a ray
@@ -801,8 +802,11 @@ __global__ __launch_bounds__(256) void k_copy_plane(View v, const View* views, u
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= V.N) return;
     const size_t d = (size_t)dst_plane * V.N + i;
-    V.screen[d] = V.screen[i];
-    V.coords[d] = V.coords[i];
+    const uint32_t w = V.screen[i];
+    const float4 q = V.coords[i];
+    V.screen[d] = w;
+    V.coords[d] = q;
+    if (V.snap_out) V.snap_out[i] = w == HOLE ? make_float4(__int_as_float(0x7fc00000), 0.0f, 0.0f, 0.0f) : q;
 }
 
 // learned frames: the CNN's input, plane 0 as the gate leaves it (the rays
@@ -1181,7 +1185,8 @@ extern "C" int svo_create(svo_ctx** out, int device)
     for (int i = 0; ok && i <= ST_N; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&c->ev_frame_done, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; ok && i < kPreMaxSets; i++)
-        ok = hipEventCreateWithFlags(&c->ev_pre[i], hipEventDisableTiming) == hipSuccess &&
+        ok = hipEventCreateWithFlags(&c->ev_pre[2 * i], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_pre[2 * i + 1], hipEventDisableTiming) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_main[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         svo_destroy(c);
@@ -1217,15 +1222,17 @@ extern "C" int svo_destroy(svo_ctx* c)
     if (c->cnn_fork) (void)hipEventDestroy(c->cnn_fork);
     if (c->cnn_join) (void)hipEventDestroy(c->cnn_join);
     if (c->cnn_stream) (void)hipStreamDestroy(c->cnn_stream);
-    for (int i = 0; i < kPreMaxSets; i++) {
-        if (c->ev_pre[i]) (void)hipEventDestroy(c->ev_pre[i]);
-        if (c->ev_main[i]) (void)hipEventDestroy(c->ev_main[i]);
-    }
+    for (auto& e : c->ev_pre)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->ev_main)
+        if (e) (void)hipEventDestroy(e);
     for (auto& st : c->pre_streams)
         if (st) (void)hipStreamDestroy(st);
     for (auto& e : c->end_ring)
         if (e) (void)hipEventDestroy(e);
     if (c->ev_frame_done) (void)hipEventDestroy(c->ev_frame_done);
+    for (auto& e : c->ev_snap)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SVO_OK;
@@ -1654,10 +1661,11 @@ static int build_graph(svo_ctx* c)
     return rc;
 }
 
-static int frame_warp_graph(svo_ctx* c, const svo_frame_params* p)
+static int frame_warp_graph(svo_ctx* c, const svo_frame_params* p, float4* snap)
 {
     if (!c->graph[0]) RC(build_graph(c));
     View v = make_view(c, p);
+    v.snap_out = snap;
     v.plane_lo = c->plane1_holes ? 2 : 1;
     v.n_planes = 3 - v.plane_lo;
     int ax, ay, ex, ey;
@@ -1681,6 +1689,7 @@ static int frame_warp_graph(svo_ctx* c, const svo_frame_params* p)
 // bucket), four ray-queue counters.
 constexpr int kPreHdr = 64;
 constexpr int kPreBuckets = 24;  // bucket b: blocks whose longest known ray has floor(log2(trips)) = b
+constexpr uint8_t kPreNone = 0xff;  // k_pre_scan_pred: the pixel is not traced ahead
 
 static size_t pre_blocks(const svo_ctx* c) { return (size_t)((c->res_x + 7) / 8) * (size_t)((c->res_y + 7) / 8); }
 
@@ -1699,6 +1708,38 @@ static void free_pre(svo_ctx* c)
     c->pre_alloc_sets = 0;
     c->pre_frames = 0;
     for (auto& m : c->main_recorded) m = false;
+    (void)hipFree(c->d_snap_xyz);
+    (void)hipFree(c->d_pred_keys);
+    (void)hipFree(c->d_pred_xyz);
+    (void)hipFree(c->d_pred_cov);
+    c->d_snap_xyz = nullptr;
+    c->d_pred_keys = nullptr;
+    c->d_pred_xyz = nullptr;
+    c->d_pred_cov = nullptr;
+    c->pred_alloc_sets = 0;
+    for (int i = 0; i < kSnapSlots; i++) c->snap_frame[i] = c->cam_frame[i] = -1;
+    c->snap_pending = -1;
+}
+
+// the predicted-hole buffers: snapshots and per-set projection scratch
+static int ensure_pred(svo_ctx* c)
+{
+    if (c->pred_alloc_sets == c->pre_alloc_sets && c->d_snap_xyz) return SVO_OK;
+    const size_t K = (size_t)c->pre_alloc_sets, N = c->N;
+    for (int i = 0; i < kSnapSlots; i++)
+        if (!c->ev_snap[i]) HIPCHECK(hipEventCreateWithFlags(&c->ev_snap[i], hipEventDisableTiming));
+    if (hipMalloc(&c->d_snap_xyz, kSnapSlots * N * 16) != hipSuccess ||
+        hipMalloc(&c->d_pred_keys, K * N * 8) != hipSuccess || hipMalloc(&c->d_pred_xyz, K * N * 16) != hipSuccess ||
+        hipMalloc(&c->d_pred_cov, K * N) != hipSuccess) {
+        free_pre(c);
+        return svo_fail(SVO_ERR_NOMEM, "predicted-hole buffers for %zu sets of %zu pixels", K, N);
+    }
+    HIPCHECK(hipMemset(c->d_pred_keys, 0xff, K * N * 8));
+    HIPCHECK(hipMemset(c->d_pred_cov, 0, K * N));
+    HIPCHECK(hipDeviceSynchronize());
+    c->pred_alloc_sets = (int)K;
+    for (int i = 0; i < kSnapSlots; i++) c->snap_frame[i] = -1;
+    return SVO_OK;
 }
 
 static int ensure_pre(svo_ctx* c)
@@ -1707,22 +1748,54 @@ static int ensure_pre(svo_ctx* c)
     HIPCHECK(hipStreamSynchronize(c->stream));  // no frame reads the old sets
     free_pre(c);
     const size_t K = (size_t)c->pre_sets, n = K * c->N;
-    for (int i = 0; i < c->pre_sets; i++)
-        if (!c->pre_streams[i]) HIPCHECK(hipStreamCreateWithFlags(&c->pre_streams[i], hipStreamNonBlocking));
+    for (int i = 0; i < c->pre_sets; i++)  // part 1; part 0's stream is made when first used
+        if (!c->pre_streams[2 * i + 1])
+            HIPCHECK(hipStreamCreateWithFlags(&c->pre_streams[2 * i + 1], hipStreamNonBlocking));
     if (hipMalloc(&c->d_pre_stamp, n * 4) != hipSuccess || hipMalloc(&c->d_pre_word, n * 4) != hipSuccess ||
         hipMalloc(&c->d_pre_xyz, n * 16) != hipSuccess ||
-        hipMalloc(&c->d_pre_list, K * (c->N + kPreHdr) * 4) != hipSuccess ||
-        hipMalloc(&c->d_pre_blk, K * pre_blocks(c) * 4) != hipSuccess ||
-        hipMalloc(&c->d_pre_queue, K * 16) != hipSuccess || hipMalloc(&c->d_pre_bkt, n) != hipSuccess) {
+        hipMalloc(&c->d_pre_list, 2 * K * (c->N + kPreHdr) * 4) != hipSuccess ||
+        hipMalloc(&c->d_pre_blk, 2 * K * pre_blocks(c) * 4) != hipSuccess ||
+        hipMalloc(&c->d_pre_queue, 2 * K * 16) != hipSuccess || hipMalloc(&c->d_pre_bkt, 2 * n) != hipSuccess) {
         free_pre(c);
         return svo_fail(SVO_ERR_NOMEM, "%d pretrace sets of %zu pixels", c->pre_sets, c->N);
     }
     HIPCHECK(hipMemset(c->d_pre_stamp, 0, n * 4));  // stamps start at 1
-    HIPCHECK(hipMemset(c->d_pre_queue, 0, K * 16));
+    HIPCHECK(hipMemset(c->d_pre_queue, 0, 2 * K * 16));
     HIPCHECK(hipDeviceSynchronize());
     c->pre_alloc_sets = c->pre_sets;
     for (auto& q : c->pre_seq) q = 0;
     return SVO_OK;
+}
+
+// A warp or learned frame being issued: record its camera; with predicted
+// holes on, point its copy at snapshot slot frame % kSnapSlots (the slot's
+// last reader, a pretrace of a frame before this one, has completed: the
+// frames in between waited for their pretraces).  Returns the snapshot
+// buffer or null.
+static float4* snap_begin(svo_ctx* c, const svo_frame_params* p)
+{
+    c->snap_pending = -1;
+    if (c->pre_predict <= 0 || c->pre_sets <= 0 || p->frame < 0) return nullptr;
+    if (ensure_pre(c) || ensure_pred(c)) {
+        c->pre_predict = 0;  // out of memory: long blocks instead (svo_last_error says why)
+        return nullptr;
+    }
+    const int slot = p->frame % kSnapSlots;
+    c->cam_hist[slot] = *p;
+    c->cam_frame[slot] = p->frame;
+    c->snap_frame[slot] = -1;  // being rewritten
+    c->snap_pending = slot;
+    return c->d_snap_xyz + (size_t)slot * c->N;
+}
+
+// after the frame's launches are queued: its snapshot is complete when the
+// frame is
+static void snap_end(svo_ctx* c, const svo_frame_params* p)
+{
+    const int slot = c->snap_pending;
+    c->snap_pending = -1;
+    if (slot < 0) return;
+    if (hipEventRecord(c->ev_snap[slot], c->stream) == hipSuccess) c->snap_frame[slot] = p->frame;
 }
 
 __device__ __forceinline__ uint32_t trip_bucket(uint32_t t) { return t ? 31u - (uint32_t)__clz(t) : 0u; }
@@ -1786,7 +1859,7 @@ __global__ __launch_bounds__(256) void k_pre_scan(const uint32_t* __restrict__ k
 // hdr[1] = the pixels of buckets >= split_bucket (0 when split_bucket is
 // outside 0..23).
 __global__ __launch_bounds__(256) void k_pre_emit(const uint8_t* __restrict__ bkt, int res_x, int res_y, int ax,
-                                                  int ay, int ex, int ey, int split_bucket,
+                                                  int ay, int ex, int ey, int split_bucket, int by_bkt,
                                                   const uint32_t* __restrict__ blk, uint32_t* __restrict__ hdr,
                                                   uint32_t* __restrict__ list)
 {
@@ -1808,7 +1881,13 @@ __global__ __launch_bounds__(256) void k_pre_emit(const uint8_t* __restrict__ bk
     uint32_t take = 0;  // bit j: pixel (x, y0 + j) is pretraced
     for (int j = 0; j < h; j++) {
         const int y = y0 + j;
-        if (whole || (x >= ax && x < ax + ex && y >= ay && y < ay + ey)) {
+        if (by_bkt) {  // k_pre_scan_pred: the pixel's bucket, or kPreNone
+            bk[j] = bkt[(size_t)y * res_x + x];
+            if (bk[j] != kPreNone) {
+                take |= 1u << j;
+                atomicAdd(&hist[bk[j]], 1u);
+            }
+        } else if (whole || (x >= ax && x < ax + ex && y >= ay && y < ay + ey)) {
             take |= 1u << j;
             bk[j] = bkt[(size_t)y * res_x + x];
             atomicAdd(&hist[bk[j]], 1u);
@@ -1830,6 +1909,149 @@ __global__ __launch_bounds__(256) void k_pre_emit(const uint8_t* __restrict__ bk
         if ((take >> j) & 1u) list[first[bk[j]] + atomicAdd(&hist[bk[j]], 1u)] = (uint32_t)x | ((uint32_t)(y0 + j) << 16);
 }
 
+// ---- predicted holes (pre_predict > 0).  Camera of one projection: the
+// reference's raycast_proj arithmetic (kernel.cl:543-562), op order kept.
+struct PredCam {
+    float m0x, m0y, m0z;
+    float r00, r01, r02, r10, r11, r12, r20, r21, r22;
+};
+
+static PredCam pred_cam(const svo_frame_params* p)
+{
+    PredCam k;
+    k.m0x = p->m0[0]; k.m0y = p->m0[1]; k.m0z = p->m0[2];
+    k.r00 = p->row0[0]; k.r01 = p->row0[1]; k.r02 = p->row0[2];
+    k.r10 = p->row1[0]; k.r11 = p->row1[1]; k.r12 = p->row1[2];
+    k.r20 = p->row2[0]; k.r21 = p->row2[1]; k.r22 = p->row2[2];
+    return k;
+}
+
+// target pixel of position q under camera k (-1: behind, too near, a NaN
+// hole marker, or off-screen incl. the last column and row), and its depth key
+__device__ __forceinline__ int pred_target(float4 q, const PredCam& k, int res_x, int res_y, uint32_t* zkey)
+{
+    const float qx = q.x - k.m0x, qy = q.y - k.m0y, qz = q.z - k.m0z;
+    const float hz = dot3(qx, qy, qz, k.r20, k.r21, k.r22);
+    if (!(hz >= 0.05f)) return -1;
+    const float hx = dot3(qx, qy, qz, k.r00, k.r01, k.r02);
+    const float hy = dot3(qx, qy, qz, k.r10, k.r11, k.r12);
+    const float sfx = (hx * (float)res_y + 0.0f) / hz + (float)res_x / 2 - 0.0f;
+    const float sfy = (hy * (float)res_y + 0.0f) / hz + (float)res_y / 2 - 0.0f;
+    if (!(sfx >= 0.0f && sfy >= 0.0f && sfx < (float)(res_x - 1) && sfy < (float)(res_y - 1))) return -1;
+    *zkey = (uint32_t)(int)(hz * 1000.0f) & 0xffffffu;
+    return (int)sfy * res_x + (int)sfx;
+}
+
+// nearest sample per target pixel of an intermediate camera: key = depth << 32 | source
+__global__ __launch_bounds__(256) void k_pred_scatter(const float4* __restrict__ src, PredCam k, int res_x, int res_y,
+                                                      uint32_t N, unsigned long long* __restrict__ keys)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    uint32_t z;
+    const int t = pred_target(src[i], k, res_x, res_y, &z);
+    if (t < 0) return;
+    const unsigned long long key = ((unsigned long long)z << 32) | i;
+    if (keys[t] > key) atomicMin(&keys[t], key);
+}
+
+// the winners of k_pred_scatter (keys back to all ones): their positions
+// into dst (NaN where none), or, with dst == null, projected through the
+// final camera `fin` into the coverage bytes
+__global__ __launch_bounds__(256) void k_pred_resolve(unsigned long long* __restrict__ keys,
+                                                      const float4* __restrict__ src, float4* __restrict__ dst,
+                                                      PredCam fin, int res_x, int res_y, uint32_t N,
+                                                      uint8_t* __restrict__ cov)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    const unsigned long long key = keys[i];
+    if (key == ~0ull) {
+        if (dst) dst[i] = make_float4(__int_as_float(0x7fc00000), 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    keys[i] = ~0ull;
+    const float4 q = src[(uint32_t)key];
+    if (dst) {
+        dst[i] = q;
+        return;
+    }
+    uint32_t z;
+    const int t = pred_target(q, fin, res_x, res_y, &z);
+    if (t >= 0) cov[t] = 1;
+}
+
+// every sample of src projected through the final camera into the coverage bytes
+__global__ __launch_bounds__(256) void k_pred_cover(const float4* __restrict__ src, PredCam fin, int res_x, int res_y,
+                                                    uint32_t N, uint8_t* __restrict__ cov)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    uint32_t z;
+    const int t = pred_target(src[i], fin, res_x, res_y, &z);
+    if (t >= 0) cov[t] = 1;
+}
+
+// Pretrace list, pass 1 of the predicted-hole mode (layout of k_pre_scan: a
+// thread per column of 8 pixels of an 8x8 block).  A pixel is traced ahead
+// when it lies in the refresh rectangle, or in a 2x2 quad (even x and y, inside
+// the whole 16x16 blocks the hole list covers, kernel.cl:262-272) that no
+// predicted sample reaches, in a block whose longest known ray has at least
+// `trips` trips -- unless the block's longest known ray has at least `whole`
+// (> 0) trips: part 0 traces those blocks whole.  bkt[pixel] = its bucket, or
+// kPreNone; the coverage bytes are cleared for the next use.
+__global__ __launch_bounds__(256) void k_pre_scan_pred(const uint32_t* __restrict__ known, uint8_t* __restrict__ cov,
+                                                       int res_x, int res_y, int qx_end, int qy_end, int ax, int ay,
+                                                       int ex, int ey, uint32_t trips, uint32_t whole,
+                                                       uint32_t* __restrict__ blk, uint8_t* __restrict__ bkt,
+                                                       uint32_t* __restrict__ hdr)
+{
+    __shared__ uint32_t hist[kPreBuckets];
+    const int t = threadIdx.x;
+    if (t < kPreBuckets) hist[t] = 0;
+    __syncthreads();
+    const int x = blockIdx.x * 64 + (t & 63);
+    const int by = blockIdx.y * 4 + (t >> 6);
+    const int y0 = by * 8;
+    const bool col_ok = x < res_x && y0 < res_y;
+    const int h = col_ok ? min(8, res_y - y0) : 0;
+    uint32_t kn[8];
+    uint32_t m = 0, cv = 0;  // cv bit j: pixel (x, y0 + j) is reached
+    for (int j = 0; j < 8; j++) {
+        const size_t o = (size_t)(y0 + j) * res_x + x;
+        kn[j] = j < h ? known[o] : 0u;
+        cv |= (j < h && cov[o]) ? 1u << j : 0u;
+        m = max(m, kn[j]);
+    }
+    const uint32_t ncv = (uint32_t)__shfl_xor((int)cv, 1);  // the quad's other column (x ^ 1)
+    for (int o = 1; o < 8; o <<= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    const bool long_blk = m >= trips;
+    const bool part0 = whole > 0u && m >= whole;  // traced whole by part 0
+    const bool qx_ok = (x | 1) < qx_end;
+    uint32_t n = 0;
+    for (int j = 0; j < h; j++) {
+        const int y = y0 + j;
+        const size_t o = (size_t)y * res_x + x;
+        const uint32_t qm = 3u << (j & 6);  // the quad's two rows
+        const bool hole_quad = qx_ok && (y | 1) < qy_end && ((cv | ncv) & qm) == 0u;
+        const bool take = !part0 && ((x >= ax && x < ax + ex && y >= ay && y < ay + ey) || (hole_quad && long_blk));
+        uint8_t b = kPreNone;
+        if (take) {
+            b = (uint8_t)trip_bucket(kn[j]);
+            atomicAdd(&hist[b], 1u);
+            n++;
+        }
+        bkt[o] = b;
+        if ((cv >> j) & 1u) cov[o] = 0;
+    }
+    for (int o = 1; o < 8; o <<= 1) n += (uint32_t)__shfl_xor((int)n, o);
+    const int nbx = (res_x + 7) >> 3;
+    const int bx = x >> 3;
+    if ((t & 7) == 0 && bx < nbx && y0 < res_y) blk[(size_t)by * nbx + bx] = n;
+    __syncthreads();
+    if (t < kPreBuckets && hist[t]) atomicAdd(&hdr[8 + t], hist[t]);
+}
+
 // Pretrace a warp frame on the set's own stream: the refresh rectangle and
 // the long blocks (k_pre_scan), longest known rays first, with the frame's
 // camera, into set s, stamped with the frame's id.  Successive frames use
@@ -1839,30 +2061,85 @@ __global__ __launch_bounds__(256) void k_pre_emit(const uint8_t* __restrict__ bk
 // pretrace.  A primary ray depends only on the camera, the pixel and the
 // octree (kernel.cl:594-694, :846-942), so the copied results are those the
 // frame's own rays would compute.
-static int pretrace(svo_ctx* c, const svo_frame_params* p)
+// whether frame p's pretrace can predict its holes: the snapshot of frame
+// g-k is stored (its event recorded) and the cameras of frames g-k+1 .. g-1
+// are known
+static bool predict_ready(const svo_ctx* c, const svo_frame_params* p)
 {
-    RC(ensure_pre(c));
-    const int s = (int)(c->pre_frames % (uint32_t)c->pre_sets);
-    const uint32_t id = c->pre_frames + 1u;
+    const int k = c->pre_predict, g = p->frame;
+    if (k <= 0 || !c->d_snap_xyz || g - k < 0) return false;
+    if (c->snap_frame[(g - k) % kSnapSlots] != g - k) return false;
+    for (int j = 1; j < k; j++)
+        if (c->cam_frame[(g - k + j) % kSnapSlots] != g - k + j) return false;
+    return true;
+}
+
+// one part of set s's pretrace: list passes (long blocks, the longest
+// blocks only, or predicted holes), then the ray launch into the set's
+// results, stamped `id`; records ev_pre[q]
+enum PrePart { PRE_LONG, PRE_WHOLE, PRE_PREDICT };
+static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part, PrePart kind, uint32_t id)
+{
+    const int q = 2 * s + part;
     const size_t N = c->N;
-    hipStream_t st = c->pre_streams[s];
-    if (c->timing == 2) (void)hipEventRecord(c->ev[0], st);  // the frame starts with its pretrace
+    if (!c->pre_streams[q]) HIPCHECK(hipStreamCreateWithFlags(&c->pre_streams[q], hipStreamNonBlocking));
+    hipStream_t st = c->pre_streams[q];
+    if (c->timing == 2 && part == 1) (void)hipEventRecord(c->ev[0], st);  // the frame starts with its pretrace
     if (c->main_recorded[s]) HIPCHECK(hipStreamWaitEvent(st, c->ev_main[s], 0));
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
-    uint32_t* hdr = c->d_pre_list + (size_t)s * (N + kPreHdr);
+    if (kind == PRE_WHOLE) ax = ay = ex = ey = 0;  // the refresh rectangle is part 1's
+    uint32_t* hdr = c->d_pre_list + (size_t)q * (N + kPreHdr);
     uint32_t* list = hdr + kPreHdr;
-    uint32_t* blk = c->d_pre_blk + (size_t)s * pre_blocks(c);
+    uint32_t* blk = c->d_pre_blk + (size_t)q * pre_blocks(c);
     hipLaunchKernelGGL(k_memset, dim3(1), dim3(256), 0, st, hdr, 0u, (size_t)kPreHdr);
     HIPCHECK(hipGetLastError());
     const dim3 grid((unsigned)((c->res_x + 63) / 64), (unsigned)(((c->res_y + 7) / 8 + 3) / 4));
-    uint8_t* bkt = c->d_pre_bkt + (size_t)s * N;
-    hipLaunchKernelGGL(k_pre_scan, grid, dim3(256), 0, st, (const uint32_t*)c->d_known, c->res_x, c->res_y, ax, ay,
-                       ex, ey, c->pre_trips, blk, bkt, hdr);
+    uint8_t* bkt = c->d_pre_bkt + (size_t)q * N;
+    if (kind == PRE_PREDICT) {
+        // frame g's hole quads as predicted from frame g-k's final plane 0:
+        // nearest sample per pixel through the cameras g-k+1 .. g-1, then
+        // coverage under camera g (no sample lost by a later z-test is
+        // counted, so the quads left cover nearly every hole quad of g)
+        const int k = c->pre_predict, g = p->frame;
+        const int src_slot = (g - k) % kSnapSlots;
+        HIPCHECK(hipStreamWaitEvent(st, c->ev_snap[src_slot], 0));
+        const unsigned nb = (unsigned)((N + 255) / 256);
+        unsigned long long* keys = c->d_pred_keys + (size_t)s * N;
+        float4* inter = c->d_pred_xyz + (size_t)s * N;
+        uint8_t* cov = c->d_pred_cov + (size_t)s * N;
+        const float4* cur = c->d_snap_xyz + (size_t)src_slot * N;
+        const PredCam fin = pred_cam(p);
+        if (k == 1) {
+            hipLaunchKernelGGL(k_pred_cover, dim3(nb), dim3(256), 0, st, cur, fin, c->res_x, c->res_y, (uint32_t)N,
+                               cov);
+        }
+        for (int j = 1; j < k; j++) {
+            const PredCam mid = pred_cam(&c->cam_hist[(g - k + j) % kSnapSlots]);
+            hipLaunchKernelGGL(k_pred_scatter, dim3(nb), dim3(256), 0, st, cur, mid, c->res_x, c->res_y,
+                               (uint32_t)N, keys);
+            const bool last = j == k - 1;
+            // (k == 3: the intermediate positions go to `inter`, which the
+            // next scatter reads; the first scatter read the snapshot)
+            hipLaunchKernelGGL(k_pred_resolve, dim3(nb), dim3(256), 0, st, keys, cur, last ? nullptr : inter, fin,
+                               c->res_x, c->res_y, (uint32_t)N, cov);
+            cur = inter;
+        }
+        HIPCHECK(hipGetLastError());
+        // blocks part 0 traces whole are left to it
+        const uint32_t whole = c->pre_early ? c->pre_whole_trips : 0u;
+        hipLaunchKernelGGL(k_pre_scan_pred, grid, dim3(256), 0, st, (const uint32_t*)c->d_known, cov, c->res_x,
+                           c->res_y, (int)(c->res_x / 16 * 16), (int)(c->res_y / 16 * 16), ax, ay, ex, ey,
+                           c->pre_trips, whole, blk, bkt, hdr);
+    } else {
+        const uint32_t trips = kind == PRE_WHOLE ? c->pre_whole_trips : c->pre_trips;
+        hipLaunchKernelGGL(k_pre_scan, grid, dim3(256), 0, st, (const uint32_t*)c->d_known, c->res_x, c->res_y, ax,
+                           ay, ex, ey, trips, blk, bkt, hdr);
+    }
     HIPCHECK(hipGetLastError());
     const int split_bucket = c->pre_split_trips > 0 ? 31 - __builtin_clz((unsigned)c->pre_split_trips) : -1;
     hipLaunchKernelGGL(k_pre_emit, grid, dim3(256), 0, st, (const uint8_t*)bkt, c->res_x, c->res_y, ax, ay, ex, ey,
-                       split_bucket, (const uint32_t*)blk, hdr, list);
+                       split_bucket, kind == PRE_PREDICT ? 1 : 0, (const uint32_t*)blk, hdr, list);
     HIPCHECK(hipGetLastError());
     View v = make_view(c, p);  // no pretrace fields: pre_active is clear between frames
     set_rays(v, list, hdr, 0, 0, 0, 0, nullptr);
@@ -1875,12 +2152,29 @@ static int pretrace(svo_ctx* c, const svo_frame_params* p)
     Split split;
     if (split_bucket >= 0) {
         split.count = hdr + 1;
-        split.queue = c->d_pre_queue + 4 * s + 2;
+        split.queue = c->d_pre_queue + 4 * q + 2;
         split.shift = c->pre_split_shift;
     }
-    RC(launch_rays_v(c, st, v, nullptr, nullptr, 1, c->d_pre_queue + 4 * s, c->pre_seq[s], c->d_queue + 16, 6,
+    RC(launch_rays_v(c, st, v, nullptr, nullptr, 1, c->d_pre_queue + 4 * q, c->pre_seq[q], c->d_queue + 16, 6,
                      c->pre_wg, split));
-    HIPCHECK(hipEventRecord(c->ev_pre[s], st));  // the frame's ray launch waits for it
+    HIPCHECK(hipEventRecord(c->ev_pre[q], st));  // the frame's ray launch waits for it
+    return SVO_OK;
+}
+
+// Pretrace a warp frame into set s (one part, or two with predicted holes
+// and a whole-block threshold: part 0 starts at once, part 1 after the
+// snapshot it projects).  Parts write different pixels, except where the
+// known trips change between their list passes: a pixel traced by both gets
+// the same result twice.
+static int pretrace(svo_ctx* c, const svo_frame_params* p)
+{
+    RC(ensure_pre(c));
+    const int s = (int)(c->pre_frames % (uint32_t)c->pre_sets);
+    const uint32_t id = c->pre_frames + 1u;
+    const bool pred = predict_ready(c, p);
+    c->pre_early = pred && c->pre_whole_trips > 0;
+    if (c->pre_early) RC(pretrace_part(c, p, s, 0, PRE_WHOLE, id));
+    RC(pretrace_part(c, p, s, 1, pred ? PRE_PREDICT : PRE_LONG, id));
     c->pre_frames++;
     c->pre_active = true;
     c->pre_set = s;
@@ -1911,7 +2205,10 @@ static bool want_pretrace(svo_ctx* c, const svo_frame_params* p)
 // the frame's ray launch copies its pretrace's results: it waits for them
 static int wait_pretrace(svo_ctx* c)
 {
-    if (c->pre_active) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[c->pre_set], 0));
+    if (c->pre_active) {
+        HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[2 * c->pre_set + 1], 0));
+        if (c->pre_early) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[2 * c->pre_set], 0));
+    }
     return SVO_OK;
 }
 
@@ -1927,20 +2224,23 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     const bool pre = want_pretrace(c, p);
     const bool graph = c->use_graph && c->timing != 1 && p->frame >= 2 && !pre;
     if (graph && !c->graph[0]) RC(build_graph(c));
+    float4* snap = snap_begin(c, p);
     frame_begin(c);
     if (pre) RC(pretrace(c, p));
     int rc;
     if (graph) {
-        rc = frame_warp_graph(c, p);
+        rc = frame_warp_graph(c, p, snap);
     } else {
         rc = warp_front(c, p);
         const Launch L = single(c);
-        const View v = make_view(c, p);
+        View v = make_view(c, p);
+        v.snap_out = snap;
         if (!rc) mark(c, ST_COPY);
         if (!rc) rc = launch_copy(L, v, 2);
         if (!rc) mark(c, ST_FILTER);
         if (!rc) rc = launch_fill(L, v, nullptr, 1, 1);
     }
+    if (!rc) snap_end(c, p);
     pretrace_done(c);
     RC(rc);
     return frame_end(c);
@@ -1970,7 +2270,7 @@ static int ensure_cnn_buffers(svo_ctx* c)
 // replaces the filtered colour of their hole pixels in the image.  Screen
 // words keep the reference filter's result.  cnn == NULL keeps the filter's
 // colours (gate only).
-static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
+static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes, float4* snap);
 
 extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
 {
@@ -1980,16 +2280,19 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     HIPCHECK(hipSetDevice(c->device));
     RC(ensure_cnn_buffers(c));
     take_image(c);
+    const bool pre = want_pretrace(c, p);
+    float4* snap = snap_begin(c, p);
     frame_begin(c);
-    if (want_pretrace(c, p)) RC(pretrace(c, p));
-    const int rc = learned_body(c, cnn, p, max_cnn_holes);
+    if (pre) RC(pretrace(c, p));
+    const int rc = learned_body(c, cnn, p, max_cnn_holes, snap);
+    if (!rc) snap_end(c, p);
     pretrace_done(c);
     RC(rc);
     return frame_end(c);
 }
 
 // the learned frame after its pretrace (if any) is issued
-static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
+static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes, float4* snap)
 {
     RC(warp_compact(c, p));
     int ax, ay, ex, ey;
@@ -2026,7 +2329,9 @@ static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int
     RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
     const Launch L = single(c);
     mark(c, ST_COPY);
+    v.snap_out = snap;
     RC(launch_copy(L, v, 2));
+    v.snap_out = nullptr;
     mark(c, ST_FILTER);
     RC(launch_fill(L, v, nullptr, 1, 1));
     if (cnn) {
@@ -2101,6 +2406,10 @@ extern "C" int svo_set_pretrace(svo_ctx* c, const svo_pretrace_config* cfg)
         return svo_fail(SVO_ERR_ARG, "pretrace sets %d: 0 (off) or 2..%d", sets, kPreMaxSets);
     if (long_trips < 1 || long_trips > SVO_MAX_RAY_ITERS)
         return svo_fail(SVO_ERR_ARG, "long_trips %d outside 1..%d", long_trips, (int)SVO_MAX_RAY_ITERS);
+    if (cfg->predict < 0 || cfg->predict > kSnapSlots - 1)
+        return svo_fail(SVO_ERR_ARG, "predict %d outside 0..%d", cfg->predict, kSnapSlots - 1);
+    if (cfg->whole_trips < 0 || cfg->whole_trips > SVO_MAX_RAY_ITERS)
+        return svo_fail(SVO_ERR_ARG, "whole_trips %d outside 0..%d", cfg->whole_trips, (int)SVO_MAX_RAY_ITERS);
     HIPCHECK(hipSetDevice(c->device));
     if (sets != c->pre_sets) {
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -2112,6 +2421,8 @@ extern "C" int svo_set_pretrace(svo_ctx* c, const svo_pretrace_config* cfg)
     c->pre_split_trips = cfg->split_trips;
     c->pre_split_shift = 31 - __builtin_clz((unsigned)sl);
     c->pre_only_queued = cfg->only_queued != 0;
+    c->pre_predict = cfg->predict;
+    c->pre_whole_trips = (uint32_t)cfg->whole_trips;
     return SVO_OK;
 }
 
@@ -2124,6 +2435,8 @@ extern "C" int svo_get_pretrace(svo_ctx* c, svo_pretrace_config* cfg)
     cfg->split_trips = c->pre_split_trips;
     cfg->split_lanes = 1 << c->pre_split_shift;
     cfg->only_queued = c->pre_only_queued ? 1 : 0;
+    cfg->predict = c->pre_predict;
+    cfg->whole_trips = (int)c->pre_whole_trips;
     return SVO_OK;
 }
 

@@ -11,7 +11,8 @@ are handed out (svo_set_ray_block); full frames also search the trip count
 that puts a screen block on the long-rays-first list (svo_set_long_first),
 and learned frames the CNN's workgroups alongside the ray launch
 (svo_set_cnn_overlap); warp and learned frames also search the pretrace's
-result sets and block threshold (svo_set_pretrace).  Every candidate replays the same
+result sets, its mode (long blocks or predicted holes) and block threshold
+(svo_set_pretrace).  Every candidate replays the same
 frame window from the same starting state (the window starts with the two
 clearing frames), so candidates see identical work; the frames are issued
 back to back and the score is the mean time between the completions of
@@ -83,13 +84,14 @@ def _search(space, coordinate, measure):
     return best, best_ms
 
 
-# warp frames also search the pretrace's sets and block threshold
-WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[6, 4, 8], pre_long=[256, 128, 384])
+# warp frames also search the pretrace: its sets, its mode (pre_pred: 0 =
+# long blocks, k = holes predicted from frame g-k) and block threshold
+WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[6, 4, 8], pre_pred=[0, 3, 2], pre_long=[256, 1, 128, 384])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
 LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[6, 4, 8],
-                     pre_long=[256, 128, 384])
+                     pre_pred=[0, 3, 2], pre_long=[256, 1, 128, 384])
 
 
 def apply(ctx, cfg):
@@ -105,8 +107,8 @@ def apply(ctx, cfg):
         ctx.set_ray_block(int(cfg["ray_block"]))
     if "cnn_wgs" in cfg:
         ctx.set_cnn_overlap(int(cfg["cnn_wgs"]))
-    if "pre_sets" in cfg or "pre_long" in cfg:
-        ctx.set_pretrace(sets=cfg.get("pre_sets"), long_trips=cfg.get("pre_long"))
+    if "pre_sets" in cfg or "pre_long" in cfg or "pre_pred" in cfg:
+        ctx.set_pretrace(sets=cfg.get("pre_sets"), long_trips=cfg.get("pre_long"), predict=cfg.get("pre_pred"))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None, cnn=None, max_cnn_holes=64):

@@ -344,21 +344,24 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * a frame is pretraced only while the warp frame before it is still on the
  * GPU (the caller issues frames ahead): a frame the GPU starts at once has
  * no earlier frame to overlap, and its own rays finish sooner without one.
- * Defaults: 6 sets, long_trips 256, 1 workgroup per CU, no split,
- * only_queued 1, predict 0, whole_trips 0.  A primary ray depends
+ * Defaults: 4 sets, long_trips 1, 1 workgroup per CU, no split,
+ * only_queued 1, predict 2, whole_trips 0.  A primary ray depends
  * only on the camera, the pixel and the octree (kernel.cl:594-694,
  * :846-942): results do not depend on these settings.  HIP maps streams onto
  * at most GPU_MAX_HW_QUEUES hardware queues (4 unless the environment sets
  * it before the first HIP call); pretraces whose streams share one run one
  * after the other.
  * predict = k (1..3) replaces the long blocks by predicted holes: every warp
- * or learned frame also stores its final plane-0 positions (after its rays,
- * holes marked) in a ring of 4 snapshots, and frame g's pretrace, once frame
- * g-k has completed, projects frame g-k's snapshot through the cameras of
- * frames g-k+1 .. g-1 (keeping the nearest sample per pixel, as the warp
- * does) and then g, and traces the 2x2 quads no sample reaches -- the hole
- * quads frame g's warp can leave -- in the 8x8 blocks whose last traced rays
- * include one of at least long_trips trips, with the refresh rectangle.  A
+ * or learned frame also stores its plane-0 positions after its projection
+ * (before its rays; holes marked) in a ring of 4 snapshots, and frame g's
+ * pretrace, once frame g-k's projection has run, projects frame g-k's
+ * snapshot through the cameras of frames g-k+1 .. g-1 (keeping the nearest
+ * sample per pixel, as the warp does) and then g, and traces the 2x2 quads
+ * no sample reaches -- the hole quads frame g's warp can leave -- in the 8x8
+ * blocks whose last traced rays include one of at least long_trips trips,
+ * with the refresh rectangle.  A frame with a pretrace traces its rays on
+ * none of the pretrace's lists as soon as the lists are made, and copies
+ * the pretraced results once the pretrace has completed.  A
  * frame whose frames g-k .. g-1 did not all pass through svo_frame_warp or
  * svo_frame_warp_learned (consecutive frame counters) falls back to the long
  * blocks.  With predict > 0 and whole_trips > 0, the 8x8 blocks whose last

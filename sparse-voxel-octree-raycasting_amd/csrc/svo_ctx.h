@@ -99,8 +99,8 @@ struct svo_ctx {
     // into one of pre_sets result sets (set s on pre_streams[s]) while
     // earlier frames still run; the frame's own ray launch copies those of
     // its rays
-    int pre_sets = 6;              // 0 = off
-    uint32_t pre_trips = 256;
+    int pre_sets = 4;              // 0 = off
+    uint32_t pre_trips = 1;
     int pre_wg = 1;                // workgroups per CU of the pretrace launch (0: wg_per_cu)
     int pre_split_trips = 0;       // blocks with a known ray this long go to the split waves (0: no split)
     int pre_split_shift = 3;       // log2 of the lanes per split wave
@@ -122,6 +122,7 @@ struct svo_ctx {
     uint8_t* d_pre_bkt = nullptr;     // per set and part: one byte per pixel
     int pre_alloc_sets = 0;           // sets the buffers hold (0: not set up)
     hipEvent_t ev_pre[2 * kPreMaxSets] = {};  // a set's pretrace part is complete
+    hipEvent_t ev_list[2 * kPreMaxSets] = {};  // ... its pixel list (bucket bytes) is final
     hipEvent_t ev_main[kPreMaxSets] = {};  // a set's frame is complete
     bool main_recorded[kPreMaxSets] = {};
     uint32_t pre_frames = 0;  // frames pretraced since the sets were cleared; the stamp is this count
@@ -129,15 +130,15 @@ struct svo_ctx {
     bool pre_active = false;  // the frame being issued has a pretrace in set pre_set, stamp pre_id
     bool pre_early = false;   // ... with a part 0
     // predicted-hole pretrace (pre_predict = k > 0): each warp/learned frame's
-    // copy stores its final plane-0 positions (holes as NaN) in snapshot slot
-    // frame % kSnapSlots; frame g's pretrace projects snapshot g-k through
-    // the cameras of frames g-k+1 .. g and traces the hole quads no sample
-    // reaches (and the refresh rectangle)
-    int pre_predict = 0;
+    // projection (resolve) stores its plane-0 positions (holes as NaN) in
+    // snapshot slot frame % kSnapSlots; frame g's pretrace projects snapshot
+    // g-k through the cameras of frames g-k+1 .. g and traces the hole quads
+    // no sample reaches (and the refresh rectangle)
+    int pre_predict = 2;
     uint32_t pre_whole_trips = 0;           // part 0's block threshold (0: no part 0)
     float4* d_snap_xyz = nullptr;           // kSnapSlots x N
     int snap_frame[kSnapSlots] = {-1, -1, -1, -1};  // frame counter each slot holds (-1: none)
-    hipEvent_t ev_snap[kSnapSlots] = {};    // the slot's frame has completed
+    hipEvent_t ev_snap[kSnapSlots] = {};    // the slot's frame has written it
     svo_frame_params cam_hist[kSnapSlots] = {};  // cameras of recent frames, by frame counter
     int cam_frame[kSnapSlots] = {-1, -1, -1, -1};
     int snap_pending = -1;                  // slot written by the frame being issued (-1: none)

@@ -158,11 +158,19 @@ struct View {
     const float4* pre_xyz;
     uint32_t* stamp_out;        // pretrace launches: stamp per traced pixel (or null)
     uint32_t pre_id;
+    // with a pretrace the frame's rays run as two launches: pre_mode 1 traces
+    // the rays whose pixel is on neither part's pretrace list (bucket byte
+    // kPreNone; part 0's list may be absent) while the pretrace still runs,
+    // then pre_mode 2 copies the pretraced results; pre_mode 0 does both
+    const uint8_t* pre_bkt0;
+    const uint8_t* pre_bkt1;
+    int pre_mode;
     uint32_t* trips_out;        // per pixel: loop trips of the last ray traced there (or null)
-    float4* snap_out;           // warp frames' copy: final plane-0 positions, holes as NaN (or null)
+    float4* snap_out;           // warp frames' resolve: plane-0 positions after the warp, holes as NaN (or null)
 };
 
 constexpr uint32_t QUAD_NONE = 0xffffffffu;  // list entry without a ray
+constexpr uint8_t kPreNone = 0xff;           // pretrace list passes: the pixel is not traced ahead
 constexpr uint32_t NO_BLOCK = 0xffffffffu;
 
 #define VIEW_OF(idx) const View& V = views ? views[(idx)] : v
@@ -235,6 +243,36 @@ __device__ __forceinline__ uint32_t view_rays(const View& V)
     return (V.list ? min(*V.list_count, V.N) : 0u) + (V.list2 ? min(*V.list2_count, V.N) : 0u) + V.n_rect;
 }
 
+// Ray k of a view (its pixel lists, then its rectangle in 64-pixel blocks):
+// false when it has none; blk = its 8x8 screen block for the long-ray flags
+// of full frames (NO_BLOCK when not needed).
+__device__ __forceinline__ bool ray_pixel(const View& V, uint32_t k, int& x, int& y, uint32_t& blk)
+{
+    const uint32_t H = V.list ? min(*V.list_count, V.N) : 0u;
+    const uint32_t H2 = V.list2 ? min(*V.list2_count, V.N) : 0u;
+    if (k < H + H2) {
+        const uint32_t w = (k < H) ? V.list[k] : V.list2[k - H];
+        if (w == QUAD_NONE) return false;
+        x = (int)(w & 0xffffu);
+        y = (int)(w >> 16);
+        blk = V.block_long ? (uint32_t)((y >> (6 - V.blk_shift)) * V.tiles_x + (x >> V.blk_shift)) : NO_BLOCK;
+    } else {
+        k -= H + H2;
+        const uint32_t t = k >> 6, w = k & 63u;
+        if (V.rect_skip && V.rect_skip[t]) return false;
+        blk = t;
+        int tx = (int)(t % (uint32_t)V.tiles_x), ty = (int)(t / (uint32_t)V.tiles_x);
+        const int bs = V.blk_shift;
+        int lx = (tx << bs) + (int)(w & ((1u << bs) - 1u)), ly = (ty << (6 - bs)) + (int)(w >> bs);
+        if (lx >= V.ext_x || ly >= V.ext_y) return false;
+        x = V.add_x + lx;
+        y = V.add_y + ly;
+    }
+    // quad rays may pick the pixel one past the row or frame (their flat
+    // offset then lies in the next row or in plane 1, inside the buffers)
+    return V.edge_ok || (x < V.res_x && y < V.res_y);
+}
+
 // Ray set-up: kernel.cl:883-908 and CAST_RAY prologue :115-150.  Returns
 // RAY_NONE (no ray), RAY_TRACE, or RAY_MERGED (the pixel's result came from
 // the frame's pretrace and has been written).
@@ -253,33 +291,14 @@ __device__ __forceinline__ int init_ray(Ray& r, uint32_t k, const RayArgs& a)
         k -= a.view_first[vi];
     }
     const View& V = a.views ? a.views[vi] : a.v;
-    const uint32_t H = V.list ? min(*V.list_count, V.N) : 0u;
-    const uint32_t H2 = V.list2 ? min(*V.list2_count, V.N) : 0u;
     int x, y;
-    if (k < H + H2) {
-        const uint32_t w = (k < H) ? V.list[k] : V.list2[k - H];
-        if (w == QUAD_NONE) return RAY_NONE;
-        x = (int)(w & 0xffffu);
-        y = (int)(w >> 16);
-        r.blk = V.block_long ? (uint32_t)((y >> (6 - V.blk_shift)) * V.tiles_x + (x >> V.blk_shift)) : NO_BLOCK;
-    } else {
-        k -= H + H2;
-        const uint32_t t = k >> 6, w = k & 63u;
-        if (V.rect_skip && V.rect_skip[t]) return RAY_NONE;
-        r.blk = t;
-        int tx = (int)(t % (uint32_t)V.tiles_x), ty = (int)(t / (uint32_t)V.tiles_x);
-        const int bs = V.blk_shift;
-        int lx = (tx << bs) + (int)(w & ((1u << bs) - 1u)), ly = (ty << (6 - bs)) + (int)(w >> bs);
-        if (lx >= V.ext_x || ly >= V.ext_y) return RAY_NONE;
-        x = V.add_x + lx;
-        y = V.add_y + ly;
-    }
-    // quad rays may pick the pixel one past the row or frame (their flat
-    // offset then lies in the next row or in plane 1, inside the buffers)
-    if (!V.edge_ok && (x >= V.res_x || y >= V.res_y)) return RAY_NONE;
+    if (!ray_pixel(V, k, x, y, r.blk)) return RAY_NONE;
     r.pix = (uint32_t)(y * V.res_x + x);
     r.view = vi;
-    if (V.pre_stamp && V.pre_stamp[r.pix] == V.pre_id) {
+    if (V.pre_mode == 1 &&
+        (V.pre_bkt1[r.pix] != kPreNone || (V.pre_bkt0 && V.pre_bkt0[r.pix] != kPreNone)))
+        return RAY_NONE;  // on a pretrace list: copied by the pre_mode 2 launch
+    if (V.pre_mode != 1 && V.pre_stamp && V.pre_stamp[r.pix] == V.pre_id) {
         // traced ahead with this frame's camera: the same word and position
         // finish_ray would write
         const uint32_t word = V.pre_word[r.pix];
@@ -292,6 +311,7 @@ __device__ __forceinline__ int init_ray(Ray& r, uint32_t k, const RayArgs& a)
         if (V.ray_image) V.ray_image[r.pix] = colorize_word(word);
         return RAY_MERGED;
     }
+    if (V.pre_mode == 2) return RAY_NONE;  // traced by the pre_mode 1 launch
 
     const int D = a.depth;
     const float SCALE_MAX = (float)(1 << (D + 1));
@@ -619,6 +639,35 @@ __global__ __launch_bounds__(RAY_BLOCK) void k_rays(RayArgs a)
     if (lane == 0 && longest) atomicMax(a.max_trips, longest);
 }
 
+// The copy half of a frame's rays with a pretrace: every ray of the view
+// whose pixel carries the frame's stamp takes the pretraced word and
+// position (what finish_ray would write; the other rays were traced by the
+// launch before).  One thread per ray; merged rays counted into *merged.
+__global__ __launch_bounds__(256) void k_pre_merge(View v, unsigned long long* merged)
+{
+    const View& V = v;
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    int x = 0, y = 0;
+    uint32_t blk;
+    bool took = false;
+    if (k < view_rays(V) && ray_pixel(V, k, x, y, blk)) {
+        const uint32_t pix = (uint32_t)(y * V.res_x + x);
+        if (V.pre_stamp[pix] == V.pre_id) {
+            const uint32_t word = V.pre_word[pix];
+            const float4 q = V.pre_xyz[pix];
+            V.screen[pix] = word;
+            float* c = (float*)V.coords + 4u * pix;
+            c[0] = q.x;
+            c[1] = q.y;
+            c[2] = q.z;
+            if (V.ray_image) V.ray_image[pix] = colorize_word(word);
+            took = true;
+        }
+    }
+    const unsigned long long b = __ballot(took);
+    if (__lane_id() == 0 && b) atomicAdd(merged, (unsigned long long)__popcll(b));
+}
+
 // ---------------------------------------------------------------- warp
 __global__ __launch_bounds__(256) void k_memset(uint32_t* __restrict__ dst, uint32_t val, size_t n)
 {
@@ -664,7 +713,9 @@ __global__ __launch_bounds__(256) void k_proj_scatter(View v, const View* views)
 
 // Resolve: winner words/coords into plane 0 (kernel.cl:571-585 semantics
 // against the plane-0 word w0), key reset; clear=1 also writes the hole word
-// to every pixel without a winner (the per-frame memset of plane 0).
+// to every pixel without a winner (the per-frame memset of plane 0).  With
+// snap_out (warp frames with predicted holes) the winners' positions, NaN
+// elsewhere, also go to the frame's snapshot.
 __global__ __launch_bounds__(256) void k_proj_resolve(View v, const View* views, int clear)
 {
     VIEW_OF(blockIdx.z);
@@ -682,10 +733,12 @@ __global__ __launch_bounds__(256) void k_proj_resolve(View v, const View* views,
             const float hz = dot3(qx, qy, qz, V.r20, V.r21, V.r22);
             V.screen[i] = (depth << 8) + (uint32_t)(key & 255u);
             V.coords[i] = make_float4(c.x, c.y, c.z, hz);
+            if (V.snap_out) V.snap_out[i] = make_float4(c.x, c.y, c.z, hz);
             return;
         }
     }
     if (clear) V.screen[i] = HOLE;
+    if (V.snap_out) V.snap_out[i] = make_float4(__int_as_float(0x7fc00000), 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- compaction
@@ -802,11 +855,8 @@ __global__ __launch_bounds__(256) void k_copy_plane(View v, const View* views, u
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= V.N) return;
     const size_t d = (size_t)dst_plane * V.N + i;
-    const uint32_t w = V.screen[i];
-    const float4 q = V.coords[i];
-    V.screen[d] = w;
-    V.coords[d] = q;
-    if (V.snap_out) V.snap_out[i] = w == HOLE ? make_float4(__int_as_float(0x7fc00000), 0.0f, 0.0f, 0.0f) : q;
+    V.screen[d] = V.screen[i];
+    V.coords[d] = V.coords[i];
 }
 
 // learned frames: the CNN's input, plane 0 as the gate leaves it (the rays
@@ -1187,6 +1237,8 @@ extern "C" int svo_create(svo_ctx** out, int device)
     for (int i = 0; ok && i < kPreMaxSets; i++)
         ok = hipEventCreateWithFlags(&c->ev_pre[2 * i], hipEventDisableTiming) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_pre[2 * i + 1], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_list[2 * i], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_list[2 * i + 1], hipEventDisableTiming) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_main[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         svo_destroy(c);
@@ -1223,6 +1275,8 @@ extern "C" int svo_destroy(svo_ctx* c)
     if (c->cnn_join) (void)hipEventDestroy(c->cnn_join);
     if (c->cnn_stream) (void)hipStreamDestroy(c->cnn_stream);
     for (auto& e : c->ev_pre)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->ev_list)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->ev_main)
         if (e) (void)hipEventDestroy(e);
@@ -1582,6 +1636,9 @@ static int launch_rays(svo_ctx* c, const svo_frame_params* p, const uint32_t* li
     return launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq);
 }
 
+static float4* snap_out(const svo_ctx* c);
+static void snap_end(svo_ctx* c, const svo_frame_params* p);
+
 // the warp stages of raycast_draw up to the hole compaction
 static int warp_compact(svo_ctx* c, const svo_frame_params* p)
 {
@@ -1596,10 +1653,13 @@ static int warp_compact(svo_ctx* c, const svo_frame_params* p)
     View v = make_view(c, p);
     v.plane_lo = c->plane1_holes ? 2 : 1;
     v.n_planes = 3 - v.plane_lo;
+    v.snap_out = snap_out(c);
     mark(c, ST_SCATTER);
     RC(launch_scatter(L, v, v.n_planes));
     mark(c, ST_RESOLVE);
     RC(launch_resolve(L, v, 1));
+    snap_end(c, p);  // later frames' pretraces may read the snapshot from here on
+    v.snap_out = nullptr;
     mark(c, ST_COUNT);
     RC(launch_count(L, v));
     mark(c, ST_SCAN);
@@ -1608,7 +1668,30 @@ static int warp_compact(svo_ctx* c, const svo_frame_params* p)
     return launch_write(L, v);
 }
 
-static int wait_pretrace(svo_ctx* c);
+// The frame's hole (and gate) and refresh-rectangle rays.  With a
+// pretrace, two launches: once the pretrace's pixel lists are final, the
+// rays on none of them are traced (overlapping the pretrace's own rays);
+// once the pretrace has completed, its results are copied for the rest.
+static int frame_rays(svo_ctx* c, View v)
+{
+    if (!c->pre_active) return launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq);
+    const int s = c->pre_set;
+    HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_list[2 * s + 1], 0));
+    if (c->pre_early) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_list[2 * s], 0));
+    v.pre_bkt1 = c->d_pre_bkt + (size_t)(2 * s + 1) * c->N;
+    v.pre_bkt0 = c->pre_early ? c->d_pre_bkt + (size_t)(2 * s) * c->N : nullptr;
+    v.pre_mode = 1;
+    RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
+    HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[2 * s + 1], 0));
+    if (c->pre_early) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[2 * s], 0));
+    v.pre_mode = 2;
+    // one thread per possible ray: both lists hold at most N pixels each
+    const size_t n = (v.list ? c->N : 0) + (v.list2 ? c->N : 0) + v.n_rect;
+    hipLaunchKernelGGL(k_pre_merge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, v,
+                       (unsigned long long*)(c->d_queue + 12));
+    HIPCHECK(hipGetLastError());
+    return SVO_OK;
+}
 
 // the warp stages of raycast_draw up to and including the hole and tile rays
 static int warp_front(svo_ctx* c, const svo_frame_params* p)
@@ -1617,9 +1700,9 @@ static int warp_front(svo_ctx* c, const svo_frame_params* p)
     int ax, ay, ex, ey;
     svo_tile_schedule(c->res_x, c->res_y, p->frame, &ax, &ay, &ex, &ey);
     mark(c, ST_RAYS);
-    // the frame's projection and compaction overlap the end of its pretrace
-    RC(wait_pretrace(c));
-    return launch_rays(c, p, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
+    View v = make_view(c, p);
+    set_rays(v, c->d_ids + 2 * c->S, c->d_ids, ax, ay, ex, ey, nullptr);
+    return frame_rays(c, v);
 }
 
 // Capture the warp frame's launches (frames >= 2: no clear) into one graph.
@@ -1689,7 +1772,6 @@ static int frame_warp_graph(svo_ctx* c, const svo_frame_params* p, float4* snap)
 // bucket), four ray-queue counters.
 constexpr int kPreHdr = 64;
 constexpr int kPreBuckets = 24;  // bucket b: blocks whose longest known ray has floor(log2(trips)) = b
-constexpr uint8_t kPreNone = 0xff;  // k_pre_scan_pred: the pixel is not traced ahead
 
 static size_t pre_blocks(const svo_ctx* c) { return (size_t)((c->res_x + 7) / 8) * (size_t)((c->res_y + 7) / 8); }
 
@@ -1767,6 +1849,13 @@ static int ensure_pre(svo_ctx* c)
     return SVO_OK;
 }
 
+// the snapshot slot the frame being issued writes (its projection's
+// resolve), or null
+static float4* snap_out(const svo_ctx* c)
+{
+    return c->snap_pending >= 0 ? c->d_snap_xyz + (size_t)c->snap_pending * c->N : nullptr;
+}
+
 // A warp or learned frame being issued: record its camera; with predicted
 // holes on, point its copy at snapshot slot frame % kSnapSlots (the slot's
 // last reader, a pretrace of a frame before this one, has completed: the
@@ -1785,11 +1874,11 @@ static float4* snap_begin(svo_ctx* c, const svo_frame_params* p)
     c->cam_frame[slot] = p->frame;
     c->snap_frame[slot] = -1;  // being rewritten
     c->snap_pending = slot;
-    return c->d_snap_xyz + (size_t)slot * c->N;
+    return snap_out(c);
 }
 
-// after the frame's launches are queued: its snapshot is complete when the
-// frame is
+// after the launch that writes the frame's snapshot is queued: a pretrace
+// may read it once this event has completed (no-op when already recorded)
 static void snap_end(svo_ctx* c, const svo_frame_params* p)
 {
     const int slot = c->snap_pending;
@@ -1842,6 +1931,8 @@ __global__ __launch_bounds__(256) void k_pre_scan(const uint32_t* __restrict__ k
             const uint32_t b = trip_bucket(kn[j]);
             bkt[(size_t)(y0 + j) * res_x + x] = (uint8_t)b;
             atomicAdd(&hist[b], 1u);
+        } else {
+            bkt[(size_t)(y0 + j) * res_x + x] = kPreNone;
         }
     const int nbx = (res_x + 7) >> 3;
     const int bx = x >> 3;
@@ -2141,6 +2232,7 @@ static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part,
     hipLaunchKernelGGL(k_pre_emit, grid, dim3(256), 0, st, (const uint8_t*)bkt, c->res_x, c->res_y, ax, ay, ex, ey,
                        split_bucket, kind == PRE_PREDICT ? 1 : 0, (const uint32_t*)blk, hdr, list);
     HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(c->ev_list[q], st));  // bucket bytes final: the frame's first ray launch may read them
     View v = make_view(c, p);  // no pretrace fields: pre_active is clear between frames
     set_rays(v, list, hdr, 0, 0, 0, 0, nullptr);
     v.screen = c->d_pre_word + s * N;
@@ -2202,15 +2294,6 @@ static bool want_pretrace(svo_ctx* c, const svo_frame_params* p)
     return c->frame_done_recorded && hipEventQuery(c->ev_frame_done) == hipErrorNotReady;
 }
 
-// the frame's ray launch copies its pretrace's results: it waits for them
-static int wait_pretrace(svo_ctx* c)
-{
-    if (c->pre_active) {
-        HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[2 * c->pre_set + 1], 0));
-        if (c->pre_early) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_pre[2 * c->pre_set], 0));
-    }
-    return SVO_OK;
-}
 
 extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
 {
@@ -2233,8 +2316,7 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     } else {
         rc = warp_front(c, p);
         const Launch L = single(c);
-        View v = make_view(c, p);
-        v.snap_out = snap;
+        const View v = make_view(c, p);
         if (!rc) mark(c, ST_COPY);
         if (!rc) rc = launch_copy(L, v, 2);
         if (!rc) mark(c, ST_FILTER);
@@ -2270,7 +2352,7 @@ static int ensure_cnn_buffers(svo_ctx* c)
 // replaces the filtered colour of their hole pixels in the image.  Screen
 // words keep the reference filter's result.  cnn == NULL keeps the filter's
 // colours (gate only).
-static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes, float4* snap);
+static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
 
 extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
 {
@@ -2281,10 +2363,10 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     RC(ensure_cnn_buffers(c));
     take_image(c);
     const bool pre = want_pretrace(c, p);
-    float4* snap = snap_begin(c, p);
+    snap_begin(c, p);
     frame_begin(c);
     if (pre) RC(pretrace(c, p));
-    const int rc = learned_body(c, cnn, p, max_cnn_holes, snap);
+    const int rc = learned_body(c, cnn, p, max_cnn_holes);
     if (!rc) snap_end(c, p);
     pretrace_done(c);
     RC(rc);
@@ -2292,7 +2374,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
 }
 
 // the learned frame after its pretrace (if any) is issued
-static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes, float4* snap)
+static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes)
 {
     RC(warp_compact(c, p));
     int ax, ay, ex, ey;
@@ -2325,13 +2407,10 @@ static int learned_body(svo_ctx* c, svo_cnn* cnn, const svo_frame_params* p, int
     }
     // hole list, gate rays and the refresh tile in one ray launch
     mark(c, ST_RAYS);
-    RC(wait_pretrace(c));
-    RC(launch_rays_v(c, c->stream, v, nullptr, nullptr, 1, c->d_queue, c->launch_seq));
+    RC(frame_rays(c, v));
     const Launch L = single(c);
     mark(c, ST_COPY);
-    v.snap_out = snap;
     RC(launch_copy(L, v, 2));
-    v.snap_out = nullptr;
     mark(c, ST_FILTER);
     RC(launch_fill(L, v, nullptr, 1, 1));
     if (cnn) {

@@ -1,8 +1,9 @@
 """svo_set_pretrace: warp frames whose refresh rectangle and long-ray blocks
-are traced ahead on a second stream equal frames traced entirely in the
-frame (and the oracle), for every set count, block threshold and launch
-path; the frame rays that take a result traced ahead are counted, and every
-traced or merged ray is one of the frame's hole and rectangle rays."""
+(or predicted hole quads) are traced ahead on other streams equal frames
+traced entirely in the frame (and the oracle), for every set count, block
+threshold, prediction depth and launch path; the frame rays that take a
+result traced ahead are counted, and every traced or merged ray is one of
+the frame's hole and rectangle rays."""
 import numpy as np
 import pytest
 
@@ -43,7 +44,8 @@ def test_pretrace_frames_equal_plain_frames(scene1, sets, long_trips, graph, wg,
             c.upload_octree(words, root, CONFIG1["depth"])
             c.resize(rx, ry)
             c.set_graph(graph)
-        a.set_pretrace(sets, long_trips, wg, split, lanes, only_queued=0)  # every frame, synchronised or not
+        # long blocks (predict 0), every frame, synchronised or not
+        a.set_pretrace(sets, long_trips, wg, split, lanes, only_queued=0, predict=0)
         cfg = a.get_pretrace()
         assert (cfg.sets, cfg.long_trips, cfg.wg_per_cu, cfg.split_trips, cfg.split_lanes, cfg.only_queued) == \
             (sets, long_trips, wg, split, lanes, 0)
@@ -92,7 +94,7 @@ def test_pretrace_flythrough_vs_oracle():
         with S.Context(0) as c:
             c.upload_octree(words, root, depth)
             c.resize(rx, ry)
-            c.set_pretrace(3, 512, only_queued=0)
+            c.set_pretrace(3, 512, only_queued=0, predict=0)
             longest_main, longest_pre = [], []
             for f in range(30):
                 p = W.frame_params(FLY, rows, f)
@@ -222,7 +224,7 @@ def test_pretrace_learned_frames_equal_plain_learned_frames(scene1, max_cnn_hole
         for c in (a, b):
             c.upload_octree(words, root, CONFIG1["depth"])
             c.resize(rx, ry)
-        a.set_pretrace(3, 64, only_queued=0)
+        a.set_pretrace(3, 64, only_queued=0, predict=0)
         b.set_pretrace(0)
         for f in range(14):
             print(f"pretrace learned frame {f} max_cnn_holes {max_cnn_holes}", flush=True)
@@ -294,8 +296,8 @@ def test_pretrace_predicted_holes_equal_plain_frames(scene1, predict, long_trips
             if f % 4 == 3 or f == n - 1:
                 _same(a, b, f"frame {f}")
         print("hole rays traced in the frame", in_frame, "of", holes, flush=True)
-        if long_trips == 1:  # k = 3 loses more samples to the z-tests of two frames
-            assert in_frame <= (0.05 if predict <= 2 else 0.15) * holes
+        if long_trips == 1:  # the prediction misses a few holes, more the further back it starts
+            assert in_frame <= (0.1 if predict <= 2 else 0.2) * holes
     finally:
         a.close()
         b.close()

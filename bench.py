@@ -99,6 +99,8 @@ def run_gpu(args, name):
             best, ms, _ = A.tune(c, tp, warmup=10, mode=mode)
             res[f"tuned_{mode}"] = dict(best, ms=round(ms, 4))
             log(f"[bench] autotune {mode}: {best} -> {ms:.4f} ms/frame")
+    if not args.pretrace:
+        warp.set_pretrace(sets=0)
 
     # ---- pass A: throughput, frames back to back (no per-frame host sync);
     # an event at each frame's end gives the frame-to-frame completion times
@@ -154,6 +156,8 @@ def run_gpu(args, name):
         if "tuned_warp" in res:
             t = res["tuned_warp"]
             A.apply(c, t)
+        if not args.pretrace:
+            c.set_pretrace(sets=0)
         for f in range(args.warmup):
             c.frame_warp(params[f])
         c.sync()
@@ -609,6 +613,47 @@ def run_cpu_baseline(name, n_warp_frames, n_full_frames, threads):
                 t=t_warp + t_full, frames=n_warp_frames)
 
 
+LONE_FRAME, LONE_PIXEL, LONE_TRIPS = 25, (216, 827), 3450  # flythrough_2048: the longest ray of frame 25
+
+
+def run_lone_ray(args):
+    """The latency of one long ray: the longest primary ray of frame 25 of
+    the flythrough (3450 loop trips) launched alone, then its 8x8 block (64
+    long rays in one wave), 20 launches each, one persistent workgroup per
+    CU.  Wall time per launch + sync here; the kernel time comes from a
+    rocprofv3 kernel trace of this run."""
+    import svo_amd as S
+    from svo_amd import workloads as W
+
+    name = "flythrough_2048"
+    words, root, depth, _ = W.octree(name)
+    rx, ry = W.WORKLOADS[name]["res"]
+    p = W.frame_params(name, W.path(name, LONE_FRAME + 1), LONE_FRAME)
+    out = {}
+    with S.Context(0) as c:
+        c.upload_octree(words, root, depth)
+        c.resize(rx, ry)
+        c.set_tuning(-1, 1, -1, 0)
+        x, y = LONE_PIXEL
+        for label, rect in (("lone_ray", (x, y, 1, 1)), ("block_8x8", (x & ~7, y & ~7, 8, 8))):
+            for _ in range(3):
+                c.raycast_rect(p, *rect)
+            c.sync()
+            c.ray_stats(reset=True)
+            ts = []
+            for _ in range(20):
+                t0 = time.perf_counter()
+                c.raycast_rect(p, *rect)
+                c.sync()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            n, longest = c.ray_stats(reset=True)
+            out[label] = {"rect": list(rect), "longest_trips": int(longest), "wall_ms_p50": round(pct(ts, 50), 4)}
+            log(f"[bench] {label}: {out[label]}")
+    if out["lone_ray"]["longest_trips"] != LONE_TRIPS:
+        raise SystemExit(f"lone ray: {out['lone_ray']['longest_trips']} trips, expected {LONE_TRIPS}")
+    print(json.dumps({"lone_ray": out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -619,6 +664,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=280, help="warp frames in the CPU sample")
     ap.add_argument("--cpu-full-frames", type=int, default=48, help="full frames in the CPU sample")
     ap.add_argument("--no-tune", dest="tune", action="store_false", help="skip the ray-kernel autotuner")
+    ap.add_argument("--no-pretrace", dest="pretrace", action="store_false",
+                    help="warp frames without a pretrace (single stream; for counter runs, which serialise dispatches)")
     ap.add_argument("--no-learned", dest="learned", action="store_false", help="skip the learned-fill leg")
     ap.add_argument("--max-cnn-holes", type=int, default=64)
     ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the batched-views leg")
@@ -627,8 +674,9 @@ def main():
     ap.add_argument("--no-stress", dest="stress", action="store_false",
                     help="skip the divergence-stress leg (depth-12 cave)")
     ap.add_argument("--batch-warmup", type=int, default=8)
-    ap.add_argument("--legs", default="all", choices=["all", "warp"],
-                    help="warp: only the warp-frame legs (for profiling the warp kernels)")
+    ap.add_argument("--legs", default="all", choices=["all", "warp", "lone"],
+                    help="warp: only the warp-frame legs (for profiling the warp kernels); lone: only the "
+                         "single-long-ray launches")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); sharding across GPUs is a "
@@ -636,6 +684,9 @@ def main():
         sys.exit(2)
     if args.warmup < 2:
         raise SystemExit("--warmup must be >= 2 (frames 0-1 are full clears)")
+    if args.legs == "lone":
+        run_lone_ray(args)
+        return
 
     g = run_gpu(args, args.workload)
     curve = None

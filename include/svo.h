@@ -359,7 +359,12 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * sample per pixel, as the warp does) and then g, and traces the 2x2 quads
  * no sample reaches -- the hole quads frame g's warp can leave -- in the 8x8
  * blocks whose last traced rays include one of at least long_trips trips,
- * with the refresh rectangle.  A frame with a pretrace traces its rays on
+ * with the refresh rectangle; for a learned frame (svo_frame_warp_learned)
+ * also the gate's rays as the prediction sees them: the residual holes of
+ * tiles with more than max_cnn_holes of them, and with the depth rule
+ * (svo_set_gate_depth) the pixels where a sample beyond the limit lands and
+ * the residual holes of their tiles.
+ * A frame with a pretrace traces its rays on
  * none of the pretrace's lists as soon as the lists are made, and copies
  * the pretraced results once the pretrace has completed.  A
  * frame whose frames g-k .. g-1 did not all pass through svo_frame_warp or
@@ -449,12 +454,12 @@ int svo_cnn_set_fused(svo_cnn* cnn, int mode);
  * colours everywhere. */
 int svo_frame_warp_learned(svo_ctx* ctx, svo_cnn* cnn, const svo_frame_params* p, int max_cnn_holes);
 /* The gate's confidence rule for reprojected pixels (learned frames; no
- * counterpart in the reference): a 16x16 tile holding a reprojected pixel
- * whose camera depth exceeds max_depth world units (where a pixel spans
- * several voxels, so the warped sample's colour is rarely the one the
- * pixel's ray meets) is raycast whole in the frame's ray launch -- its
- * residual holes and its reprojected pixels outside the refresh rectangle.
- * 0 (default) = off.  Trades frames/s for PSNR against full raycasting. */
+ * counterpart in the reference): a reprojected pixel outside the refresh
+ * rectangle whose camera depth exceeds max_depth world units (where a pixel
+ * spans several voxels, so the warped sample's colour is rarely the one the
+ * pixel's ray meets) is raycast in the frame's ray launch, and so are the
+ * residual holes of its 16x16 tile (the CNN gets no such tile).  0
+ * (default) = off.  Trades frames/s for PSNR against full raycasting. */
 int svo_set_gate_depth(svo_ctx* ctx, float max_depth);
 /* Gate outcome of the last svo_frame_warp_learned (blocking): tiles sent to
  * the CNN and pixels sent back to raycasting (residual holes and, with the

@@ -1023,7 +1023,7 @@ void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes, floa
             for (int i = 0; i < 16; i++) {
                 if (x0 + i >= rx || y0 + j >= ry) continue;
                 size_t o = (size_t)(y0 + j) * rx + x0 + i;
-                if (resid[o] || (far && proj[o]))
+                if (resid[o] || proj[o] == 2)
                     list[n++] = (uint32_t)(x0 + i) | ((uint32_t)(y0 + j) << 16);
             }
     }

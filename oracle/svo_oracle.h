@@ -156,7 +156,7 @@ void orc_frame_full(orc_state* s, const orc_params* p);
  * list nor the refresh rectangle traces -- have them raycast with the frame's
  * hole and tile rays; with max_depth > 0 a tile holding a reprojected pixel
  * (outside the rectangle) of camera depth above max_depth has its residual
- * holes and all those reprojected pixels raycast */
+ * holes and every such far reprojected pixel raycast */
 void orc_frame_warp_gated(orc_state* s, const orc_params* p, int max_holes, float max_depth);
 
 /* exhaustive-check helpers */

@@ -145,6 +145,7 @@ struct svo_ctx {
     unsigned long long* d_pred_keys = nullptr;  // per set: N nearest-sample keys (all ones between uses)
     float4* d_pred_xyz = nullptr;               // per set: N positions of an intermediate camera
     uint8_t* d_pred_cov = nullptr;              // per set: N coverage bytes (zero between uses)
+    uint8_t* d_pred_deep = nullptr;             // per set: N bytes, a sample beyond the gate's depth limit (zero between uses)
     int pred_alloc_sets = 0;
     int pre_set = 0;
     uint32_t pre_id = 0;

@@ -1046,11 +1046,11 @@ __global__ void k_gate_reset(View v, const View* views)
 // full 16x16 blocks) nor the refresh rectangle will trace.  0 < residual <=
 // max_cnn -> the tile goes to the CNN; more -> its residual pixels join the
 // frame's ray launch as a second list.  With a depth limit (gate_depth > 0)
-// a tile also counts its reprojected pixels (outside the refresh rectangle)
-// whose camera depth (coordinate w, written by the warp) exceeds the limit:
-// where a pixel spans several voxels, the warped sample is one of them and
-// its colour rarely the one the pixel's ray meets, so such a tile is raycast
-// whole -- its residual holes and its reprojected pixels join the list.  A
+// a tile also counts its far pixels: reprojected pixels (outside the refresh
+// rectangle) whose camera depth (coordinate w, written by the warp) exceeds
+// the limit -- where a pixel spans several voxels, the warped sample is one
+// of them and its colour rarely the one the pixel's ray meets.  A tile with
+// a far pixel sends its residual holes and its far pixels to the list.  A
 // workgroup handles GATE_TILES tiles (each wave two, lane = 4 pixels) and
 // appends its CNN tiles and gate pixels with one atomic per list, so the
 // counters see few, uncontended atomics.
@@ -1068,11 +1068,11 @@ __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max
     const uint32_t max_tiles = (uint32_t)n_tiles;
     const int fx = (res_x / 16) * 16, fy = (res_y / 16) * 16;
     const bool depth_gate = V.gate_depth > 0.0f;
-    // per pixel: residual hole; reprojected pixel outside the rectangle
+    // per pixel: residual hole; far reprojected pixel outside the rectangle
     unsigned long long m[2][4], mp[2][4];
     for (int h = 0; h < 2; h++) {
         const int lt = wave * 2 + h, t = blockIdx.x * GATE_TILES + lt;
-        uint32_t n = 0, np = 0, nf = 0;
+        uint32_t n = 0, nf = 0;
         for (int j = 0; j < 4; j++) {
             const int i = lane + 64 * j;
             const int x = (t % tiles_x) * 16 + (i & 15), y = (t / tiles_x) * 16 + (i >> 4);
@@ -1083,17 +1083,15 @@ __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max
             bool hole = in && w == HOLE && !in_rect;
             if (hole && x < fx && y < fy && quad_is_hole(V.screen, res_x, x & ~1, y & ~1))
                 hole = false;  // in the hole list
-            const bool proj = depth_gate && in && w != HOLE && !in_rect;
-            const bool far = proj && V.coords[o].w > V.gate_depth;
+            const bool far = depth_gate && in && w != HOLE && !in_rect && V.coords[o].w > V.gate_depth;
             m[h][j] = __ballot(hole);
-            mp[h][j] = __ballot(proj);
+            mp[h][j] = __ballot(far);
             n += (uint32_t)__popcll(m[h][j]);
-            np += (uint32_t)__popcll(mp[h][j]);
-            nf += (uint32_t)__popcll(__ballot(far));
+            nf += (uint32_t)__popcll(mp[h][j]);
         }
         if (lane == 0) {
             s_n[lt] = n;
-            s_proj[lt] = nf ? np : 0u;  // a tile with a far pixel sends all its reprojected pixels too
+            s_proj[lt] = nf;  // a tile with far pixels sends them (and all its residual holes)
         }
     }
     __syncthreads();
@@ -1127,9 +1125,8 @@ __global__ __launch_bounds__(256) void k_gate(View v, const View* views, int max
             if (lane == 0 && oc < max_tiles) tiles[oc] = (uint32_t)t;
             continue;
         }
-        const bool whole = s_proj[lt] != 0u;
         for (int j = 0; j < 4; j++) {
-            const unsigned long long mm = m[h][j] | (whole ? mp[h][j] : 0ull);
+            const unsigned long long mm = m[h][j] | mp[h][j];  // mp: far pixels (none without the depth rule)
             if ((mm >> lane) & 1ull) {
                 const int i = lane + 64 * j;
                 const int x = (t % tiles_x) * 16 + (i & 15), y = (t / tiles_x) * 16 + (i >> 4);
@@ -1794,10 +1791,12 @@ static void free_pre(svo_ctx* c)
     (void)hipFree(c->d_pred_keys);
     (void)hipFree(c->d_pred_xyz);
     (void)hipFree(c->d_pred_cov);
+    (void)hipFree(c->d_pred_deep);
     c->d_snap_xyz = nullptr;
     c->d_pred_keys = nullptr;
     c->d_pred_xyz = nullptr;
     c->d_pred_cov = nullptr;
+    c->d_pred_deep = nullptr;
     c->pred_alloc_sets = 0;
     for (int i = 0; i < kSnapSlots; i++) c->snap_frame[i] = c->cam_frame[i] = -1;
     c->snap_pending = -1;
@@ -1812,12 +1811,13 @@ static int ensure_pred(svo_ctx* c)
         if (!c->ev_snap[i]) HIPCHECK(hipEventCreateWithFlags(&c->ev_snap[i], hipEventDisableTiming));
     if (hipMalloc(&c->d_snap_xyz, kSnapSlots * N * 16) != hipSuccess ||
         hipMalloc(&c->d_pred_keys, K * N * 8) != hipSuccess || hipMalloc(&c->d_pred_xyz, K * N * 16) != hipSuccess ||
-        hipMalloc(&c->d_pred_cov, K * N) != hipSuccess) {
+        hipMalloc(&c->d_pred_cov, K * N) != hipSuccess || hipMalloc(&c->d_pred_deep, K * N) != hipSuccess) {
         free_pre(c);
         return svo_fail(SVO_ERR_NOMEM, "predicted-hole buffers for %zu sets of %zu pixels", K, N);
     }
     HIPCHECK(hipMemset(c->d_pred_keys, 0xff, K * N * 8));
     HIPCHECK(hipMemset(c->d_pred_cov, 0, K * N));
+    HIPCHECK(hipMemset(c->d_pred_deep, 0, K * N));
     HIPCHECK(hipDeviceSynchronize());
     c->pred_alloc_sets = (int)K;
     for (int i = 0; i < kSnapSlots; i++) c->snap_frame[i] = -1;
@@ -2018,8 +2018,9 @@ static PredCam pred_cam(const svo_frame_params* p)
 }
 
 // target pixel of position q under camera k (-1: behind, too near, a NaN
-// hole marker, or off-screen incl. the last column and row), and its depth key
-__device__ __forceinline__ int pred_target(float4 q, const PredCam& k, int res_x, int res_y, uint32_t* zkey)
+// hole marker, or off-screen incl. the last column and row), and its camera
+// depth hz (world units, the coordinate w the warp writes)
+__device__ __forceinline__ int pred_target(float4 q, const PredCam& k, int res_x, int res_y, float* hz_out)
 {
     const float qx = q.x - k.m0x, qy = q.y - k.m0y, qz = q.z - k.m0z;
     const float hz = dot3(qx, qy, qz, k.r20, k.r21, k.r22);
@@ -2029,7 +2030,7 @@ __device__ __forceinline__ int pred_target(float4 q, const PredCam& k, int res_x
     const float sfx = (hx * (float)res_y + 0.0f) / hz + (float)res_x / 2 - 0.0f;
     const float sfy = (hy * (float)res_y + 0.0f) / hz + (float)res_y / 2 - 0.0f;
     if (!(sfx >= 0.0f && sfy >= 0.0f && sfx < (float)(res_x - 1) && sfy < (float)(res_y - 1))) return -1;
-    *zkey = (uint32_t)(int)(hz * 1000.0f) & 0xffffffu;
+    *hz_out = hz;
     return (int)sfy * res_x + (int)sfx;
 }
 
@@ -2039,20 +2040,33 @@ __global__ __launch_bounds__(256) void k_pred_scatter(const float4* __restrict__
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
-    uint32_t z;
-    const int t = pred_target(src[i], k, res_x, res_y, &z);
+    float hz;
+    const int t = pred_target(src[i], k, res_x, res_y, &hz);
     if (t < 0) return;
+    const uint32_t z = (uint32_t)(int)(hz * 1000.0f) & 0xffffffu;
     const unsigned long long key = ((unsigned long long)z << 32) | i;
     if (keys[t] > key) atomicMin(&keys[t], key);
 }
 
+// a sample reaches pixel t of the final camera: coverage byte, and with a
+// depth limit (learned frames' gate) a second byte where it lies beyond it
+__device__ __forceinline__ void pred_mark(float4 q, const PredCam& fin, int res_x, int res_y, float deep_z,
+                                          uint8_t* cov, uint8_t* deep)
+{
+    float hz;
+    const int t = pred_target(q, fin, res_x, res_y, &hz);
+    if (t < 0) return;
+    cov[t] = 1;
+    if (deep_z > 0.0f && hz > deep_z) deep[t] = 1;
+}
+
 // the winners of k_pred_scatter (keys back to all ones): their positions
-// into dst (NaN where none), or, with dst == null, projected through the
-// final camera `fin` into the coverage bytes
+// into dst (NaN where none), or, with dst == null, marked under the final
+// camera `fin`
 __global__ __launch_bounds__(256) void k_pred_resolve(unsigned long long* __restrict__ keys,
                                                       const float4* __restrict__ src, float4* __restrict__ dst,
-                                                      PredCam fin, int res_x, int res_y, uint32_t N,
-                                                      uint8_t* __restrict__ cov)
+                                                      PredCam fin, int res_x, int res_y, uint32_t N, float deep_z,
+                                                      uint8_t* __restrict__ cov, uint8_t* __restrict__ deep)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
@@ -2067,51 +2081,58 @@ __global__ __launch_bounds__(256) void k_pred_resolve(unsigned long long* __rest
         dst[i] = q;
         return;
     }
-    uint32_t z;
-    const int t = pred_target(q, fin, res_x, res_y, &z);
-    if (t >= 0) cov[t] = 1;
+    pred_mark(q, fin, res_x, res_y, deep_z, cov, deep);
 }
 
-// every sample of src projected through the final camera into the coverage bytes
+// every sample of src marked under the final camera
 __global__ __launch_bounds__(256) void k_pred_cover(const float4* __restrict__ src, PredCam fin, int res_x, int res_y,
-                                                    uint32_t N, uint8_t* __restrict__ cov)
+                                                    uint32_t N, float deep_z, uint8_t* __restrict__ cov,
+                                                    uint8_t* __restrict__ deep)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
-    uint32_t z;
-    const int t = pred_target(src[i], fin, res_x, res_y, &z);
-    if (t >= 0) cov[t] = 1;
+    pred_mark(src[i], fin, res_x, res_y, deep_z, cov, deep);
 }
 
 // Pretrace list, pass 1 of the predicted-hole mode (layout of k_pre_scan: a
-// thread per column of 8 pixels of an 8x8 block).  A pixel is traced ahead
-// when it lies in the refresh rectangle, or in a 2x2 quad (even x and y, inside
-// the whole 16x16 blocks the hole list covers, kernel.cl:262-272) that no
+// thread per column of 8 pixels of an 8x8 block; a workgroup covers 64x32
+// pixels, i.e. 4x2 whole 16x16 gate tiles).  A pixel is traced ahead when it
+// lies in the refresh rectangle, or in a 2x2 quad (even x and y, inside the
+// whole 16x16 blocks the hole list covers, kernel.cl:262-272) that no
 // predicted sample reaches, in a block whose longest known ray has at least
 // `trips` trips -- unless the block's longest known ray has at least `whole`
-// (> 0) trips: part 0 traces those blocks whole.  bkt[pixel] = its bucket, or
-// kPreNone; the coverage bytes are cleared for the next use.
+// (> 0) trips: part 0 traces those blocks whole.  Learned frames (max_cnn >=
+// 0) also predict the gate (k_gate): a tile's residual holes are the pixels
+// outside the rectangle no sample reaches and no predicted hole quad holds;
+// a tile with more than max_cnn of them has them traced ahead, and with the
+// depth rule (deep != null) a tile where samples beyond the depth limit land
+// outside the rectangle has them and its residual holes traced ahead.  bkt[pixel] = its bucket, or
+// kPreNone; the coverage (and depth) bytes are cleared for the next use.
 __global__ __launch_bounds__(256) void k_pre_scan_pred(const uint32_t* __restrict__ known, uint8_t* __restrict__ cov,
-                                                       int res_x, int res_y, int qx_end, int qy_end, int ax, int ay,
-                                                       int ex, int ey, uint32_t trips, uint32_t whole,
-                                                       uint32_t* __restrict__ blk, uint8_t* __restrict__ bkt,
-                                                       uint32_t* __restrict__ hdr)
+                                                       uint8_t* __restrict__ deep, int max_cnn, int res_x, int res_y,
+                                                       int qx_end, int qy_end, int ax, int ay, int ex, int ey,
+                                                       uint32_t trips, uint32_t whole, uint32_t* __restrict__ blk,
+                                                       uint8_t* __restrict__ bkt, uint32_t* __restrict__ hdr)
 {
     __shared__ uint32_t hist[kPreBuckets];
+    __shared__ uint32_t s_res[8], s_far[8];  // per gate tile of the workgroup
     const int t = threadIdx.x;
     if (t < kPreBuckets) hist[t] = 0;
+    if (t < 8) s_res[t] = s_far[t] = 0;
     __syncthreads();
     const int x = blockIdx.x * 64 + (t & 63);
     const int by = blockIdx.y * 4 + (t >> 6);
     const int y0 = by * 8;
+    const int gt = ((t & 63) >> 4) + 4 * ((t >> 6) >> 1);  // this column's gate tile in the workgroup
     const bool col_ok = x < res_x && y0 < res_y;
     const int h = col_ok ? min(8, res_y - y0) : 0;
     uint32_t kn[8];
-    uint32_t m = 0, cv = 0;  // cv bit j: pixel (x, y0 + j) is reached
+    uint32_t m = 0, cv = 0, dp = 0;  // bit j: pixel (x, y0 + j) is reached / reached beyond the depth limit
     for (int j = 0; j < 8; j++) {
         const size_t o = (size_t)(y0 + j) * res_x + x;
         kn[j] = j < h ? known[o] : 0u;
         cv |= (j < h && cov[o]) ? 1u << j : 0u;
+        dp |= (deep && j < h && deep[o]) ? 1u << j : 0u;
         m = max(m, kn[j]);
     }
     const uint32_t ncv = (uint32_t)__shfl_xor((int)cv, 1);  // the quad's other column (x ^ 1)
@@ -2119,13 +2140,31 @@ __global__ __launch_bounds__(256) void k_pre_scan_pred(const uint32_t* __restric
     const bool long_blk = m >= trips;
     const bool part0 = whole > 0u && m >= whole;  // traced whole by part 0
     const bool qx_ok = (x | 1) < qx_end;
+    uint32_t quad = 0, res = 0, far = 0;  // bit j: predicted hole quad / residual hole / deep sample
+    for (int j = 0; j < h; j++) {
+        const int y = y0 + j;
+        const uint32_t qm = 3u << (j & 6);  // the quad's two rows
+        const bool in_rect = x >= ax && x < ax + ex && y >= ay && y < ay + ey;
+        const bool hq = qx_ok && (y | 1) < qy_end && ((cv | ncv) & qm) == 0u;
+        const bool reached = (cv >> j) & 1u;
+        quad |= hq ? 1u << j : 0u;
+        res |= (!reached && !hq && !in_rect) ? 1u << j : 0u;
+        far |= (reached && ((dp >> j) & 1u) && !in_rect) ? 1u << j : 0u;
+    }
+    if (max_cnn >= 0) {
+        if (res) atomicAdd(&s_res[gt], (uint32_t)__popc(res));
+        if (far) s_far[gt] = 1u;
+    }
+    __syncthreads();
+    const bool gate_far = max_cnn >= 0 && s_far[gt] != 0u;
+    const bool gate_res = max_cnn >= 0 && (gate_far || (int)s_res[gt] > max_cnn);
     uint32_t n = 0;
     for (int j = 0; j < h; j++) {
         const int y = y0 + j;
         const size_t o = (size_t)y * res_x + x;
-        const uint32_t qm = 3u << (j & 6);  // the quad's two rows
-        const bool hole_quad = qx_ok && (y | 1) < qy_end && ((cv | ncv) & qm) == 0u;
-        const bool take = !part0 && ((x >= ax && x < ax + ex && y >= ay && y < ay + ey) || (hole_quad && long_blk));
+        const bool in_rect = x >= ax && x < ax + ex && y >= ay && y < ay + ey;
+        const bool take = !part0 && (in_rect || (((quad >> j) & 1u) && long_blk) ||
+                                     (gate_res && ((res >> j) & 1u)) || (gate_far && ((far >> j) & 1u)));
         uint8_t b = kPreNone;
         if (take) {
             b = (uint8_t)trip_bucket(kn[j]);
@@ -2134,6 +2173,7 @@ __global__ __launch_bounds__(256) void k_pre_scan_pred(const uint32_t* __restric
         }
         bkt[o] = b;
         if ((cv >> j) & 1u) cov[o] = 0;
+        if ((dp >> j) & 1u) deep[o] = 0;
     }
     for (int o = 1; o < 8; o <<= 1) n += (uint32_t)__shfl_xor((int)n, o);
     const int nbx = (res_x + 7) >> 3;
@@ -2169,7 +2209,8 @@ static bool predict_ready(const svo_ctx* c, const svo_frame_params* p)
 // blocks only, or predicted holes), then the ray launch into the set's
 // results, stamped `id`; records ev_pre[q]
 enum PrePart { PRE_LONG, PRE_WHOLE, PRE_PREDICT };
-static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part, PrePart kind, uint32_t id)
+static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part, PrePart kind, uint32_t id,
+                         int gate_max)
 {
     const int q = 2 * s + part;
     const size_t N = c->N;
@@ -2199,11 +2240,14 @@ static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part,
         unsigned long long* keys = c->d_pred_keys + (size_t)s * N;
         float4* inter = c->d_pred_xyz + (size_t)s * N;
         uint8_t* cov = c->d_pred_cov + (size_t)s * N;
+        uint8_t* deep = c->d_pred_deep + (size_t)s * N;
+        // learned frames with the gate's depth rule: mark where a sample beyond the limit lands
+        const float deep_z = gate_max >= 0 && c->gate_depth > 0.0f ? c->gate_depth : 0.0f;
         const float4* cur = c->d_snap_xyz + (size_t)src_slot * N;
         const PredCam fin = pred_cam(p);
         if (k == 1) {
             hipLaunchKernelGGL(k_pred_cover, dim3(nb), dim3(256), 0, st, cur, fin, c->res_x, c->res_y, (uint32_t)N,
-                               cov);
+                               deep_z, cov, deep);
         }
         for (int j = 1; j < k; j++) {
             const PredCam mid = pred_cam(&c->cam_hist[(g - k + j) % kSnapSlots]);
@@ -2213,15 +2257,15 @@ static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part,
             // (k == 3: the intermediate positions go to `inter`, which the
             // next scatter reads; the first scatter read the snapshot)
             hipLaunchKernelGGL(k_pred_resolve, dim3(nb), dim3(256), 0, st, keys, cur, last ? nullptr : inter, fin,
-                               c->res_x, c->res_y, (uint32_t)N, cov);
+                               c->res_x, c->res_y, (uint32_t)N, deep_z, cov, deep);
             cur = inter;
         }
         HIPCHECK(hipGetLastError());
         // blocks part 0 traces whole are left to it
         const uint32_t whole = c->pre_early ? c->pre_whole_trips : 0u;
-        hipLaunchKernelGGL(k_pre_scan_pred, grid, dim3(256), 0, st, (const uint32_t*)c->d_known, cov, c->res_x,
-                           c->res_y, (int)(c->res_x / 16 * 16), (int)(c->res_y / 16 * 16), ax, ay, ex, ey,
-                           c->pre_trips, whole, blk, bkt, hdr);
+        hipLaunchKernelGGL(k_pre_scan_pred, grid, dim3(256), 0, st, (const uint32_t*)c->d_known, cov,
+                           deep_z > 0.0f ? deep : nullptr, gate_max, c->res_x, c->res_y, (int)(c->res_x / 16 * 16),
+                           (int)(c->res_y / 16 * 16), ax, ay, ex, ey, c->pre_trips, whole, blk, bkt, hdr);
     } else {
         const uint32_t trips = kind == PRE_WHOLE ? c->pre_whole_trips : c->pre_trips;
         hipLaunchKernelGGL(k_pre_scan, grid, dim3(256), 0, st, (const uint32_t*)c->d_known, c->res_x, c->res_y, ax,
@@ -2258,15 +2302,15 @@ static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part,
 // snapshot it projects).  Parts write different pixels, except where the
 // known trips change between their list passes: a pixel traced by both gets
 // the same result twice.
-static int pretrace(svo_ctx* c, const svo_frame_params* p)
+static int pretrace(svo_ctx* c, const svo_frame_params* p, int gate_max)
 {
     RC(ensure_pre(c));
     const int s = (int)(c->pre_frames % (uint32_t)c->pre_sets);
     const uint32_t id = c->pre_frames + 1u;
     const bool pred = predict_ready(c, p);
     c->pre_early = pred && c->pre_whole_trips > 0;
-    if (c->pre_early) RC(pretrace_part(c, p, s, 0, PRE_WHOLE, id));
-    RC(pretrace_part(c, p, s, 1, pred ? PRE_PREDICT : PRE_LONG, id));
+    if (c->pre_early) RC(pretrace_part(c, p, s, 0, PRE_WHOLE, id, gate_max));
+    RC(pretrace_part(c, p, s, 1, pred ? PRE_PREDICT : PRE_LONG, id, gate_max));
     c->pre_frames++;
     c->pre_active = true;
     c->pre_set = s;
@@ -2309,7 +2353,7 @@ extern "C" int svo_frame_warp(svo_ctx* c, const svo_frame_params* p)
     if (graph && !c->graph[0]) RC(build_graph(c));
     float4* snap = snap_begin(c, p);
     frame_begin(c);
-    if (pre) RC(pretrace(c, p));
+    if (pre) RC(pretrace(c, p, -1));  // warp frames: no gate
     int rc;
     if (graph) {
         rc = frame_warp_graph(c, p, snap);
@@ -2365,7 +2409,7 @@ extern "C" int svo_frame_warp_learned(svo_ctx* c, svo_cnn* cnn, const svo_frame_
     const bool pre = want_pretrace(c, p);
     snap_begin(c, p);
     frame_begin(c);
-    if (pre) RC(pretrace(c, p));
+    if (pre) RC(pretrace(c, p, max_cnn_holes));
     const int rc = learned_body(c, cnn, p, max_cnn_holes);
     if (!rc) snap_end(c, p);
     pretrace_done(c);

@@ -343,6 +343,50 @@ def test_pretrace_predicted_holes_flythrough_vs_oracle():
         o.close()
 
 
+@pytest.mark.parametrize("max_cnn_holes,gate_depth", [(64, 0.0), (0, 0.0), (64, 40.0)])
+def test_pretrace_predicted_gate_learned_frames(scene1, max_cnn_holes, gate_depth):
+    """Learned frames whose pretrace also predicts the gate (tiles with more
+    than max_cnn_holes residual holes, and with the depth rule the tiles
+    holding far reprojected pixels) equal learned frames without a pretrace;
+    the pretrace carries most of the gate's rays."""
+    from svo_amd import learned as L
+    words, root = scene1
+    rx, ry = CONFIG1["res"]
+    pr = path(CONFIG1, 16)
+    cnn = S.Cnn(0)
+    L.load(cnn, L.WEIGHTS)
+    a, b = S.Context(0), S.Context(0)
+    try:
+        for c in (a, b):
+            c.upload_octree(words, root, CONFIG1["depth"])
+            c.resize(rx, ry)
+            c.set_gate_depth(gate_depth)
+        a.set_pretrace(4, 1, only_queued=0, predict=2)
+        b.set_pretrace(0)
+        in_frame, total = 0, 0
+        for f in range(16):
+            print(f"predicted gate, learned frame {f} max_cnn_holes {max_cnn_holes} depth {gate_depth}", flush=True)
+            p = params(CONFIG1, pr, f)
+            a.ray_stats(reset=True)
+            a.pretrace_stats(reset=True)
+            a.frame_warp_learned(cnn, p, max_cnn_holes)
+            b.frame_warp_learned(cnn, p, max_cnn_holes)
+            a.sync()
+            rays, _ = a.ray_stats()
+            _, merged, _ = a.pretrace_stats()
+            if f >= 4:
+                in_frame += rays
+                total += rays + merged
+            if f % 3 == 1 or f == 15:
+                _same(a, b, f"learned frame {f}")
+        print("rays traced in the frame", in_frame, "of", total, flush=True)
+        assert in_frame < 0.5 * total
+    finally:
+        a.close()
+        b.close()
+        cnn.close()
+
+
 def test_pretrace_predicted_holes_learned_frames(scene1):
     """Learned frames with predicted holes equal learned frames without a
     pretrace."""

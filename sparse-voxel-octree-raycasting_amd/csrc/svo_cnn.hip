@@ -124,24 +124,32 @@ __global__ __launch_bounds__(64) void k_conv(const bf16* __restrict__ X, const b
     }
 }
 
-// dW[tap][co][ci] += sum_p dZ[p][co] X[p + tap - 1][ci]; db[co] += sum_p dZ[p][co].
-// grid = (crops, 9 taps); one wave loops over the crop's pixels, K = 16 pixels
-// per MFMA (A = dZ^T: co x pixels, B = shifted X: pixels x ci).
+// dW partial of crop b: sum over the crop's pixels p of dZ[p][co] X[p + tap
+// - 1][ci].  grid = (crops, 9 taps), DW_SLICES waves per workgroup: wave s
+// loops over pixel slice s (a multiple of 16 pixels), K = 16 pixels per MFMA
+// (A = dZ^T: co x pixels, B = shifted X: pixels x ci); the waves' sums are
+// added in slice order through LDS and written to part[b][tap][co][ci];
+// k_sum_parts adds the crops' partials in crop order, so the gradient does
+// not depend on scheduling.
+constexpr int DW_SLICES = 16;
 template <int CIN, int COUT>
-__global__ __launch_bounds__(64) void k_conv_dw(const bf16* __restrict__ X, const bf16* __restrict__ dZ,
-                                                float* dW, int H, int W)
+__global__ __launch_bounds__(64 * DW_SLICES) void k_conv_dw(const bf16* __restrict__ X, const bf16* __restrict__ dZ,
+                                                            float* __restrict__ part, int H, int W)
 {
-    const int b = blockIdx.x, tap = blockIdx.y;
+    __shared__ float s_acc[DW_SLICES][COUT * CIN];
+    const int b = blockIdx.x, tap = blockIdx.y, sl = threadIdx.x >> 6;
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int lane = threadIdx.x, row = lane & 15, kq = (lane >> 4) * 4;
+    const int lane = threadIdx.x & 63, row = lane & 15, kq = (lane >> 4) * 4;
     constexpr int MT = COUT / 16, NT = CIN / 16;
     f4v acc[MT][NT];
     for (int m = 0; m < MT; m++)
         for (int n = 0; n < NT; n++) acc[m][n] = f4v{0.f, 0.f, 0.f, 0.f};
-    const int npix = H * W;
+    const int npix = H * W;  // a multiple of 16
+    const int len = (npix / 16 + DW_SLICES - 1) / DW_SLICES * 16;
+    const int kb = min(npix, sl * len), ke = min(npix, kb + len);
     const bf16* dz = dZ + (size_t)b * npix * COUT;
     const bf16* xb = X + (size_t)b * npix * CIN;
-    for (int k0 = 0; k0 < npix; k0 += 16) {
+    for (int k0 = kb; k0 < ke; k0 += 16) {
         s4v afr[MT], bfr[NT];
         for (int m = 0; m < MT; m++) {
             short tmp[4];
@@ -164,19 +172,66 @@ __global__ __launch_bounds__(64) void k_conv_dw(const bf16* __restrict__ X, cons
         for (int n = 0; n < NT; n++)
             for (int i = 0; i < 4; i++) {
                 const int co = m * 16 + kq + i, ci = n * 16 + row;
-                atomicAdd(&dW[((size_t)tap * COUT + co) * CIN + ci], acc[m][n][i]);
+                s_acc[sl][co * CIN + ci] = acc[m][n][i];
             }
+    __syncthreads();
+    float* dw = part + ((size_t)b * 9 + tap) * COUT * CIN;
+    for (int e = threadIdx.x; e < COUT * CIN; e += 64 * DW_SLICES) {
+        float t = 0.f;
+        for (int k = 0; k < DW_SLICES; k++) t += s_acc[k][e];
+        dw[e] = t;
+    }
 }
 
-// db[co] += sum_p dZ[p][co]; each block sums 4096 pixels
+// db partial of block b: part[b][co] = sum over its DB_PIXELS pixels of
+// dZ[p][co] (each thread a fixed stride of pixels, then the threads of a
+// channel in order)
+constexpr int DB_PIXELS = 256;
 template <int COUT>
-__global__ __launch_bounds__(256) void k_conv_db(const bf16* __restrict__ dZ, float* db, int npix)
+__global__ __launch_bounds__(256) void k_conv_db(const bf16* __restrict__ dZ, float* __restrict__ part, int npix)
 {
+    __shared__ float s_sum[256];
     const int co = threadIdx.x % COUT, lane_p = threadIdx.x / COUT;
-    const int p0 = blockIdx.x * 4096, p1 = min(p0 + 4096, npix);
+    const int p0 = blockIdx.x * DB_PIXELS, p1 = min(p0 + DB_PIXELS, npix);
     float s = 0.f;
     for (int p = p0 + lane_p; p < p1; p += 256 / COUT) s += bf2f(dZ[(size_t)p * COUT + co]);
-    atomicAdd(&db[co], s);
+    s_sum[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < COUT) {
+        float t = 0.f;
+        for (int j = 0; j < 256 / COUT; j++) t += s_sum[j * COUT + threadIdx.x];
+        part[(size_t)blockIdx.x * COUT + threadIdx.x] = t;
+    }
+}
+
+// out[e] for many partials and few elements (one workgroup per element):
+// thread t sums part[t], part[t + 256], ... in order, then a fixed pairwise
+// tree over the 256 threads
+__global__ __launch_bounds__(256) void k_sum_parts_tree(const float* __restrict__ part, int n_parts, int n,
+                                                        float* __restrict__ out)
+{
+    __shared__ float s[256];
+    const int e = blockIdx.x, t = threadIdx.x;
+    float v = 0.f;
+    for (int p = t; p < n_parts; p += 256) v += part[(size_t)p * n + e];
+    s[t] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) s[t] += s[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[e] = s[0];
+}
+
+// out[e] = sum over p = 0 .. n_parts-1, in order, of part[p][e]
+__global__ __launch_bounds__(256) void k_sum_parts(const float* __restrict__ part, int n_parts, int n,
+                                                   float* __restrict__ out)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.f;
+    for (int p = 0; p < n_parts; p++) s += part[(size_t)p * n + e];
+    out[e] = s;
 }
 
 // masked MSE on RGB: loss += sum mask*(pred-t)^2 / (3*count); dZ = d loss/d pred
@@ -694,6 +749,8 @@ struct svo_cnn {
     bf16* xt = nullptr;    // explicit-mode tile inputs / outputs
     float* ot = nullptr;
     size_t cap_tiles = 0;
+    float* part = nullptr;  // gradient partials (per crop and tap, or per block)
+    size_t cap_part = 0;
 };
 
 #define RCC(x)             \
@@ -771,6 +828,7 @@ extern "C" int svo_cnn_destroy(svo_cnn* n)
     }
     (void)hipFree(n->loss);
     (void)hipFree(n->mask_count);
+    (void)hipFree(n->part);
     (void)hipFree(n->origins);
     (void)hipFree(n->xt);
     (void)hipFree(n->ot);
@@ -891,37 +949,57 @@ int cnn_forward_dev(svo_cnn* n, int B, int H, int W, const uint32_t* bcount = nu
     return SVO_OK;
 }
 
-// backward from n->out vs n->target/n->mask; gradients summed into L[].gw/gb
+// backward from n->out vs n->target/n->mask; gradients into L[].gw/gb
+// (per-crop and per-block partials added in a fixed order)
 int cnn_backward_dev(svo_cnn* n, int B, int H, int W)
 {
     const size_t pix = (size_t)B * H * W;
     const unsigned g = conv_grid(B, H, W);
-    for (auto& l : n->L) {
-        HIPCHECK(hipMemsetAsync(l.gw, 0, l.nw * 4, n->stream));
-        HIPCHECK(hipMemsetAsync(l.gb, 0, l.cout * 4, n->stream));
+    const unsigned gdb = (unsigned)((pix + DB_PIXELS - 1) / DB_PIXELS);
+    const size_t need = std::max((size_t)B * 9 * 32 * 32, (size_t)gdb * 32);
+    if (need > n->cap_part) {
+        (void)hipFree(n->part);
+        n->part = nullptr;
+        n->cap_part = 0;
+        if (hipMalloc(&n->part, need * 4) != hipSuccess) return svo_fail(SVO_ERR_NOMEM, "gradient partials");
+        n->cap_part = need;
     }
     HIPCHECK(hipMemsetAsync(n->loss, 0, 4, n->stream));
     hipLaunchKernelGGL(k_loss, dim3(nb(pix)), dim3(256), 0, n->stream, n->out, n->target, n->mask, n->dz16, n->loss,
                        (int)pix, n->mask_count);
-    const unsigned gdb = (unsigned)((pix + 4095) / 4096);
+    // weight and bias gradients of layer l from its input X and output gradient dZ
+    auto dwdb = [&](int l, const bf16* X, const bf16* dZ) {
+        CnnLayer& L = n->L[l];
+        const dim3 gdw((unsigned)B, 9), bdw(64 * DW_SLICES);
+        if (l == 3)
+            hipLaunchKernelGGL((k_conv_dw<32, 16>), gdw, bdw, 0, n->stream, X, dZ, n->part, H, W);
+        else if (l == 0)
+            hipLaunchKernelGGL((k_conv_dw<16, 32>), gdw, bdw, 0, n->stream, X, dZ, n->part, H, W);
+        else
+            hipLaunchKernelGGL((k_conv_dw<32, 32>), gdw, bdw, 0, n->stream, X, dZ, n->part, H, W);
+        hipLaunchKernelGGL(k_sum_parts, dim3(nb(L.nw)), dim3(256), 0, n->stream, (const float*)n->part, B,
+                           (int)L.nw, L.gw);
+        if (l == 3)
+            hipLaunchKernelGGL((k_conv_db<16>), dim3(gdb), dim3(256), 0, n->stream, dZ, n->part, (int)pix);
+        else
+            hipLaunchKernelGGL((k_conv_db<32>), dim3(gdb), dim3(256), 0, n->stream, dZ, n->part, (int)pix);
+        hipLaunchKernelGGL(k_sum_parts_tree, dim3((unsigned)L.cout), dim3(256), 0, n->stream, (const float*)n->part,
+                           (int)gdb, L.cout, L.gb);
+    };
     // layer 4 (32 -> 16)
-    hipLaunchKernelGGL((k_conv_dw<32, 16>), dim3(B, 9), dim3(64), 0, n->stream, n->y[2], n->dz16, n->L[3].gw, H, W);
-    hipLaunchKernelGGL((k_conv_db<16>), dim3(gdb), dim3(256), 0, n->stream, n->dz16, n->L[3].gb, (int)pix);
+    dwdb(3, n->y[2], n->dz16);
     hipLaunchKernelGGL((k_conv<16, 32, 2>), dim3(g), dim3(64), 0, n->stream, n->dz16, n->L[3].wflip,
                        (const float*)nullptr, n->y[2], (void*)n->dz32a, H, W, B, (const uint32_t*)nullptr);
     // layer 3
-    hipLaunchKernelGGL((k_conv_dw<32, 32>), dim3(B, 9), dim3(64), 0, n->stream, n->y[1], n->dz32a, n->L[2].gw, H, W);
-    hipLaunchKernelGGL((k_conv_db<32>), dim3(gdb), dim3(256), 0, n->stream, n->dz32a, n->L[2].gb, (int)pix);
+    dwdb(2, n->y[1], n->dz32a);
     hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(g), dim3(64), 0, n->stream, n->dz32a, n->L[2].wflip,
                        (const float*)nullptr, n->y[1], (void*)n->dz32b, H, W, B, (const uint32_t*)nullptr);
     // layer 2
-    hipLaunchKernelGGL((k_conv_dw<32, 32>), dim3(B, 9), dim3(64), 0, n->stream, n->y[0], n->dz32b, n->L[1].gw, H, W);
-    hipLaunchKernelGGL((k_conv_db<32>), dim3(gdb), dim3(256), 0, n->stream, n->dz32b, n->L[1].gb, (int)pix);
+    dwdb(1, n->y[0], n->dz32b);
     hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(g), dim3(64), 0, n->stream, n->dz32b, n->L[1].wflip,
                        (const float*)nullptr, n->y[0], (void*)n->dz32a, H, W, B, (const uint32_t*)nullptr);
     // layer 1 (16 -> 32)
-    hipLaunchKernelGGL((k_conv_dw<16, 32>), dim3(B, 9), dim3(64), 0, n->stream, n->x0, n->dz32a, n->L[0].gw, H, W);
-    hipLaunchKernelGGL((k_conv_db<32>), dim3(gdb), dim3(256), 0, n->stream, n->dz32a, n->L[0].gb, (int)pix);
+    dwdb(0, n->x0, n->dz32a);
     HIPCHECK(hipGetLastError());
     return SVO_OK;
 }

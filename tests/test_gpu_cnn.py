@@ -66,6 +66,21 @@ def test_gradients_match_reference(cnn):
     assert np.all(g[:9 * 32 * 16].reshape(9, 32, 16)[:, :, 5:] == 0)
 
 
+def test_gradients_are_deterministic(cnn):
+    """The weight and bias gradients are summed from per-crop and per-block
+    partials in a fixed order: the same batch gives bit-identical gradients
+    every time (16 crops of 64x64, the training shape)."""
+    rng = np.random.default_rng(3)
+    bits, _ = _batch(rng, 16, 64, 64)
+    target = rng.uniform(0, 1, (16, 64, 64, 3)).astype(np.float32)
+    mask = (rng.random((16, 64, 64)) < 0.4).astype(np.float32)
+    g0, _ = cnn.gradients(bits, target, mask)
+    for _ in range(3):
+        g1, _ = cnn.gradients(bits, target, mask)
+        assert np.array_equal(g0.view(np.uint32), g1.view(np.uint32))
+    assert np.abs(g0).max() > 0
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_fused_tile_forward_matches_reference(mode):
     """The fused per-tile inference kernel (all four layers in LDS, layer 1 on

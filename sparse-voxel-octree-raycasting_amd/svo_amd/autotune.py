@@ -86,11 +86,11 @@ def _search(space, coordinate, measure):
 
 # warp frames also search the pretrace: its sets, its mode (pre_pred: 0 =
 # long blocks, k = holes predicted from frame g-k) and block threshold
-WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[5, 4, 3, 6], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384])
+WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
-LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[5, 4, 3, 6],
+LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[4, 3, 5],
                      pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384])
 
 

@@ -122,6 +122,7 @@ def run_gpu(args, name):
     rays_warp = rays_traced + pre_merged  # the frames' hole and tile rays
     cfg = warp.get_pretrace()
     res["pretrace"] = {"sets": cfg.sets, "predict": cfg.predict, "long_trips": cfg.long_trips, "wg_per_cu": cfg.wg_per_cu,
+                       "queue_skip": cfg.queue_skip,
                        "only_queued": cfg.only_queued, "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "rays_traced_ahead_per_frame": round(pre_rays / args.steps),
                        "frame_rays_taken_per_frame": round(pre_merged / args.steps),

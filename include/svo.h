@@ -373,7 +373,15 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * traced rays include one of at least whole_trips trips are traced whole in
  * a second launch of the set that starts as soon as the frame is issued
  * (their rays are the longest; a prediction would start them too late), and
- * the predicted holes skip them.  The reference traces each frame's rays in
+ * the predicted holes skip them.  The sets run on a pool of streams the
+ * context makes together when the set count changes, taken in index order;
+ * with queue_skip = k > 0 the pool streams at index k-1, k+3, k+7, ... are
+ * left out.  HIP gives each new stream the next of GPU_MAX_HW_QUEUES
+ * hardware queues, and on MI355X one hardware queue in four runs the
+ * pretrace far slower than the others (the flythrough at 1,860 against
+ * 2,500 frames/s for the same work, with 3 sets on the other queues): the
+ * autotuner searches queue_skip.  Default 0.  The reference traces each
+ * frame's rays in
  * the frame (raycast.h:296-387), so no counterpart. */
 typedef struct {
     int sets;         /* 0 = off, 2..8 */
@@ -384,6 +392,7 @@ typedef struct {
     int only_queued;  /* 1: pretrace a frame only while the warp frame before it is still running */
     int predict;      /* 0: long blocks whole; 1..3: holes predicted from the frame that many frames back */
     int whole_trips;  /* predict > 0: blocks traced whole at once (0 = none) */
+    int queue_skip;   /* 0: the pool's streams in order; k = 1..4: leave out those at index k-1 mod 4 */
 } svo_pretrace_config;
 int svo_set_pretrace(svo_ctx* ctx, const svo_pretrace_config* cfg);
 int svo_get_pretrace(svo_ctx* ctx, svo_pretrace_config* cfg);

@@ -16,6 +16,7 @@ static const char* const kStageNames[ST_N] = {"clear",    "proj_scatter", "proj_
                                               "copy",     "fill_colorize", "cnn",          "cnn_apply"};
 
 constexpr int kPreMaxSets = 8;  // pretrace result sets at most
+constexpr int kPrePool = 12;    // pretrace part-1 streams at most (8 sets, every fourth left out)
 constexpr int kViewSlots = 8;   // pinned staging slots of the graph frame's view record
 // instances of the warp frame's graph, launched in turn: a relaunch of one
 // instance waits on the host for its previous launch, so successive frames
@@ -107,11 +108,16 @@ struct svo_ctx {
     bool pre_only_queued = true;   // pretrace a frame only while the one before it is still running
     hipEvent_t ev_frame_done = nullptr;  // the last warp frame's end
     bool frame_done_recorded = false;
-    // set s runs part 1 (the long blocks, or the predicted holes) on
-    // pre_streams[2s+1] and, with predicted holes and pre_whole_trips > 0,
+    // set s runs part 1 (the long blocks, or the predicted holes) on a
+    // stream of pre_pool and, with predicted holes and pre_whole_trips > 0,
     // part 0 (the longest blocks whole, issued without waiting for a
-    // snapshot) on pre_streams[2s]; both write the set's results
-    hipStream_t pre_streams[2 * kPreMaxSets] = {};  // created on first use
+    // snapshot) on pre_streams[2s]; both write the set's results.  The pool
+    // streams are made together, in index order (HIP gives each new stream
+    // the next hardware queue); the sets take them in order, leaving out
+    // every index i with i % 4 == pre_qskip - 1 when pre_qskip > 0
+    hipStream_t pre_streams[2 * kPreMaxSets] = {};  // part 0, created on first use
+    hipStream_t pre_pool[kPrePool] = {};
+    int pre_qskip = 0;
     uint32_t* d_known = nullptr;      // N: loop trips of the last ray traced per pixel
     uint32_t* d_pre_stamp = nullptr;  // pre_sets x N
     uint32_t* d_pre_word = nullptr;   // pre_sets x N

@@ -1253,6 +1253,8 @@ extern "C" int svo_destroy(svo_ctx* c)
     if (c->cnn_stream) (void)hipStreamSynchronize(c->cnn_stream);
     for (auto& st : c->pre_streams)
         if (st) (void)hipStreamSynchronize(st);
+    for (auto& st : c->pre_pool)
+        if (st) (void)hipStreamSynchronize(st);
     free_frame(c);
     for (auto& e : c->image_copied)
         if (e) (void)hipEventDestroy(e);
@@ -1278,6 +1280,8 @@ extern "C" int svo_destroy(svo_ctx* c)
     for (auto& e : c->ev_main)
         if (e) (void)hipEventDestroy(e);
     for (auto& st : c->pre_streams)
+        if (st) (void)hipStreamDestroy(st);
+    for (auto& st : c->pre_pool)
         if (st) (void)hipStreamDestroy(st);
     for (auto& e : c->end_ring)
         if (e) (void)hipEventDestroy(e);
@@ -1830,9 +1834,10 @@ static int ensure_pre(svo_ctx* c)
     HIPCHECK(hipStreamSynchronize(c->stream));  // no frame reads the old sets
     free_pre(c);
     const size_t K = (size_t)c->pre_sets, n = K * c->N;
-    for (int i = 0; i < c->pre_sets; i++)  // part 1; part 0's stream is made when first used
-        if (!c->pre_streams[2 * i + 1])
-            HIPCHECK(hipStreamCreateWithFlags(&c->pre_streams[2 * i + 1], hipStreamNonBlocking));
+    // the part-1 stream pool, in index order (enough for the sets with every
+    // fourth stream left out); part 0's streams are made when first used
+    for (int i = 0; i < std::min(kPrePool, c->pre_sets + (c->pre_sets + 2) / 3); i++)
+        if (!c->pre_pool[i]) HIPCHECK(hipStreamCreateWithFlags(&c->pre_pool[i], hipStreamNonBlocking));
     if (hipMalloc(&c->d_pre_stamp, n * 4) != hipSuccess || hipMalloc(&c->d_pre_word, n * 4) != hipSuccess ||
         hipMalloc(&c->d_pre_xyz, n * 16) != hipSuccess ||
         hipMalloc(&c->d_pre_list, 2 * K * (c->N + kPreHdr) * 4) != hipSuccess ||
@@ -2205,6 +2210,18 @@ static bool predict_ready(const svo_ctx* c, const svo_frame_params* p)
     return true;
 }
 
+// the pool stream of set s's part 1: the s-th index left after every i with
+// i % 4 == pre_qskip - 1 is taken out (pre_qskip > 0)
+static int pool_index(const svo_ctx* c, int s)
+{
+    int i = 0;
+    for (int k = 0;; i++) {
+        if (c->pre_qskip > 0 && i % 4 == c->pre_qskip - 1) continue;
+        if (k++ == s) break;
+    }
+    return i;
+}
+
 // one part of set s's pretrace: list passes (long blocks, the longest
 // blocks only, or predicted holes), then the ray launch into the set's
 // results, stamped `id`; records ev_pre[q]
@@ -2214,8 +2231,9 @@ static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part,
 {
     const int q = 2 * s + part;
     const size_t N = c->N;
-    if (!c->pre_streams[q]) HIPCHECK(hipStreamCreateWithFlags(&c->pre_streams[q], hipStreamNonBlocking));
-    hipStream_t st = c->pre_streams[q];
+    hipStream_t* sp = part ? &c->pre_pool[pool_index(c, s)] : &c->pre_streams[q];  // the set's stream
+    if (!*sp) HIPCHECK(hipStreamCreateWithFlags(sp, hipStreamNonBlocking));
+    hipStream_t st = *sp;
     if (c->timing == 2 && part == 1) (void)hipEventRecord(c->ev[0], st);  // the frame starts with its pretrace
     if (c->main_recorded[s]) HIPCHECK(hipStreamWaitEvent(st, c->ev_main[s], 0));
     int ax, ay, ex, ey;
@@ -2533,7 +2551,14 @@ extern "C" int svo_set_pretrace(svo_ctx* c, const svo_pretrace_config* cfg)
         return svo_fail(SVO_ERR_ARG, "predict %d outside 0..%d", cfg->predict, kSnapSlots - 1);
     if (cfg->whole_trips < 0 || cfg->whole_trips > SVO_MAX_RAY_ITERS)
         return svo_fail(SVO_ERR_ARG, "whole_trips %d outside 0..%d", cfg->whole_trips, (int)SVO_MAX_RAY_ITERS);
+    if (cfg->queue_skip < 0 || cfg->queue_skip > 4)
+        return svo_fail(SVO_ERR_ARG, "queue_skip %d outside 0..4", cfg->queue_skip);
     HIPCHECK(hipSetDevice(c->device));
+    if (cfg->queue_skip != c->pre_qskip) {
+        // the sets move to other streams: nothing may still run on the old ones
+        HIPCHECK(hipDeviceSynchronize());
+        c->pre_qskip = cfg->queue_skip;
+    }
     if (sets != c->pre_sets) {
         HIPCHECK(hipStreamSynchronize(c->stream));
         free_pre(c);
@@ -2560,6 +2585,7 @@ extern "C" int svo_get_pretrace(svo_ctx* c, svo_pretrace_config* cfg)
     cfg->only_queued = c->pre_only_queued ? 1 : 0;
     cfg->predict = c->pre_predict;
     cfg->whole_trips = (int)c->pre_whole_trips;
+    cfg->queue_skip = c->pre_qskip;
     return SVO_OK;
 }
 

@@ -182,7 +182,8 @@ EXPORTED = [
 class PretraceConfig(C.Structure):
     """svo_pretrace_config"""
     _fields_ = [("sets", C.c_int), ("long_trips", C.c_int), ("wg_per_cu", C.c_int), ("split_trips", C.c_int),
-                ("split_lanes", C.c_int), ("only_queued", C.c_int), ("predict", C.c_int), ("whole_trips", C.c_int)]
+                ("split_lanes", C.c_int), ("only_queued", C.c_int), ("predict", C.c_int), ("whole_trips", C.c_int),
+                ("queue_skip", C.c_int)]
 
 
 def _take_u32(p, n):
@@ -527,17 +528,18 @@ class Context:
         return v.value
 
     def set_pretrace(self, sets=None, long_trips=None, wg_per_cu=None, split_trips=None, split_lanes=None,
-                     only_queued=None, predict=None, whole_trips=None):
+                     only_queued=None, predict=None, whole_trips=None, queue_skip=None):
         """Trace each warp frame's refresh rectangle and long-ray blocks
         (predict = 0) or predicted hole quads (predict = k: from the frame k
         frames back; blocks with a known ray of at least whole_trips trips
-        whole, at once) ahead on streams of their own (sets = 0: off);
+        whole, at once) ahead on streams of their own (sets = 0: off;
+        queue_skip leaves every fourth stream of its pool out);
         arguments left None keep their current values.  Results are the
         same."""
         cfg = self.get_pretrace()
         for k, v in (("sets", sets), ("long_trips", long_trips), ("wg_per_cu", wg_per_cu),
                      ("split_trips", split_trips), ("split_lanes", split_lanes), ("only_queued", only_queued),
-                     ("predict", predict), ("whole_trips", whole_trips)):
+                     ("predict", predict), ("whole_trips", whole_trips), ("queue_skip", queue_skip)):
             if v is not None:
                 setattr(cfg, k, int(v))
         _check(_f["set_pretrace"](self._h, C.byref(cfg)))

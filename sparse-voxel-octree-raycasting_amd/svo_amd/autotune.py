@@ -85,13 +85,15 @@ def _search(space, coordinate, measure):
 
 
 # warp frames also search the pretrace: its sets, its mode (pre_pred: 0 =
-# long blocks, k = holes predicted from frame g-k) and block threshold
-WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384])
+# long blocks, k = holes predicted from frame g-k), block threshold and the
+# hardware queues it runs on (pre_qskip: every fourth queue of its pool left out)
+WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384],
+                  pre_qskip=[1, 0, 2, 3, 4])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
 LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[4, 3, 5],
-                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384])
+                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 0, 2, 3, 4])
 
 
 def apply(ctx, cfg):
@@ -107,8 +109,9 @@ def apply(ctx, cfg):
         ctx.set_ray_block(int(cfg["ray_block"]))
     if "cnn_wgs" in cfg:
         ctx.set_cnn_overlap(int(cfg["cnn_wgs"]))
-    if "pre_sets" in cfg or "pre_long" in cfg or "pre_pred" in cfg:
-        ctx.set_pretrace(sets=cfg.get("pre_sets"), long_trips=cfg.get("pre_long"), predict=cfg.get("pre_pred"))
+    if "pre_sets" in cfg or "pre_long" in cfg or "pre_pred" in cfg or "pre_qskip" in cfg:
+        ctx.set_pretrace(sets=cfg.get("pre_sets"), long_trips=cfg.get("pre_long"), predict=cfg.get("pre_pred"),
+                         queue_skip=cfg.get("pre_qskip"))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None, cnn=None, max_cnn_holes=64):

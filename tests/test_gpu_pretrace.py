@@ -247,15 +247,16 @@ def test_pretrace_rejects_bad_settings(scene1):
     with S.Context(0) as c:
         for args in [(1, 512), (9, 512), (-1, 512), (3, 0), (3, (1 << 22) + 1), (3, 512, 33), (3, 512, -1),
                      (3, 512, 0, -1), (3, 512, 0, 0, 3), (3, 512, 0, 0, 0), (3, 512, 0, 0, 128),
-                     (3, 512, 0, 0, 8, 1, -1), (3, 512, 0, 0, 8, 1, 4), (3, 512, 0, 0, 8, 1, 2, -1)]:
+                     (3, 512, 0, 0, 8, 1, -1), (3, 512, 0, 0, 8, 1, 4), (3, 512, 0, 0, 8, 1, 2, -1),
+                     (3, 512, 0, 0, 8, 1, 2, 0, -1), (3, 512, 0, 0, 8, 1, 2, 0, 5)]:
             with pytest.raises(S.SvoError):
                 c.set_pretrace(*args)
 
 
-@pytest.mark.parametrize("predict,long_trips,graph,sets,whole", [
-    (1, 1, True, 3, 0), (2, 1, False, 4, 0), (2, 64, True, 6, 0), (3, 1, True, 8, 0), (3, 128, False, 2, 0),
-    (2, 1, True, 4, 256), (3, 1, False, 6, 64), (2, 1, True, 2, 1)])
-def test_pretrace_predicted_holes_equal_plain_frames(scene1, predict, long_trips, graph, sets, whole):
+@pytest.mark.parametrize("predict,long_trips,graph,sets,whole,qskip", [
+    (1, 1, True, 3, 0, 0), (2, 1, False, 4, 0, 2), (2, 64, True, 6, 0, 4), (3, 1, True, 8, 0, 3),
+    (3, 128, False, 2, 0, 0), (2, 1, True, 4, 256, 1), (3, 1, False, 6, 64, 0), (2, 1, True, 2, 1, 4)])
+def test_pretrace_predicted_holes_equal_plain_frames(scene1, predict, long_trips, graph, sets, whole, qskip):
     """Predicted-hole pretrace (frame g's hole quads from frame g-k's final
     plane 0 through the cameras in between; with whole > 0 the blocks of long
     known rays traced whole by a second launch): frames equal a context
@@ -272,13 +273,16 @@ def test_pretrace_predicted_holes_equal_plain_frames(scene1, predict, long_trips
             c.upload_octree(words, root, CONFIG1["depth"])
             c.resize(rx, ry)
             c.set_graph(graph)
-        a.set_pretrace(sets, long_trips, only_queued=0, predict=predict, whole_trips=whole)
-        assert (a.get_pretrace().predict, a.get_pretrace().whole_trips) == (predict, whole)
+        a.set_pretrace(sets, long_trips, only_queued=0, predict=predict, whole_trips=whole, queue_skip=qskip)
+        got = a.get_pretrace()
+        assert (got.predict, got.whole_trips, got.queue_skip) == (predict, whole, qskip)
         b.set_pretrace(0)
         in_frame, holes = 0, 0
         for f in range(n):
             print(f"predicted pretrace k={predict} long={long_trips} graph={graph} sets={sets} whole={whole} "
-                  f"frame {f}", flush=True)
+                  f"qskip={qskip} frame {f}", flush=True)
+            if f == n // 2:  # the sets move to other streams mid-path
+                a.set_pretrace(queue_skip=(qskip + 1) % 5)
             p = params(CONFIG1, pr, f)
             a.ray_stats(reset=True)
             a.pretrace_stats(reset=True)

@@ -392,6 +392,15 @@ def run_stress(args, name="cave_4096", n_timed=190):
     for c in (warp, full):
         c.upload_octree(words, root, depth)
         c.resize(rx, ry)
+    tuned = None
+    if args.tune and args.pretrace:
+        # the pretrace's sets and hardware queues only, on a held-out path
+        from svo_amd import autotune as A
+        trows = W.path(name, 50, seed_off=TUNE_SEED_OFF)
+        tp = [W.frame_params(name, trows, f) for f in range(50)]
+        best, ms, _ = A.tune(warp, tp, warmup=10, mode="warp", space={"pre_sets": [4, 3], "pre_qskip": [1, 0, 2, 3, 4]})
+        tuned = dict(best, ms=round(ms, 4))
+        log(f"[bench] stress autotune: {tuned}")
     for f in range(10):
         warp.frame_warp(params[f])
     warp.sync()
@@ -439,7 +448,7 @@ def run_stress(args, name="cave_4096", n_timed=190):
             "hole_list_fraction_mean": round(float(np.mean(holes)), 4),
             "hole_list_fraction_max": round(float(np.max(holes)), 4),
             "rays_per_warp_frame": round(rays / n_timed), "rays_traced_ahead_per_frame": round(pre_rays / n_timed),
-            "longest_ray_trips": int(longest)}
+            "longest_ray_trips": int(longest), "tuned_pretrace": tuned}
 
 
 def run_batch(args):

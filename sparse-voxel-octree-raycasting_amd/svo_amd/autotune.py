@@ -98,7 +98,8 @@ LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_s
 
 def apply(ctx, cfg):
     """Apply a knob setting (as returned by tune / tune_batch) to a context."""
-    ctx.set_tuning(cfg["refill"], cfg["wg_per_cu"], cfg["iters"], cfg["lds_cache_words"])
+    # knobs missing from cfg keep their current settings (-1)
+    ctx.set_tuning(cfg.get("refill", -1), cfg.get("wg_per_cu", -1), cfg.get("iters", -1), cfg.get("lds_cache_words", -1))
     if "spread" in cfg:
         ctx.set_ray_spread(int(cfg["spread"]), True)
     if "lanes" in cfg:

@@ -40,6 +40,24 @@ def test_warp_space_adds_pretrace():
     assert all(k in A.WARP_SPACE for k in A.DEFAULT_SPACE)
 
 
+class _Recorder:
+    def __init__(self):
+        self.calls = []
+
+    def __getattr__(self, name):
+        return lambda *a, **k: self.calls.append((name, a, k))
+
+
+def test_apply_a_partial_setting():
+    """A setting of some knobs only (the divergence stress tunes the
+    pretrace alone) keeps the ray-kernel knobs it does not name."""
+    r = _Recorder()
+    A.apply(r, {"pre_sets": 3, "pre_qskip": 2})
+    assert ("set_tuning", (-1, -1, -1, -1), {}) in r.calls
+    pre = [c for c in r.calls if c[0] == "set_pretrace"]
+    assert len(pre) == 1 and pre[0][2]["sets"] == 3 and pre[0][2]["queue_skip"] == 2 and pre[0][2]["predict"] is None
+
+
 def test_learned_space_adds_cnn_workgroups():
     assert "cnn_wgs" in A.LEARNED_SPACE and "cnn_wgs" not in A.DEFAULT_SPACE
     assert all(k in A.LEARNED_SPACE for k in A.DEFAULT_SPACE)

@@ -344,8 +344,8 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * a frame is pretraced only while the warp frame before it is still on the
  * GPU (the caller issues frames ahead): a frame the GPU starts at once has
  * no earlier frame to overlap, and its own rays finish sooner without one.
- * Defaults: 4 sets, long_trips 1, 1 workgroup per CU, no split,
- * only_queued 1, predict 2, whole_trips 0.  A primary ray depends
+ * Defaults: 3 sets, long_trips 1, 1 workgroup per CU, no split,
+ * only_queued 1, predict 2, whole_trips 0, queue_skip 2.  A primary ray depends
  * only on the camera, the pixel and the octree (kernel.cl:594-694,
  * :846-942): results do not depend on these settings.  HIP maps streams onto
  * at most GPU_MAX_HW_QUEUES hardware queues (4 unless the environment sets
@@ -380,7 +380,8 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * hardware queues, and on MI355X one hardware queue in four runs the
  * pretrace far slower than the others (the flythrough at 1,860 against
  * 2,500 frames/s for the same work, with 3 sets on the other queues): the
- * autotuner searches queue_skip.  Default 0.  The reference traces each
+ * autotuner searches queue_skip.  The default, 2, suits a process whose
+ * first streams are this context's own.  The reference traces each
  * frame's rays in
  * the frame (raycast.h:296-387), so no counterpart. */
 typedef struct {

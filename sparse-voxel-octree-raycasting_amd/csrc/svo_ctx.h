@@ -100,7 +100,7 @@ struct svo_ctx {
     // into one of pre_sets result sets (set s on pre_streams[s]) while
     // earlier frames still run; the frame's own ray launch copies those of
     // its rays
-    int pre_sets = 4;              // 0 = off
+    int pre_sets = 3;              // 0 = off
     uint32_t pre_trips = 1;
     int pre_wg = 1;                // workgroups per CU of the pretrace launch (0: wg_per_cu)
     int pre_split_trips = 0;       // blocks with a known ray this long go to the split waves (0: no split)
@@ -117,7 +117,7 @@ struct svo_ctx {
     // every index i with i % 4 == pre_qskip - 1 when pre_qskip > 0
     hipStream_t pre_streams[2 * kPreMaxSets] = {};  // part 0, created on first use
     hipStream_t pre_pool[kPrePool] = {};
-    int pre_qskip = 0;
+    int pre_qskip = 2;  // a context whose two streams are the process's first: its pool starts at queue 2
     uint32_t* d_known = nullptr;      // N: loop trips of the last ray traced per pixel
     uint32_t* d_pre_stamp = nullptr;  // pre_sets x N
     uint32_t* d_pre_word = nullptr;   // pre_sets x N

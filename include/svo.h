@@ -345,7 +345,7 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * GPU (the caller issues frames ahead): a frame the GPU starts at once has
  * no earlier frame to overlap, and its own rays finish sooner without one.
  * Defaults: 3 sets, long_trips 1, 1 workgroup per CU, no split,
- * only_queued 1, predict 2, whole_trips 0, queue_skip 2.  A primary ray depends
+ * only_queued 1, predict 2, whole_trips 0, queue_skip 3.  A primary ray depends
  * only on the camera, the pixel and the octree (kernel.cl:594-694,
  * :846-942): results do not depend on these settings.  HIP maps streams onto
  * at most GPU_MAX_HW_QUEUES hardware queues (4 unless the environment sets
@@ -373,15 +373,20 @@ int svo_set_long_first(svo_ctx* ctx, int long_trips);
  * traced rays include one of at least whole_trips trips are traced whole in
  * a second launch of the set that starts as soon as the frame is issued
  * (their rays are the longest; a prediction would start them too late), and
- * the predicted holes skip them.  The sets run on a pool of streams the
- * context makes together when the set count changes, taken in index order;
+ * the predicted holes skip them.  The sets run on a pool of streams, taken
+ * in index order: the context makes the pool for its default set count
+ * right after its own two streams in svo_create, and more streams when the
+ * set count grows;
  * with queue_skip = k > 0 the pool streams at index k-1, k+3, k+7, ... are
  * left out.  HIP gives each new stream the next of GPU_MAX_HW_QUEUES
  * hardware queues, and on MI355X one hardware queue in four runs the
  * pretrace far slower than the others (the flythrough at 1,860 against
  * 2,500 frames/s for the same work, with 3 sets on the other queues): the
- * autotuner searches queue_skip.  The default, 2, suits a process whose
- * first streams are this context's own.  The reference traces each
+ * autotuner searches queue_skip.  The slow queues have been those four,
+ * eight, ... after the main stream's (the main stream's pipe, with queues
+ * dealt to four pipes in turn); with the pool right behind the context's
+ * two streams that is pool index 2, 6, 10, which the default, 3, leaves
+ * out.  The reference traces each
  * frame's rays in
  * the frame (raycast.h:296-387), so no counterpart. */
 typedef struct {

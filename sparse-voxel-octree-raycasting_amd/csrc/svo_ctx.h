@@ -112,12 +112,13 @@ struct svo_ctx {
     // stream of pre_pool and, with predicted holes and pre_whole_trips > 0,
     // part 0 (the longest blocks whole, issued without waiting for a
     // snapshot) on pre_streams[2s]; both write the set's results.  The pool
-    // streams are made together, in index order (HIP gives each new stream
-    // the next hardware queue); the sets take them in order, leaving out
-    // every index i with i % 4 == pre_qskip - 1 when pre_qskip > 0
+    // streams are made in index order, the default count right after the
+    // context's own two streams (HIP gives each new stream the next hardware
+    // queue); the sets take them in order, leaving out every index i with
+    // i % 4 == pre_qskip - 1 when pre_qskip > 0
     hipStream_t pre_streams[2 * kPreMaxSets] = {};  // part 0, created on first use
     hipStream_t pre_pool[kPrePool] = {};
-    int pre_qskip = 2;  // a context whose two streams are the process's first: its pool starts at queue 2
+    int pre_qskip = 3;  // pool index 2 is four queues after the main stream's: its pipe
     uint32_t* d_known = nullptr;      // N: loop trips of the last ray traced per pixel
     uint32_t* d_pre_stamp = nullptr;  // pre_sets x N
     uint32_t* d_pre_word = nullptr;   // pre_sets x N

@@ -1210,6 +1210,9 @@ static void drop_graph(svo_ctx* c)
     for (auto& g : c->graph) g = nullptr;
 }
 
+// pretrace pool streams for `sets` sets with every fourth stream left out
+static int pool_size(int sets) { return std::min(kPrePool, sets + (sets + 2) / 3); }
+
 extern "C" int svo_create(svo_ctx** out, int device)
 {
     if (!out) return svo_fail(SVO_ERR_ARG, "null context pointer");
@@ -1225,8 +1228,12 @@ extern "C" int svo_create(svo_ctx** out, int device)
         c->n_cu = prop.multiProcessorCount;
     c->cnn_overlap_wgs = std::max(1, c->n_cu / 2);
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipStreamCreateWithFlags(&c->cnn_stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&c->cnn_fork, hipEventDisableTiming) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->cnn_stream, hipStreamNonBlocking) == hipSuccess;
+    // the default pretrace's stream pool right behind the context's own two
+    // streams, so the queue the default queue_skip leaves out is the slow one
+    for (int i = 0; ok && i < pool_size(c->pre_sets); i++)
+        ok = hipStreamCreateWithFlags(&c->pre_pool[i], hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&c->cnn_fork, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->cnn_join, hipEventDisableTiming) == hipSuccess &&
               hipMalloc(&c->d_queue, 128) == hipSuccess && hipMemset(c->d_queue, 0, 128) == hipSuccess;
     for (int i = 0; ok && i <= ST_N; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
@@ -1836,7 +1843,7 @@ static int ensure_pre(svo_ctx* c)
     const size_t K = (size_t)c->pre_sets, n = K * c->N;
     // the part-1 stream pool, in index order (enough for the sets with every
     // fourth stream left out); part 0's streams are made when first used
-    for (int i = 0; i < std::min(kPrePool, c->pre_sets + (c->pre_sets + 2) / 3); i++)
+    for (int i = 0; i < pool_size(c->pre_sets); i++)
         if (!c->pre_pool[i]) HIPCHECK(hipStreamCreateWithFlags(&c->pre_pool[i], hipStreamNonBlocking));
     if (hipMalloc(&c->d_pre_stamp, n * 4) != hipSuccess || hipMalloc(&c->d_pre_word, n * 4) != hipSuccess ||
         hipMalloc(&c->d_pre_xyz, n * 16) != hipSuccess ||

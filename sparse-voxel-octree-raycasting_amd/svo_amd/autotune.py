@@ -88,12 +88,12 @@ def _search(space, coordinate, measure):
 # long blocks, k = holes predicted from frame g-k), block threshold and the
 # hardware queues it runs on (pre_qskip: every fourth queue of its pool left out)
 WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384],
-                  pre_qskip=[1, 2, 0, 3, 4])
+                  pre_qskip=[1, 3, 0, 2, 4])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
 LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[4, 3, 5],
-                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 2, 0, 3, 4])
+                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 3, 0, 2, 4])
 
 
 def apply(ctx, cfg):

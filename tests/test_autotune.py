@@ -36,7 +36,7 @@ def test_full_space_adds_long_first():
 def test_warp_space_adds_pretrace():
     assert "pre_sets" in A.WARP_SPACE and "pre_long" in A.WARP_SPACE and "pre_sets" not in A.DEFAULT_SPACE
     assert 0 in A.WARP_SPACE["pre_pred"] and max(A.WARP_SPACE["pre_pred"]) <= 3
-    assert all(0 <= q <= 4 for q in A.WARP_SPACE["pre_qskip"]) and A.WARP_SPACE["pre_qskip"][1] == 2
+    assert all(0 <= q <= 4 for q in A.WARP_SPACE["pre_qskip"]) and A.WARP_SPACE["pre_qskip"][1] == 3
     assert all(k in A.WARP_SPACE for k in A.DEFAULT_SPACE)
 
 

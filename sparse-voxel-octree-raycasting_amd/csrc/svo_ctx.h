@@ -97,7 +97,7 @@ struct svo_ctx {
     bool long_valid = false;
     // pretrace of warp frames (svo_set_pretrace): the refresh rectangle and
     // the 8x8 blocks whose last rays ran at least pre_trips trips are traced
-    // into one of pre_sets result sets (set s on pre_streams[s]) while
+    // into one of pre_sets result sets (each on a stream of its own) while
     // earlier frames still run; the frame's own ray launch copies those of
     // its rays
     int pre_sets = 3;              // 0 = off

@@ -85,15 +85,16 @@ def _search(space, coordinate, measure):
 
 
 # warp frames also search the pretrace: its sets, its mode (pre_pred: 0 =
-# long blocks, k = holes predicted from frame g-k), block threshold and the
-# hardware queues it runs on (pre_qskip: every fourth queue of its pool left out)
+# long blocks, k = holes predicted from frame g-k), block threshold, the
+# hardware queues it runs on (pre_qskip: every fourth queue of its pool left
+# out) and its launch's workgroups per CU (pre_wg)
 WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384],
-                  pre_qskip=[1, 3, 0, 2, 4])
+                  pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
 LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[4, 3, 5],
-                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 3, 0, 2, 4])
+                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1])
 
 
 def apply(ctx, cfg):
@@ -110,9 +111,9 @@ def apply(ctx, cfg):
         ctx.set_ray_block(int(cfg["ray_block"]))
     if "cnn_wgs" in cfg:
         ctx.set_cnn_overlap(int(cfg["cnn_wgs"]))
-    if "pre_sets" in cfg or "pre_long" in cfg or "pre_pred" in cfg or "pre_qskip" in cfg:
+    if any(k in cfg for k in ("pre_sets", "pre_long", "pre_pred", "pre_qskip", "pre_wg")):
         ctx.set_pretrace(sets=cfg.get("pre_sets"), long_trips=cfg.get("pre_long"), predict=cfg.get("pre_pred"),
-                         queue_skip=cfg.get("pre_qskip"))
+                         queue_skip=cfg.get("pre_qskip"), wg_per_cu=cfg.get("pre_wg"))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None, cnn=None, max_cnn_holes=64):

@@ -56,6 +56,7 @@ def test_apply_a_partial_setting():
     assert ("set_tuning", (-1, -1, -1, -1), {}) in r.calls
     pre = [c for c in r.calls if c[0] == "set_pretrace"]
     assert len(pre) == 1 and pre[0][2]["sets"] == 3 and pre[0][2]["queue_skip"] == 2 and pre[0][2]["predict"] is None
+    assert pre[0][2]["wg_per_cu"] is None
 
 
 def test_learned_space_adds_cnn_workgroups():

@@ -87,14 +87,18 @@ def _search(space, coordinate, measure):
 # warp frames also search the pretrace: its sets, its mode (pre_pred: 0 =
 # long blocks, k = holes predicted from frame g-k), block threshold, the
 # hardware queues it runs on (pre_qskip: every fourth queue of its pool left
-# out) and its launch's workgroups per CU (pre_wg)
+# out), its launch's workgroups per CU (pre_wg) and the waves of their own
+# that its longest rays get (pre_split: known trips at least this many,
+# pre_split_lanes rays per wave)
 WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384],
-                  pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1])
+                  pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1],
+                  pre_split=[1024, 2048, 0, 3072], pre_split_lanes=[4, 1])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
 LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[4, 3, 5],
-                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1])
+                     pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1],
+                     pre_split=[1024, 2048, 0, 3072], pre_split_lanes=[4, 1])
 
 
 def apply(ctx, cfg):
@@ -111,9 +115,11 @@ def apply(ctx, cfg):
         ctx.set_ray_block(int(cfg["ray_block"]))
     if "cnn_wgs" in cfg:
         ctx.set_cnn_overlap(int(cfg["cnn_wgs"]))
-    if any(k in cfg for k in ("pre_sets", "pre_long", "pre_pred", "pre_qskip", "pre_wg")):
+    if any(k in cfg for k in ("pre_sets", "pre_long", "pre_pred", "pre_qskip", "pre_wg", "pre_split",
+                              "pre_split_lanes")):
         ctx.set_pretrace(sets=cfg.get("pre_sets"), long_trips=cfg.get("pre_long"), predict=cfg.get("pre_pred"),
-                         queue_skip=cfg.get("pre_qskip"), wg_per_cu=cfg.get("pre_wg"))
+                         queue_skip=cfg.get("pre_qskip"), wg_per_cu=cfg.get("pre_wg"),
+                         split_trips=cfg.get("pre_split"), split_lanes=cfg.get("pre_split_lanes"))
 
 
 def tune(ctx, params, warmup=4, mode="warp", space=None, coordinate=True, log=None, cnn=None, max_cnn_holes=64):

@@ -394,13 +394,16 @@ def run_stress(args, name="cave_4096", n_timed=190):
         c.resize(rx, ry)
     tuned = None
     if args.tune and args.pretrace:
-        # the pretrace's sets, hardware queues and split hand-out only, on a
-        # held-out path (the cave's longest rays run ~770 trips)
+        # the pretrace only (off, its sets, mode, hardware queues and split
+        # hand-out), on a held-out path: the cave's holes are a fifth of the
+        # screen and its longest rays run ~770 trips, so tracing ahead may
+        # not pay here
         from svo_amd import autotune as A
         trows = W.path(name, 50, seed_off=TUNE_SEED_OFF)
         tp = [W.frame_params(name, trows, f) for f in range(50)]
         best, ms, _ = A.tune(warp, tp, warmup=10, mode="warp",
-                             space={"pre_sets": [4, 3], "pre_qskip": [1, 3, 0, 2, 4], "pre_split": [256, 0, 512]})
+                             space={"pre_sets": [4, 3, 0], "pre_pred": [0, 2], "pre_qskip": [1, 3, 0, 2, 4],
+                                    "pre_split": [256, 0, 512]})
         tuned = dict(best, ms=round(ms, 4))
         log(f"[bench] stress autotune: {tuned}")
     for f in range(10):

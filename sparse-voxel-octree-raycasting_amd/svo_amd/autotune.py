@@ -90,13 +90,13 @@ def _search(space, coordinate, measure):
 # out), its launch's workgroups per CU (pre_wg) and the waves of their own
 # that its longest rays get (pre_split: known trips at least this many,
 # pre_split_lanes rays per wave)
-WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384],
+WARP_SPACE = dict(DEFAULT_SPACE, pre_sets=[4, 3, 5, 0], pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384],
                   pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1],
                   pre_split=[1024, 2048, 0], pre_split_lanes=[4, 1])
 # full frames also search the long-ray-first threshold (svo_set_long_first)
 FULL_SPACE = dict(DEFAULT_SPACE, long_trips=[0, 1024, 512, 2048])
 # learned frames also search the CNN's workgroups alongside the ray launch
-LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[4, 3, 5],
+LEARNED_SPACE = dict(DEFAULT_SPACE, cnn_wgs=[256, 128, 96, 160, 192, 512], pre_sets=[4, 3, 5, 0],
                      pre_pred=[0, 2, 3], pre_long=[256, 1, 128, 384], pre_qskip=[1, 3, 0, 2, 4], pre_wg=[2, 1],
                      pre_split=[1024, 2048, 0], pre_split_lanes=[4, 1])
 

@@ -307,13 +307,16 @@ def test_pretrace_predicted_holes_equal_plain_frames(scene1, predict, long_trips
         b.close()
 
 
-def test_pretrace_predicted_holes_flythrough_vs_oracle():
-    """The headline workload with predicted holes (k = 2, blocks of rays
-    of 1024 trips and more whole): frames 0..29
-    against the oracle (plane 0, ids, image); the frame's own launch traces
-    under 5% of the hole rays from frame 4 on.  A jump in the frame counter
-    (no snapshot of frame g-2) falls back to the long blocks and still
-    equals the oracle."""
+@pytest.mark.parametrize("sets,whole,split,lanes", [(4, 1024, 0, 8), (3, 0, 2048, 1)])
+def test_pretrace_predicted_holes_flythrough_vs_oracle(sets, whole, split, lanes):
+    """The headline workload with predicted holes (k = 2; with whole > 0
+    the blocks of rays of that many trips and more whole; with split > 0
+    the rays of at least that many known trips in waves of their own,
+    `lanes` per wave -- the bench's tuned setting): frames 0..29 against the
+    oracle (plane 0, ids, image); the frame's own launch traces under 5% of
+    the hole rays from frame 4 on.  A jump in the frame counter (no snapshot
+    of frame g-2) falls back to the long blocks and still equals the
+    oracle."""
     words, root, depth, _ = W.octree(FLY)
     rx, ry = W.WORKLOADS[FLY]["res"]
     rows = W.path(FLY, 30)
@@ -323,12 +326,14 @@ def test_pretrace_predicted_holes_flythrough_vs_oracle():
         with S.Context(0) as c:
             c.upload_octree(words, root, depth)
             c.resize(rx, ry)
-            c.set_pretrace(4, 1, only_queued=0, predict=2, whole_trips=1024)
+            c.set_pretrace(sets, 1, split_trips=split, split_lanes=lanes, only_queued=0, predict=2,
+                           whole_trips=whole)
             in_frame, holes = 0, 0
             for f in range(30):
                 fi = f if f < 20 else f + 100  # frame counters 120.. after frame 19
                 p = W.frame_params(FLY, rows, f, frame_index=fi)
-                print(f"predicted pretrace flythrough frame {f} (counter {fi})", flush=True)
+                print(f"predicted pretrace flythrough sets={sets} whole={whole} split={split}/{lanes} frame {f} "
+                      f"(counter {fi})", flush=True)
                 c.ray_stats(reset=True)
                 c.pretrace_stats(reset=True)
                 c.frame_warp(p)

@@ -2051,9 +2051,13 @@ __device__ __forceinline__ int pred_target(float4 q, const PredCam& k, int res_x
 }
 
 // nearest sample per target pixel of an intermediate camera: key = depth << 32 | source
+// (hdr != null: block 0 also clears the pretrace list header, which the
+// list passes after this kernel fill -- one launch fewer in the chain)
 __global__ __launch_bounds__(256) void k_pred_scatter(const float4* __restrict__ src, PredCam k, int res_x, int res_y,
-                                                      uint32_t N, unsigned long long* __restrict__ keys)
+                                                      uint32_t N, unsigned long long* __restrict__ keys,
+                                                      uint32_t* __restrict__ hdr)
 {
+    if (hdr && blockIdx.x == 0 && threadIdx.x < (unsigned)kPreHdr) hdr[threadIdx.x] = 0u;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
     float hz;
@@ -2103,8 +2107,9 @@ __global__ __launch_bounds__(256) void k_pred_resolve(unsigned long long* __rest
 // every sample of src marked under the final camera
 __global__ __launch_bounds__(256) void k_pred_cover(const float4* __restrict__ src, PredCam fin, int res_x, int res_y,
                                                     uint32_t N, float deep_z, uint8_t* __restrict__ cov,
-                                                    uint8_t* __restrict__ deep)
+                                                    uint8_t* __restrict__ deep, uint32_t* __restrict__ hdr)
 {
+    if (hdr && blockIdx.x == 0 && threadIdx.x < (unsigned)kPreHdr) hdr[threadIdx.x] = 0u;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
     pred_mark(src[i], fin, res_x, res_y, deep_z, cov, deep);
@@ -2253,8 +2258,10 @@ static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part,
     uint32_t* hdr = c->d_pre_list + (size_t)q * (N + kPreHdr);
     uint32_t* list = hdr + kPreHdr;
     uint32_t* blk = c->d_pre_blk + (size_t)q * pre_blocks(c);
-    hipLaunchKernelGGL(k_memset, dim3(1), dim3(256), 0, st, hdr, 0u, (size_t)kPreHdr);
-    HIPCHECK(hipGetLastError());
+    if (kind != PRE_PREDICT) {  // the prediction's first kernel clears the header itself
+        hipLaunchKernelGGL(k_memset, dim3(1), dim3(256), 0, st, hdr, 0u, (size_t)kPreHdr);
+        HIPCHECK(hipGetLastError());
+    }
     const dim3 grid((unsigned)((c->res_x + 63) / 64), (unsigned)(((c->res_y + 7) / 8 + 3) / 4));
     uint8_t* bkt = c->d_pre_bkt + (size_t)q * N;
     if (kind == PRE_PREDICT) {
@@ -2276,12 +2283,12 @@ static int pretrace_part(svo_ctx* c, const svo_frame_params* p, int s, int part,
         const PredCam fin = pred_cam(p);
         if (k == 1) {
             hipLaunchKernelGGL(k_pred_cover, dim3(nb), dim3(256), 0, st, cur, fin, c->res_x, c->res_y, (uint32_t)N,
-                               deep_z, cov, deep);
+                               deep_z, cov, deep, hdr);
         }
         for (int j = 1; j < k; j++) {
             const PredCam mid = pred_cam(&c->cam_hist[(g - k + j) % kSnapSlots]);
             hipLaunchKernelGGL(k_pred_scatter, dim3(nb), dim3(256), 0, st, cur, mid, c->res_x, c->res_y,
-                               (uint32_t)N, keys);
+                               (uint32_t)N, keys, j == 1 ? hdr : nullptr);
             const bool last = j == k - 1;
             // (k == 3: the intermediate positions go to `inter`, which the
             // next scatter reads; the first scatter read the snapshot)
